@@ -1,0 +1,149 @@
+/* Beam geometry and X-ray tube model of the MI355X transport engine
+ * (part of libdxmc_hip.so).  These functions turn the settings of DXMClib's
+ * beam classes -- as OpenDXMC sets them in beamsettingsmodel.cpp and stores
+ * them in hdf5wrapper.cpp:548-585 -- into the exposure list and spectra that
+ * dxg_transport_run() consumes (include/dxmc_hip.h).  All lengths in cm,
+ * angles in radians, energies in keV.  Status codes as in dxmc_hip.h.
+ *
+ * Geometry convention (DXMClib's own is not in the reference tree):
+ *   CT and CBCT sources turn about a rotation axis n through an isocentre c at
+ *   radius R = SDD / 2.  At gantry angle a the beam direction is
+ *   b(a) = e1 cos a + e2 sin a with e1 = normalise(y x n) (x x n if n || y),
+ *   e2 = n x e1; the source sits at c - R b(a); the exposure cosines are
+ *   c0 = n x b(a) and c1 = n, so that c0 x c1 = b(a) (beamactorcontainer.cpp:52).
+ *   The fan half angle is asin(FOV / SDD) (the fan touches the scan field of
+ *   view circle) and the slice half angle atan(collimation / SDD).
+ */
+#ifndef DXMC_BEAMS_H
+#define DXMC_BEAMS_H
+
+#include "dxmc_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Tube(voltage, anodeAngle) with addTubeFiltrationMaterial(Z, mm)
+ * (beamsettingsmodel.cpp:257-347).  Model: Kramers thick-target
+ * bremsstrahlung (kVp - E)/E photons per keV, anode self-absorption through
+ * half the Kanaya-Okayama electron range divided by sin(anode angle), tungsten
+ * K lines above 69.525 kV with Green-Cosslett (kVp/E_K - 1)^1.63 intensity, and
+ * Beer-Lambert added filtration.  energy_step_kev is the bin width (0 = 0.5). */
+typedef struct dxg_tube {
+    double voltage_kv;
+    double anode_angle_deg;
+    const int32_t* filtration_Z;
+    const double* filtration_mm;
+    uint64_t n_filtration;
+    double energy_step_kev;
+} dxg_tube;
+
+/* Tube::getSpecter: bin energies and photon weights normalised to sum 1.
+ * Call with energy = weight = NULL to get the count in *n. */
+int dxg_tube_spectrum(const dxg_tube* tube, double* energy_kev, double* weight, uint64_t cap, uint64_t* n);
+/* Beam::tubeAlHalfValueLayer() (mm Al, air-kerma weighted) and
+ * Beam::tubeMeanSpecterEnergy() (keV, photon weighted) (beamsettingsmodel.cpp:331-343). */
+int dxg_tube_hvl_mm_al(const dxg_tube* tube, double* hvl_mm);
+int dxg_tube_mean_energy(const dxg_tube* tube, double* mean_kev);
+
+/* CTAECFilter(start, stop, weights) (datacontainer.cpp:59): tube current
+ * weights along start->stop, linearly interpolated at the projection of the
+ * exposure position and normalised to mean one.  n = 0 means no AEC. */
+typedef struct dxg_aec {
+    double start[3];
+    double stop[3];
+    const double* weights;
+    uint64_t n;
+} dxg_aec;
+
+/* PencilBeam<false>: position, direction, energy (20..150 keV clamp,
+ * beamsettingsmodel.cpp:664), numberOfExposures, numberOfParticlesPerExposure.
+ * Uses spectrum 0, no bowtie. */
+typedef struct dxg_pencil_beam {
+    double position[3];
+    double direction[3];
+    uint64_t n_exposures;
+    uint64_t particles_per_exposure;
+} dxg_pencil_beam;
+
+/* DXBeam<false>: position, directionCosines, collimationHalfAngles,
+ * numberOfExposures (dxmc_specialization.cpp:21-90 computes position and
+ * cosines from rotation centre, SPD and angles). */
+typedef struct dxg_dx_beam {
+    double position[3];
+    double cosines[6];
+    double collimation_half_angles[2];
+    uint64_t n_exposures;
+    uint64_t particles_per_exposure;
+} dxg_dx_beam;
+
+/* CTSpiralBeam<false> / CTSpiralDualEnergyBeam<false> (beamsettingsmodel.cpp:1150-1837,
+ * hdf5wrapper.cpp:548-585).  The gantry turns about start->stop; the table
+ * advances pitch * collimation per turn.  Exposure i sits at angle
+ * start_angle + i * step_angle.  For dual energy, tube B is offset by
+ * tube_b_offset_angle and exposures interleave A, B (beamactorcontainer.cpp:135-146),
+ * tube A uses spectrum/bowtie 0 with weight relative_mas_a, tube B spectrum/bowtie 1
+ * with weight relative_mas_b; fov_b is tube B's field of view.  has_bowtie = 0
+ * leaves bowtie_index -1. */
+typedef struct dxg_ct_spiral_beam {
+    double start[3];
+    double stop[3];
+    double start_angle;
+    double step_angle;
+    double pitch;
+    double collimation_cm;
+    double sdd_cm;
+    double fov_cm;
+    uint64_t particles_per_exposure;
+    int32_t dual_source;
+    int32_t has_bowtie;
+    double fov_b_cm;
+    double tube_b_offset_angle;
+    double relative_mas_a;
+    double relative_mas_b;
+    dxg_aec aec;
+} dxg_ct_spiral_beam;
+
+/* CTSequentialBeam<false> (beamsettingsmodel.cpp:900-1150): number_of_slices
+ * axial rotations at position + k * slice_spacing * scan_normal, each from
+ * start_angle in step_angle steps over one full turn. */
+typedef struct dxg_ct_sequential_beam {
+    double position[3];
+    double scan_normal[3];
+    uint64_t number_of_slices;
+    double slice_spacing_cm;
+    double start_angle;
+    double step_angle;
+    double collimation_cm;
+    double sdd_cm;
+    double fov_cm;
+    uint64_t particles_per_exposure;
+    int32_t has_bowtie;
+    int32_t pad;
+} dxg_ct_sequential_beam;
+
+/* CBCTBeam<false> (beamsettingsmodel.cpp:720-900): isocenter, rotationAxis,
+ * SDD, start/stop/step angle, collimationHalfAngles. */
+typedef struct dxg_cbct_beam {
+    double isocenter[3];
+    double rotation_axis[3];
+    double start_angle;
+    double stop_angle;
+    double step_angle;
+    double sdd_cm;
+    double collimation_half_angles[2];
+    uint64_t particles_per_exposure;
+} dxg_cbct_beam;
+
+/* Exposure builders: numberOfExposures() with out = NULL, else fill up to cap. */
+int dxg_pencil_exposures(const dxg_pencil_beam* b, dxg_exposure* out, uint64_t cap, uint64_t* n);
+int dxg_dx_exposures(const dxg_dx_beam* b, dxg_exposure* out, uint64_t cap, uint64_t* n);
+int dxg_ct_spiral_exposures(const dxg_ct_spiral_beam* b, dxg_exposure* out, uint64_t cap, uint64_t* n);
+int dxg_ct_sequential_exposures(const dxg_ct_sequential_beam* b, dxg_exposure* out, uint64_t cap, uint64_t* n);
+int dxg_cbct_exposures(const dxg_cbct_beam* b, dxg_exposure* out, uint64_t cap, uint64_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DXMC_BEAMS_H */

@@ -1,0 +1,190 @@
+/* C-ABI of the MI355X voxel photon-transport engine (libdxmc_hip.so).
+ *
+ * This is the drop-in boundary for DXMClib's voxel dose path as OpenDXMC
+ * drives it from SimulationPipeline::worker<C> (simulationpipeline.cpp:124-235).
+ * Plain pointers and sizes only; no C++ or torch types cross it.  Every call
+ * returns DXG_OK (0) or a negative status and never throws; the message of the
+ * last failure on the calling thread is in dxg_last_error().  The C++ facade
+ * under include/dxmc/ rebuilds World / AAVoxelGrid / Transport /
+ * TransportProgress / DoseScore on top of these entry points.
+ *
+ * Units follow the reference: lengths in cm (datacontainer.cpp:247-252),
+ * densities in g/cm3, energies in keV, angles in radians, dose in mGy
+ * (datacontainer.hpp:109).  The voxel grid is centred on the origin
+ * (datacontainer.cpp:174-180) and indexed i = x + y*nx + z*nx*ny
+ * (otherphantomimportpipeline.cpp:44).
+ */
+#ifndef DXMC_HIP_H
+#define DXMC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DXG_OK = 0,
+    DXG_ERR_ARGUMENT = -1,     /* null pointer, bad size, bad enum */
+    DXG_ERR_STATE = -2,        /* call out of order (e.g. run before build) */
+    DXG_ERR_MATERIAL = -3,     /* Material::byWeight failure */
+    DXG_ERR_DATA = -4,         /* material index >= number of materials */
+    DXG_ERR_PHYSICS = -5,      /* physics list missing or corrupt */
+    DXG_ERR_DEVICE = -6,       /* HIP runtime failure / no device */
+    DXG_ERR_UNSUPPORTED = -7,  /* feature outside this build (e.g. level 2) */
+    DXG_ERR_CANCELLED = -8     /* progress stop flag was set during the run */
+};
+
+typedef struct dxg_world dxg_world;
+typedef struct dxg_progress dxg_progress;
+
+/* Material<5>::byWeight(std::map<Z, mass fraction>)  (simulationpipeline.cpp:136).
+ * Fractions are normalised to one. */
+typedef struct dxg_material {
+    const int32_t* Z;
+    const double* mass_fraction;
+    uint64_t n;
+} dxg_material;
+
+/* One exposure of a beam: Beam::exposure(i) with position(), directionCosines()
+ * and collimationHalfAngles() (beamactorcontainer.cpp:114-150).  The beam axis
+ * is cross(cosines[0..2], cosines[3..5]) (beamactorcontainer.cpp:52).  A
+ * particle leaves `position` in the direction
+ *   c0*sin(ax) + c1*sin(ay) + axis*sqrt(1 - sin^2 ax - sin^2 ay),
+ * with ax, ay uniform in +-collimation_half_angles (beamactorcontainer.cpp:44-69);
+ * its weight is `weight` times the bowtie weight at ax. */
+typedef struct dxg_exposure {
+    double position[3];
+    double cosines[6];
+    double collimation_half_angles[2];
+    double weight;
+    uint32_t spectrum_index;  /* into dxg_beam.spectra */
+    int32_t bowtie_index;     /* into dxg_beam.bowties, or -1 */
+    uint64_t n_particles;     /* numberOfParticlesPerExposure() */
+} dxg_exposure;
+
+/* Discrete photon spectrum: a particle takes energy[k] with probability
+ * weight[k] / sum(weight).  A monoenergetic beam has n = 1. */
+typedef struct dxg_spectrum {
+    const double* energy_kev;
+    const double* weight;
+    uint64_t n;
+} dxg_spectrum;
+
+/* BowtieFilter(vector<pair<angle, weight>>) (bowtiefilterreader.cpp:92).
+ * Linear interpolation in |angle|, clamped at the ends, normalised so the
+ * central ray has weight one. */
+typedef struct dxg_bowtie {
+    const double* angle_rad;
+    const double* weight;
+    uint64_t n;
+} dxg_bowtie;
+
+/* Everything Transport::operator()(world, beam, ...) needs from a Beam. */
+typedef struct dxg_beam {
+    const dxg_exposure* exposures;
+    uint64_t n_exposures;
+    const dxg_spectrum* spectra;
+    uint64_t n_spectra;
+    const dxg_bowtie* bowties;
+    uint64_t n_bowties;
+} dxg_beam;
+
+/* Device-side event counters of the last dxg_transport_run. */
+typedef struct dxg_counters {
+    uint64_t histories;
+    uint64_t steps;            /* Woodcock steps, real + virtual */
+    uint64_t real_events;      /* real interactions */
+    uint64_t deposits;         /* scored deposits (tally updates) */
+    double emitted_energy;     /* sum of w*E over emitted photons (keV) */
+    double escaped_energy;     /* sum of w*E leaving the grid (keV) */
+    double deposited_energy;   /* sum of scored w*dE (keV) */
+    double kernel_ms;          /* summed duration of the transport kernels */
+    uint64_t kernel_launches;
+} dxg_counters;
+
+const char* dxg_last_error(void);
+
+/* ---- World<AAVoxelGrid<5, CORRECTION, 255>> (simulationpipeline.cpp:127-131).
+ * correction_level is the CORRECTION template argument: 0 = free electrons,
+ * 1 = form factor / incoherent scattering function (the reference default,
+ * simulationpipeline.hpp:62); 2 is reported as DXG_ERR_UNSUPPORTED. */
+int dxg_world_create(dxg_world** out, int correction_level);
+void dxg_world_destroy(dxg_world* w);
+
+/* AAVoxelGrid::setData(dims, density, material, materials) (simulationpipeline.cpp:149).
+ * Copies the arrays.  Fails with DXG_ERR_DATA when a material index is not
+ * below n_mats (the readiness check of simulationpipeline.cpp:50-58) and with
+ * DXG_ERR_MATERIAL when a material cannot be built (:136-140). */
+int dxg_grid_set_data(dxg_world* w, const uint64_t dims[3], const double* density,
+                      const uint8_t* material, const dxg_material* mats, uint64_t n_mats);
+
+/* AAVoxelGrid::setSpacing(std::array<double,3> cm) (simulationpipeline.cpp:150). */
+int dxg_grid_set_spacing(dxg_world* w, const double spacing_cm[3]);
+
+/* World::build() (simulationpipeline.cpp:153): attenuation tables, Woodcock
+ * majorant and sampling tables are built and the grid is made resident in
+ * device memory.  Clears the scores. */
+int dxg_world_build(dxg_world* w);
+
+/* AAVoxelGrid::size() (simulationpipeline.cpp:174). */
+uint64_t dxg_grid_size(const dxg_world* w);
+
+/* Transport::operator()(world, beam, progress, useBeamCalibration)
+ * (simulationpipeline.cpp:163-166).  Blocking.  Scores accumulate over calls.
+ * `seed` keys the per-history random streams; histories of successive calls
+ * on one world are numbered consecutively so their streams never overlap.
+ * `progress` may be null.  When another thread calls
+ * dxg_progress_request_stop() the run stops taking new histories and returns
+ * DXG_ERR_CANCELLED (the reference returns without collecting dose,
+ * simulationpipeline.cpp:169-170).  apply_calibration is accepted for
+ * signature parity; the calibration factor applied is 1. */
+int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress,
+                      int apply_calibration, uint64_t seed);
+
+/* Zero the energy scores and the history count. */
+int dxg_world_clear_scores(dxg_world* w);
+
+/* EnergyScore per voxel: sum of w*dE (keV), sum of (w*dE)^2 (keV^2), events. */
+int dxg_read_energy_scores(const dxg_world* w, double* sum_e, double* sum_e2, uint64_t* n_events);
+
+/* DoseScore per voxel: vgrid.doseScored(i).{dose(), variance(), numberOfEvents()}
+ * (simulationpipeline.cpp:178,204,219).  dose = sum_e / (rho V) in mGy;
+ * variance (mGy^2) of that sum over the H histories run so far,
+ * H/(H-1) * (sum_e2 - sum_e^2 / H), converted like the dose.  Any output
+ * pointer may be null. */
+int dxg_read_dose_scores(const dxg_world* w, double* dose_mgy, double* variance_mgy2, uint64_t* n_events);
+
+/* Histories run into the scores so far. */
+uint64_t dxg_world_histories(const dxg_world* w);
+
+int dxg_get_counters(const dxg_world* w, dxg_counters* out);
+
+/* ---- TransportProgress (simulationpipeline.hpp:23,65; simulationpipeline.cpp:112-122,259-262).
+ * All progress calls are thread-safe and may be made from any thread. */
+int dxg_progress_create(dxg_progress** out);
+void dxg_progress_destroy(dxg_progress* p);
+void dxg_progress_request_stop(dxg_progress* p);        /* setStopSimulation() */
+int dxg_progress_continue(const dxg_progress* p);       /* continueSimulation() */
+void dxg_progress_get(const dxg_progress* p, uint64_t* n, uint64_t* total);  /* progress() */
+/* message(): copies at most cap-1 bytes plus a terminating zero. */
+void dxg_progress_message(const dxg_progress* p, char* buf, uint64_t cap);
+
+/* ---- Material::attenuationValues(E) (ctsegmentationpipeline.cpp:73-98):
+ * out[4*i + {0,1,2,3}] = photoelectric, incoherent, coherent, total (cm2/g). */
+int dxg_material_attenuation(const dxg_material* m, const double* energy_kev, uint64_t n, double* out);
+
+/* ---- Deterministic traversal through the kernel's own device functions:
+ * the slab entry of each ray into the grid box, then p = fma(steps[k], d, p)
+ * and the voxel index floor((p - box_min) * (1/spacing)).  out_index is -1
+ * outside the grid, out_material the material id or -1.  Needs a built world;
+ * pos/dir are n_rays x 3 float, steps n_rays x n_steps float. */
+int dxg_debug_traverse(const dxg_world* w, uint64_t n_rays, const float* pos, const float* dir,
+                       uint64_t n_steps, const float* steps, int64_t* out_index, int32_t* out_material);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DXMC_HIP_H */

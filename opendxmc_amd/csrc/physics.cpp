@@ -1,0 +1,313 @@
+// Physics list reader, mixture rule and device table builder.
+#include "physics.hpp"
+
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <mutex>
+
+namespace dxg {
+
+namespace {
+
+template <typename T>
+bool readVec(std::ifstream& f, std::vector<T>& v, size_t n)
+{
+    v.resize(n);
+    f.read(reinterpret_cast<char*>(v.data()), static_cast<std::streamsize>(n * sizeof(T)));
+    return static_cast<size_t>(f.gcount()) == n * sizeof(T);
+}
+
+template <typename T>
+bool readOne(std::ifstream& f, T& v)
+{
+    f.read(reinterpret_cast<char*>(&v), sizeof(T));
+    return f.gcount() == sizeof(T);
+}
+
+// Interpolate log(v) linearly in log(e); `le` ascending, duplicates mark edges.
+double interpLogLog(const std::vector<double>& le, const std::vector<double>& lv, double logE)
+{
+    if (logE <= le.front())
+        return std::exp(lv.front());
+    if (logE >= le.back())
+        return std::exp(lv.back());
+    const auto it = std::upper_bound(le.begin(), le.end(), logE);
+    const size_t b = static_cast<size_t>(it - le.begin());
+    const size_t a = b - 1;
+    const double t = (logE - le[a]) / (le[b] - le[a]);
+    return std::exp(lv[a] + t * (lv[b] - lv[a]));
+}
+
+double interpLinear(const std::vector<double>& xs, const std::vector<double>& v, double x)
+{
+    if (x <= xs.front())
+        return v.front();
+    if (x >= xs.back())
+        return v.back();
+    const auto it = std::upper_bound(xs.begin(), xs.end(), x);
+    const size_t b = static_cast<size_t>(it - xs.begin());
+    const size_t a = b - 1;
+    return v[a] + (x - xs[a]) / (xs[b] - xs[a]) * (v[b] - v[a]);
+}
+
+double safeLog(double v) { return std::log(std::max(v, 1e-300)); }
+
+float roundUp(double v)
+{
+    float f = static_cast<float>(v);
+    if (static_cast<double>(f) < v)
+        f = std::nextafter(f, 3.0e38f);
+    return f;
+}
+
+}  // namespace
+
+bool PhysicsList::load(const std::string& path, std::string& err)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        err = "cannot open physics list '" + path + "'";
+        return false;
+    }
+    char magic[8] = {};
+    f.read(magic, 8);
+    uint32_t zmax = 0, ne = 0, nx = 0;
+    if (std::memcmp(magic, "DXGPHYS1", 8) != 0 || !readOne(f, zmax) || !readOne(f, ne) || !readOne(f, nx)) {
+        err = "physics list '" + path + "' has a bad header";
+        return false;
+    }
+    std::vector<double> e;
+    if (!readVec(f, e, ne) || !readVec(f, x_, nx)) {
+        err = "physics list truncated";
+        return false;
+    }
+    log_e_.resize(ne);
+    for (uint32_t i = 0; i < ne; ++i)
+        log_e_[i] = std::log(e[i]);
+    std::vector<ElementData> els(zmax);
+    for (auto& el : els) {
+        std::vector<double> inc, coh, pe_e, pe, shells;
+        uint32_t npe = 0, nsh = 0;
+        bool ok = readOne(f, el.A) && readVec(f, inc, ne) && readVec(f, coh, ne) && readVec(f, el.F, nx)
+            && readVec(f, el.S, nx) && readOne(f, npe) && readVec(f, pe_e, npe) && readVec(f, pe, npe)
+            && readOne(f, nsh) && readVec(f, shells, static_cast<size_t>(nsh) * 5);
+        if (!ok || el.A <= 0) {
+            err = "physics list element block truncated";
+            return false;
+        }
+        el.log_inc.resize(ne);
+        el.log_coh.resize(ne);
+        for (uint32_t i = 0; i < ne; ++i) {
+            el.log_inc[i] = safeLog(inc[i]);
+            el.log_coh[i] = safeLog(coh[i]);
+        }
+        el.pe_log_e.resize(npe);
+        el.pe_log_v.resize(npe);
+        for (uint32_t i = 0; i < npe; ++i) {
+            el.pe_log_e[i] = std::log(pe_e[i]);
+            el.pe_log_v[i] = safeLog(pe[i]);
+        }
+    }
+    elements_ = std::move(els);
+    return true;
+}
+
+double PhysicsList::photo(int z, double e) const
+{
+    const auto& el = element(z);
+    return interpLogLog(el.pe_log_e, el.pe_log_v, std::log(e));
+}
+double PhysicsList::incoherent(int z, double e) const { return interpLogLog(log_e_, element(z).log_inc, std::log(e)); }
+double PhysicsList::coherent(int z, double e) const { return interpLogLog(log_e_, element(z).log_coh, std::log(e)); }
+double PhysicsList::formFactor(int z, double x) const { return interpLinear(x_, element(z).F, x); }
+double PhysicsList::scatterFunction(int z, double x) const { return interpLinear(x_, element(z).S, x); }
+
+void MixMaterial::attenuation(const PhysicsList& pl, double e, double& pe, double& inc, double& coh) const
+{
+    pe = inc = coh = 0;
+    for (size_t i = 0; i < Z.size(); ++i) {
+        const double n = atoms_per_gram[i] * 1e-24;   // barn -> cm^2
+        pe += n * pl.photo(Z[i], e);
+        inc += n * pl.incoherent(Z[i], e);
+        coh += n * pl.coherent(Z[i], e);
+    }
+}
+
+double MixMaterial::S(const PhysicsList& pl, double x) const
+{
+    double s = 0;
+    for (size_t i = 0; i < Z.size(); ++i)
+        s += atoms_per_gram[i] * pl.scatterFunction(Z[i], x);
+    return s;
+}
+
+double MixMaterial::F2(const PhysicsList& pl, double x) const
+{
+    double s = 0;
+    for (size_t i = 0; i < Z.size(); ++i) {
+        const double f = pl.formFactor(Z[i], x);
+        s += atoms_per_gram[i] * f * f;
+    }
+    return s;
+}
+
+bool makeMaterial(const PhysicsList& pl, const int32_t* Z, const double* w, uint64_t n, MixMaterial& out, std::string& err)
+{
+    if (n == 0 || Z == nullptr || w == nullptr) {
+        err = "material has no elements";
+        return false;
+    }
+    double total = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (Z[i] < 1 || Z[i] > pl.zmax()) {
+            err = "material element Z=" + std::to_string(Z[i]) + " outside the physics list";
+            return false;
+        }
+        if (!std::isfinite(w[i]) || w[i] < 0) {
+            err = "material mass fraction must be finite and non-negative";
+            return false;
+        }
+        total += w[i];
+    }
+    if (!(total > 0)) {
+        err = "material mass fractions sum to zero";
+        return false;
+    }
+    MixMaterial m;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (w[i] == 0)
+            continue;
+        const double apg = (w[i] / total) * kNA / pl.element(Z[i]).A;
+        m.Z.push_back(Z[i]);
+        m.atoms_per_gram.push_back(apg);
+        m.electrons_per_gram += apg * Z[i];
+    }
+    out = std::move(m);
+    return true;
+}
+
+void buildTables(const PhysicsList& pl, const std::vector<MixMaterial>& mats, const std::vector<double>& rho_max, DeviceTables& t)
+{
+    const int nm = static_cast<int>(mats.size());
+    t.n_mat = nm;
+    t.mu_tot.assign(static_cast<size_t>(nm) * kLutN, 0.f);
+    t.mu_parts.assign(static_cast<size_t>(nm) * kLutN * 4, 0.f);
+    t.majorant.assign(kLutN, 0.f);
+    std::vector<double> tot(static_cast<size_t>(nm) * kLutN);
+    for (int m = 0; m < nm; ++m) {
+        for (int j = 0; j < kLutN; ++j) {
+            const double e = kLutE0 + j * kLutDE;
+            double pe, inc, coh;
+            mats[static_cast<size_t>(m)].attenuation(pl, e, pe, inc, coh);
+            const size_t o = static_cast<size_t>(m) * kLutN + j;
+            tot[o] = pe + inc + coh;
+            t.mu_tot[o] = static_cast<float>(pe + inc + coh);
+            t.mu_parts[4 * o + 0] = static_cast<float>(pe);
+            t.mu_parts[4 * o + 1] = static_cast<float>(pe + inc);
+            t.mu_parts[4 * o + 2] = static_cast<float>(pe + inc + coh);
+            t.mu_parts[4 * o + 3] = 0.f;
+        }
+    }
+    // Bin majorant: linear interpolation never exceeds the larger end node, so
+    // max over materials of max(node j, node j+1) * rho_max bounds every voxel
+    // over [E_j, E_j+1].  The FP32 value is rounded up and padded by 1e-6 to
+    // cover the FP32 evaluation of mu * rho in the kernel.
+    for (int j = 0; j < kLutN; ++j) {
+        double mx = 0;
+        for (int m = 0; m < nm; ++m) {
+            const size_t o = static_cast<size_t>(m) * kLutN + j;
+            const double hi = j + 1 < kLutN ? std::max(tot[o], tot[o + 1]) : tot[o];
+            mx = std::max(mx, hi * rho_max[static_cast<size_t>(m)]);
+        }
+        // a vacuum grid still needs a positive majorant: photons then cross it in one step
+        t.majorant[static_cast<size_t>(j)] = roundUp(std::max(mx * (1.0 + 1e-6), 1e-20));
+    }
+    // Sampling tables for correction level 1.
+    constexpr int kFine = 16384;
+    t.s_table.assign(static_cast<size_t>(nm) * kSfN, 0.f);
+    t.ray_fwd.assign(static_cast<size_t>(nm) * kSfN, 0.f);
+    t.ray_inv.assign(static_cast<size_t>(nm) * kRayInvN, 0.f);
+    std::vector<double> ucum(kFine + 1), acum(kFine + 1);
+    for (int m = 0; m < nm; ++m) {
+        const MixMaterial& mat = mats[static_cast<size_t>(m)];
+        for (int k = 0; k < kSfN; ++k) {
+            const double x = kXMax * k / (kSfN - 1);
+            t.s_table[static_cast<size_t>(m) * kSfN + k] = static_cast<float>(std::min(1.0, mat.S(pl, x) / mat.electrons_per_gram));
+        }
+        // cumulative A(u) = int_0^u F^2(sqrt(u')) du' by the trapezoid rule on a fine x grid
+        ucum[0] = 0;
+        acum[0] = 0;
+        double fprev = mat.F2(pl, 0.0);
+        for (int k = 1; k <= kFine; ++k) {
+            const double x = kXMax * k / kFine;
+            const double u = x * x;
+            const double f = mat.F2(pl, x);
+            ucum[k] = u;
+            acum[k] = acum[k - 1] + 0.5 * (f + fprev) * (u - ucum[k - 1]);
+            fprev = f;
+        }
+        const double atot = acum[kFine];
+        for (int k = 0; k < kSfN; ++k) {
+            const double x = kXMax * k / (kSfN - 1);
+            const double u = x * x;
+            // locate u in the fine grid
+            const auto it = std::upper_bound(ucum.begin(), ucum.end(), u);
+            size_t b = std::min(static_cast<size_t>(it - ucum.begin()), static_cast<size_t>(kFine));
+            size_t a = b - 1;
+            const double tt = ucum[b] > ucum[a] ? (u - ucum[a]) / (ucum[b] - ucum[a]) : 0.0;
+            const double av = std::min(atot, acum[a] + tt * (acum[b] - acum[a]));
+            t.ray_fwd[static_cast<size_t>(m) * kSfN + k] = static_cast<float>(av / atot);
+        }
+        t.ray_fwd[static_cast<size_t>(m) * kSfN + kSfN - 1] = 1.0f;
+        for (int k = 0; k < kRayInvN; ++k) {
+            const double target = atot * k / (kRayInvN - 1);
+            const auto it = std::lower_bound(acum.begin(), acum.end(), target);
+            size_t b = std::min(static_cast<size_t>(it - acum.begin()), static_cast<size_t>(kFine));
+            double u;
+            if (b == 0) {
+                u = 0;
+            } else {
+                const size_t a = b - 1;
+                const double span = acum[b] - acum[a];
+                u = span > 0 ? ucum[a] + (target - acum[a]) / span * (ucum[b] - ucum[a]) : ucum[b];
+            }
+            t.ray_inv[static_cast<size_t>(m) * kRayInvN + k] = static_cast<float>(u / (kXMax * kXMax));
+        }
+    }
+}
+
+const PhysicsList* sharedPhysics(std::string& err)
+{
+    static std::mutex mtx;
+    static std::unique_ptr<PhysicsList> list;
+    std::lock_guard<std::mutex> lk(mtx);
+    if (!list) {
+        auto pl = std::make_unique<PhysicsList>();
+        if (!pl->load(defaultPhysicsListPath(), err))
+            return nullptr;
+        list = std::move(pl);
+    }
+    return list.get();
+}
+
+std::string defaultPhysicsListPath()
+{
+    if (const char* env = std::getenv("DXG_PHYSICS_LIST"))
+        return env;
+    Dl_info info {};
+    if (dladdr(reinterpret_cast<void*>(&defaultPhysicsListPath), &info) && info.dli_fname) {
+        std::string so(info.dli_fname);
+        const auto slash = so.find_last_of('/');
+        const std::string dir = slash == std::string::npos ? "." : so.substr(0, slash);
+        return dir + "/physics/physics_list.bin";
+    }
+    return "physics/physics_list.bin";
+}
+
+}  // namespace dxg

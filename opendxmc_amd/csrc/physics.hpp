@@ -1,0 +1,95 @@
+// Host-side physics for the transport engine: the physics list reader, the
+// Material<5>::byWeight mixture rule and the FP32 device tables built by
+// World::build() (simulationpipeline.cpp:133-153).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace dxg {
+
+constexpr double kNA = 6.02214076e23;
+constexpr double kHcKeVAng = 12.398419843320026;   // h c [keV Angstrom]
+constexpr double kMeC2KeV = 510.99895000;          // electron rest energy [keV]
+constexpr double kKeVToJ = 1.602176634e-16;
+
+// Attenuation table energy grid: uniform, E_j = kLutE0 + j * kLutDE.
+constexpr double kLutE0 = 1.0;
+constexpr double kLutDE = 0.25;
+constexpr double kLutEMax = 200.0;
+constexpr int kLutN = 797;   // (kLutEMax - kLutE0) / kLutDE + 1
+
+// Form-factor sampling tables: x = sin(theta/2)/lambda in [0, kXMax] (1/Angstrom).
+constexpr double kXMax = 16.0;
+constexpr int kSfN = 1024;   // nodes of the S(x) and F^2 forward tables
+constexpr int kRayInvN = 1024;   // nodes of the inverse F^2 table
+
+// Photons below this energy deposit what is left and stop.
+constexpr double kCutoffKeV = 1.0;
+
+struct ElementData {
+    double A = 0;
+    std::vector<double> log_inc, log_coh;     // on the common log-energy grid
+    std::vector<double> F, S;                 // on the x grid
+    std::vector<double> pe_log_e, pe_log_v;   // edge-resolved photoelectric table
+};
+
+class PhysicsList {
+public:
+    bool load(const std::string& path, std::string& err);
+    bool loaded() const { return !elements_.empty(); }
+    int zmax() const { return static_cast<int>(elements_.size()); }
+    const ElementData& element(int z) const { return elements_[static_cast<size_t>(z - 1)]; }
+
+    // barn/atom at energy E (keV), log-log interpolation
+    double photo(int z, double e) const;
+    double incoherent(int z, double e) const;
+    double coherent(int z, double e) const;
+    double formFactor(int z, double x) const;
+    double scatterFunction(int z, double x) const;
+
+private:
+    std::vector<double> log_e_, x_;
+    std::vector<ElementData> elements_;
+};
+
+// A material built from element mass fractions (Material<5>::byWeight).
+struct MixMaterial {
+    std::vector<int> Z;
+    std::vector<double> atoms_per_gram;
+    double electrons_per_gram = 0;
+
+    // cm^2/g
+    void attenuation(const PhysicsList& pl, double e, double& pe, double& inc, double& coh) const;
+    double S(const PhysicsList& pl, double x) const;    // per gram
+    double F2(const PhysicsList& pl, double x) const;   // per gram
+};
+
+bool makeMaterial(const PhysicsList& pl, const int32_t* Z, const double* w, uint64_t n, MixMaterial& out, std::string& err);
+
+// FP32 tables in the device layout (see transport.hip for the consumers).
+struct DeviceTables {
+    int n_mat = 0;
+    std::vector<float> mu_tot;     // [m][j] total mass attenuation (cm^2/g), kLutN per material
+    std::vector<float> mu_parts;   // [m][j][4]: pe, pe+inc, total, 0 (cm^2/g)
+    std::vector<float> majorant;   // [j] 1/cm, valid over [E_j, E_j+1]
+    std::vector<float> s_table;    // [m][k] S(x_k)/S(inf), x_k = k kXMax/(kSfN-1)
+    std::vector<float> ray_fwd;    // [m][k] A(x_k)/A(kXMax), A(x) = int_0^{x^2} F^2 du
+    std::vector<float> ray_inv;    // [m][k] u(y_k)/kXMax^2, y_k = k/(kRayInvN-1)
+};
+
+void buildTables(const PhysicsList& pl, const std::vector<MixMaterial>& mats, const std::vector<double>& rho_max,
+                 DeviceTables& t);
+
+// Locate the physics list next to the shared library (physics/physics_list.bin),
+// or at $DXG_PHYSICS_LIST.
+std::string defaultPhysicsListPath();
+
+// Process-wide physics list, loaded on first use; nullptr and `err` on failure.
+const PhysicsList* sharedPhysics(std::string& err);
+
+// Thread-local message behind dxg_last_error(); returns `code`.
+int setError(int code, const std::string& msg);
+
+}  // namespace dxg

@@ -1,0 +1,682 @@
+// C-ABI of the transport engine: World / AAVoxelGrid / Transport /
+// TransportProgress / DoseScore on one MI355X (include/dxmc_hip.h).
+#include "dxmc_hip.h"
+
+#include "device.hpp"
+#include "physics.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) { return dxg::setError(code, msg); }
+
+const dxg::PhysicsList* physics(std::string& err) { return dxg::sharedPhysics(err); }
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        const hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess)                                                             \
+            return fail(DXG_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count)
+    {
+        if (count == n && ptr)
+            return hipSuccess;
+        release();
+        n = count;
+        return hipMalloc(reinterpret_cast<void**>(&ptr), std::max<size_t>(count, 1) * sizeof(T));
+    }
+    hipError_t upload(const std::vector<T>& v, hipStream_t s)
+    {
+        hipError_t e = alloc(v.size());
+        if (e != hipSuccess || v.empty())
+            return e;
+        return hipMemcpyAsync(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    }
+};
+
+}  // namespace
+
+struct dxg_progress {
+    std::atomic<uint64_t> n { 0 };
+    std::atomic<uint64_t> total { 0 };
+    std::atomic<bool> stop { false };
+    mutable std::mutex msg_mutex;
+    std::string msg;
+    void setMessage(const std::string& m)
+    {
+        std::lock_guard<std::mutex> lk(msg_mutex);
+        msg = m;
+    }
+};
+
+struct dxg_world {
+    int level = 1;
+    bool has_data = false, has_spacing = false, built = false;
+    uint64_t dims[3] = { 0, 0, 0 };
+    double spacing[3] = { 0, 0, 0 };
+    std::vector<double> density;
+    std::vector<uint8_t> material;
+    std::vector<dxg::MixMaterial> mats;
+    std::vector<double> rho_max;
+    dxg::DeviceTables tables;
+
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf<uint8_t> d_material;
+    DevBuf<float> d_density;
+    DevBuf<float> d_mu_tot;
+    DevBuf<float4> d_mu_parts;
+    DevBuf<float> d_majorant, d_s, d_rfwd, d_rinv;
+    DevBuf<double> d_tally;
+    DevBuf<unsigned long long> d_pool;
+    DevBuf<dxg::DevCounters> d_counters;
+    // per-run beam tables
+    DevBuf<dxg::DevExposure> d_ex;
+    DevBuf<unsigned long long> d_ex_start;
+    DevBuf<float> d_spec_e, d_spec_p, d_bow_a, d_bow_w;
+    DevBuf<int> d_spec_alias, d_spec_off, d_spec_n, d_bow_off, d_bow_n;
+
+    uint64_t histories = 0;
+    dxg_counters counters {};
+
+    ~dxg_world()
+    {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+    uint64_t size() const { return dims[0] * dims[1] * dims[2]; }
+};
+
+namespace dxg {
+int setError(int code, const std::string& msg)
+{
+    g_error = msg;
+    return code;
+}
+}  // namespace dxg
+
+namespace {
+
+void fillParams(const dxg_world* w, dxg::DevParams& p)
+{
+    std::memset(&p, 0, sizeof p);
+    p.material = w->d_material.ptr;
+    p.density = w->d_density.ptr;
+    p.nx = static_cast<int>(w->dims[0]);
+    p.ny = static_cast<int>(w->dims[1]);
+    p.nz = static_cast<int>(w->dims[2]);
+    p.n_mat = static_cast<int>(w->mats.size());
+    for (int a = 0; a < 3; ++a) {
+        const float half = 0.5f * (static_cast<float>(w->dims[a]) * static_cast<float>(w->spacing[a]));
+        p.bmin[a] = -half;
+        p.bmax[a] = half;
+        p.inv_spacing[a] = 1.0f / static_cast<float>(w->spacing[a]);
+    }
+    p.level = w->level;
+    p.mu_tot = w->d_mu_tot.ptr;
+    p.mu_parts = w->d_mu_parts.ptr;
+    p.majorant = w->d_majorant.ptr;
+    p.s_table = w->d_s.ptr;
+    p.ray_fwd = w->d_rfwd.ptr;
+    p.ray_inv = w->d_rinv.ptr;
+    p.tally = w->d_tally.ptr;
+    p.pool = w->d_pool.ptr;
+    p.counters = w->d_counters.ptr;
+}
+
+// Walker alias table (Vose's construction).
+void aliasTable(const std::vector<double>& weight, std::vector<float>& prob, std::vector<int>& alias)
+{
+    const size_t n = weight.size();
+    double tot = 0;
+    for (double v : weight)
+        tot += v;
+    std::vector<double> q(n);
+    for (size_t i = 0; i < n; ++i)
+        q[i] = weight[i] / tot * static_cast<double>(n);
+    std::vector<size_t> small, large;
+    for (size_t i = 0; i < n; ++i)
+        (q[i] < 1.0 ? small : large).push_back(i);
+    prob.assign(n, 1.0f);
+    alias.assign(n, 0);
+    for (size_t i = 0; i < n; ++i)
+        alias[i] = static_cast<int>(i);
+    while (!small.empty() && !large.empty()) {
+        const size_t s = small.back();
+        small.pop_back();
+        const size_t l = large.back();
+        prob[s] = static_cast<float>(q[s]);
+        alias[s] = static_cast<int>(l);
+        q[l] = (q[l] + q[s]) - 1.0;
+        if (q[l] < 1.0) {
+            large.pop_back();
+            small.push_back(l);
+        }
+    }
+    for (size_t i : large)
+        prob[i] = 1.0f;
+    for (size_t i : small)
+        prob[i] = 1.0f;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dxg_last_error(void) { return g_error.c_str(); }
+
+int dxg_world_create(dxg_world** out, int correction_level)
+{
+    if (!out)
+        return fail(DXG_ERR_ARGUMENT, "dxg_world_create: null output pointer");
+    *out = nullptr;
+    if (correction_level == 2)
+        return fail(DXG_ERR_UNSUPPORTED, "correction level 2 (impulse approximation, fluorescence) is not available in this build");
+    if (correction_level < 0 || correction_level > 2)
+        return fail(DXG_ERR_ARGUMENT, "correction level must be 0, 1 or 2");
+    auto w = std::make_unique<dxg_world>();
+    w->level = correction_level;
+    *out = w.release();
+    return DXG_OK;
+}
+
+void dxg_world_destroy(dxg_world* w) { delete w; }
+
+int dxg_grid_set_data(dxg_world* w, const uint64_t dims[3], const double* density, const uint8_t* material,
+                      const dxg_material* mats, uint64_t n_mats)
+{
+    if (!w || !dims || !density || !material || (!mats && n_mats))
+        return fail(DXG_ERR_ARGUMENT, "dxg_grid_set_data: null argument");
+    if (dims[0] == 0 || dims[1] == 0 || dims[2] == 0)
+        return fail(DXG_ERR_ARGUMENT, "dxg_grid_set_data: zero dimension");
+    const uint64_t n = dims[0] * dims[1] * dims[2];
+    if (n >= (1ull << 31))
+        return fail(DXG_ERR_ARGUMENT, "dxg_grid_set_data: grid has 2^31 or more voxels");
+    if (n_mats == 0 || n_mats > 255)
+        return fail(DXG_ERR_ARGUMENT, "dxg_grid_set_data: need 1..255 materials");
+    std::string err;
+    const dxg::PhysicsList* pl = physics(err);
+    if (!pl)
+        return fail(DXG_ERR_PHYSICS, err);
+    std::vector<dxg::MixMaterial> built(n_mats);
+    for (uint64_t i = 0; i < n_mats; ++i)
+        if (!dxg::makeMaterial(*pl, mats[i].Z, mats[i].mass_fraction, mats[i].n, built[i], err))
+            return fail(DXG_ERR_MATERIAL, "material " + std::to_string(i) + ": " + err);
+    std::vector<double> rho_max(n_mats, 0.0);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint8_t m = material[i];
+        if (m >= n_mats)
+            return fail(DXG_ERR_DATA, "voxel " + std::to_string(i) + " has material index " + std::to_string(m) + " >= " + std::to_string(n_mats));
+        const double d = density[i];
+        if (!(d >= 0) || !std::isfinite(d))
+            return fail(DXG_ERR_DATA, "voxel " + std::to_string(i) + " has a negative or non-finite density");
+        rho_max[m] = std::max(rho_max[m], d);
+    }
+    for (int a = 0; a < 3; ++a)
+        w->dims[a] = dims[a];
+    w->density.assign(density, density + n);
+    w->material.assign(material, material + n);
+    w->mats = std::move(built);
+    w->rho_max = std::move(rho_max);
+    w->has_data = true;
+    w->built = false;
+    return DXG_OK;
+}
+
+int dxg_grid_set_spacing(dxg_world* w, const double spacing_cm[3])
+{
+    if (!w || !spacing_cm)
+        return fail(DXG_ERR_ARGUMENT, "dxg_grid_set_spacing: null argument");
+    for (int a = 0; a < 3; ++a)
+        if (!(spacing_cm[a] > 0) || !std::isfinite(spacing_cm[a]))
+            return fail(DXG_ERR_ARGUMENT, "dxg_grid_set_spacing: spacing must be positive");
+    for (int a = 0; a < 3; ++a)
+        w->spacing[a] = spacing_cm[a];
+    w->has_spacing = true;
+    w->built = false;
+    return DXG_OK;
+}
+
+int dxg_world_build(dxg_world* w)
+{
+    if (!w)
+        return fail(DXG_ERR_ARGUMENT, "dxg_world_build: null world");
+    if (!w->has_data || !w->has_spacing)
+        return fail(DXG_ERR_STATE, "dxg_world_build: setData and setSpacing must come first");
+    std::string err;
+    const dxg::PhysicsList* pl = physics(err);
+    if (!pl)
+        return fail(DXG_ERR_PHYSICS, err);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(DXG_ERR_DEVICE, "no HIP device is visible");
+    HIP_TRY(hipGetDevice(&w->device));
+    if (!w->stream)
+        HIP_TRY(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+    dxg::buildTables(*pl, w->mats, w->rho_max, w->tables);
+    const uint64_t n = w->size();
+    std::vector<float> dens32(n);
+    for (uint64_t i = 0; i < n; ++i)
+        dens32[i] = static_cast<float>(w->density[i]);
+    HIP_TRY(w->d_material.upload(w->material, w->stream));
+    HIP_TRY(w->d_density.upload(dens32, w->stream));
+    HIP_TRY(w->d_mu_tot.upload(w->tables.mu_tot, w->stream));
+    {
+        std::vector<float4> parts(w->tables.mu_parts.size() / 4);
+        std::memcpy(parts.data(), w->tables.mu_parts.data(), w->tables.mu_parts.size() * sizeof(float));
+        HIP_TRY(w->d_mu_parts.upload(parts, w->stream));
+    }
+    HIP_TRY(w->d_majorant.upload(w->tables.majorant, w->stream));
+    HIP_TRY(w->d_s.upload(w->tables.s_table, w->stream));
+    HIP_TRY(w->d_rfwd.upload(w->tables.ray_fwd, w->stream));
+    HIP_TRY(w->d_rinv.upload(w->tables.ray_inv, w->stream));
+    HIP_TRY(w->d_tally.alloc(4 * n));
+    HIP_TRY(hipMemsetAsync(w->d_tally.ptr, 0, 4 * n * sizeof(double), w->stream));
+    HIP_TRY(w->d_pool.alloc(1));
+    HIP_TRY(w->d_counters.alloc(1));
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    w->histories = 0;
+    w->counters = dxg_counters {};
+    w->built = true;
+    return DXG_OK;
+}
+
+uint64_t dxg_grid_size(const dxg_world* w) { return w ? w->size() : 0; }
+
+uint64_t dxg_world_histories(const dxg_world* w) { return w ? w->histories : 0; }
+
+int dxg_world_clear_scores(dxg_world* w)
+{
+    if (!w || !w->built)
+        return fail(DXG_ERR_STATE, "dxg_world_clear_scores: world not built");
+    HIP_TRY(hipMemsetAsync(w->d_tally.ptr, 0, 4 * w->size() * sizeof(double), w->stream));
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    w->histories = 0;
+    return DXG_OK;
+}
+
+int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress, int apply_calibration, uint64_t seed)
+{
+    (void)apply_calibration;
+    if (!w || !beam)
+        return fail(DXG_ERR_ARGUMENT, "dxg_transport_run: null argument");
+    if (!w->built)
+        return fail(DXG_ERR_STATE, "dxg_transport_run: World::build() has not run");
+    if (beam->n_exposures == 0 || !beam->exposures)
+        return fail(DXG_ERR_ARGUMENT, "dxg_transport_run: beam has no exposures");
+    if (beam->n_spectra == 0 || !beam->spectra)
+        return fail(DXG_ERR_ARGUMENT, "dxg_transport_run: beam has no spectrum");
+    HIP_TRY(hipSetDevice(w->device));
+
+    // ---- spectra -> alias tables
+    std::vector<float> se, sp;
+    std::vector<int> sa, soff, sn;
+    for (uint64_t s = 0; s < beam->n_spectra; ++s) {
+        const dxg_spectrum& spec = beam->spectra[s];
+        if (spec.n == 0 || !spec.energy_kev || !spec.weight)
+            return fail(DXG_ERR_ARGUMENT, "spectrum " + std::to_string(s) + " is empty");
+        double tot = 0;
+        std::vector<double> wv(spec.n);
+        for (uint64_t k = 0; k < spec.n; ++k) {
+            const double e = spec.energy_kev[k];
+            if (!(e >= dxg::kCutoffKeV && e <= dxg::kLutEMax))
+                return fail(DXG_ERR_ARGUMENT, "spectrum energy " + std::to_string(e) + " keV outside [1, 200] keV");
+            if (!(spec.weight[k] >= 0) || !std::isfinite(spec.weight[k]))
+                return fail(DXG_ERR_ARGUMENT, "spectrum weights must be finite and non-negative");
+            wv[k] = spec.weight[k];
+            tot += wv[k];
+        }
+        if (!(tot > 0))
+            return fail(DXG_ERR_ARGUMENT, "spectrum weights sum to zero");
+        std::vector<float> prob;
+        std::vector<int> alias;
+        aliasTable(wv, prob, alias);
+        soff.push_back(static_cast<int>(se.size()));
+        sn.push_back(static_cast<int>(spec.n));
+        for (uint64_t k = 0; k < spec.n; ++k) {
+            se.push_back(static_cast<float>(spec.energy_kev[k]));
+            sp.push_back(prob[k]);
+            sa.push_back(alias[k]);
+        }
+    }
+    // ---- bowties, sorted by |angle| and normalised to the central ray
+    std::vector<float> ba, bw;
+    std::vector<int> boff, bn;
+    for (uint64_t b = 0; b < beam->n_bowties; ++b) {
+        const dxg_bowtie& bt = beam->bowties[b];
+        if (bt.n == 0 || !bt.angle_rad || !bt.weight)
+            return fail(DXG_ERR_ARGUMENT, "bowtie " + std::to_string(b) + " is empty");
+        std::vector<std::pair<double, double>> nodes;
+        for (uint64_t k = 0; k < bt.n; ++k)
+            nodes.emplace_back(std::fabs(bt.angle_rad[k]), bt.weight[k]);
+        std::sort(nodes.begin(), nodes.end());
+        double w0 = nodes.front().second;
+        if (nodes.front().first > 0 && nodes.size() > 1) {
+            // value at angle 0 is the first node (clamped)
+            w0 = nodes.front().second;
+        }
+        if (!(w0 > 0))
+            return fail(DXG_ERR_ARGUMENT, "bowtie central weight must be positive");
+        boff.push_back(static_cast<int>(ba.size()));
+        bn.push_back(static_cast<int>(nodes.size()));
+        for (auto& nd : nodes) {
+            ba.push_back(static_cast<float>(nd.first));
+            bw.push_back(static_cast<float>(nd.second / w0));
+        }
+    }
+    if (ba.empty()) {
+        ba.push_back(0.f);
+        bw.push_back(1.f);
+    }
+    // ---- exposures
+    std::vector<dxg::DevExposure> ex(beam->n_exposures);
+    std::vector<unsigned long long> ex_start(beam->n_exposures);
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < beam->n_exposures; ++i) {
+        const dxg_exposure& e = beam->exposures[i];
+        if (e.spectrum_index >= beam->n_spectra)
+            return fail(DXG_ERR_ARGUMENT, "exposure " + std::to_string(i) + " references a missing spectrum");
+        if (e.bowtie_index >= 0 && static_cast<uint64_t>(e.bowtie_index) >= beam->n_bowties)
+            return fail(DXG_ERR_ARGUMENT, "exposure " + std::to_string(i) + " references a missing bowtie");
+        if (!(std::fabs(e.collimation_half_angles[0]) < 1.5707963) || !(std::fabs(e.collimation_half_angles[1]) < 1.5707963))
+            return fail(DXG_ERR_ARGUMENT, "collimation half angles must be below pi/2");
+        dxg::DevExposure& d = ex[i];
+        std::memset(&d, 0, sizeof d);
+        const double* c0 = e.cosines;
+        const double* c1 = e.cosines + 3;
+        const double ax[3] = { c0[1] * c1[2] - c0[2] * c1[1], c0[2] * c1[0] - c0[0] * c1[2], c0[0] * c1[1] - c0[1] * c1[0] };
+        const double axn = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+        if (!(axn > 1e-12))
+            return fail(DXG_ERR_ARGUMENT, "exposure direction cosines are parallel");
+        for (int a = 0; a < 3; ++a) {
+            d.pos[a] = static_cast<float>(e.position[a]);
+            d.c0[a] = static_cast<float>(c0[a]);
+            d.c1[a] = static_cast<float>(c1[a]);
+            d.axis[a] = static_cast<float>(ax[a] / axn);
+        }
+        d.half_angle[0] = static_cast<float>(e.collimation_half_angles[0]);
+        d.half_angle[1] = static_cast<float>(e.collimation_half_angles[1]);
+        d.weight = static_cast<float>(e.weight);
+        d.spectrum = static_cast<int32_t>(e.spectrum_index);
+        d.bowtie = e.bowtie_index;
+        ex_start[i] = total;
+        total += e.n_particles;
+    }
+    if (total == 0)
+        return fail(DXG_ERR_ARGUMENT, "beam has zero particles");
+    HIP_TRY(w->d_ex.upload(ex, w->stream));
+    HIP_TRY(w->d_ex_start.upload(ex_start, w->stream));
+    HIP_TRY(w->d_spec_e.upload(se, w->stream));
+    HIP_TRY(w->d_spec_p.upload(sp, w->stream));
+    HIP_TRY(w->d_spec_alias.upload(sa, w->stream));
+    HIP_TRY(w->d_spec_off.upload(soff, w->stream));
+    HIP_TRY(w->d_spec_n.upload(sn, w->stream));
+    HIP_TRY(w->d_bow_a.upload(ba, w->stream));
+    HIP_TRY(w->d_bow_w.upload(bw, w->stream));
+    if (boff.empty()) {
+        boff.push_back(0);
+        bn.push_back(1);
+    }
+    HIP_TRY(w->d_bow_off.upload(boff, w->stream));
+    HIP_TRY(w->d_bow_n.upload(bn, w->stream));
+
+    dxg::DevParams p;
+    fillParams(w, p);
+    p.exposures = w->d_ex.ptr;
+    p.ex_start = w->d_ex_start.ptr;
+    p.n_ex = static_cast<int>(beam->n_exposures);
+    p.spec_energy = w->d_spec_e.ptr;
+    p.spec_prob = w->d_spec_p.ptr;
+    p.spec_alias = w->d_spec_alias.ptr;
+    p.spec_off = w->d_spec_off.ptr;
+    p.spec_n = w->d_spec_n.ptr;
+    p.bow_angle = w->d_bow_a.ptr;
+    p.bow_weight = w->d_bow_w.ptr;
+    p.bow_off = w->d_bow_off.ptr;
+    p.bow_n = w->d_bow_n.ptr;
+    p.seed = seed;
+    p.rng_offset = w->histories;
+
+    // ---- launch geometry: fill every CU to the occupancy limit
+    int cus = 0, per_cu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, w->device));
+    const int block = dxg::transportBlockSize();
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dxg::transportKernelSymbol(), block, 0));
+    per_cu = std::max(per_cu, 1);
+    int blocks = cus * per_cu;
+    if (const char* env = std::getenv("DXG_BLOCKS"))
+        blocks = std::max(1, std::atoi(env));
+    const uint64_t lanes = static_cast<uint64_t>(blocks) * block;
+    uint64_t chunk = std::max<uint64_t>(lanes * 64, 1ull << 20);
+    if (const char* env = std::getenv("DXG_CHUNK"))
+        chunk = std::max<uint64_t>(1, std::strtoull(env, nullptr, 10));
+
+    if (progress) {
+        progress->stop.store(false);
+        progress->n.store(0);
+        progress->total.store(total);
+        progress->setMessage("Simulating " + std::to_string(total) + " histories");
+    }
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    dxg_counters acc {};
+    int status = DXG_OK;
+    for (uint64_t h0 = 0; h0 < total; h0 += chunk) {
+        if (progress && progress->stop.load()) {
+            status = fail(DXG_ERR_CANCELLED, "transport cancelled by the progress stop flag");
+            break;
+        }
+        p.h_begin = h0;
+        p.h_end = std::min(total, h0 + chunk);
+        dxg::launchReset(w->d_pool.ptr, w->d_counters.ptr, w->stream);
+        HIP_TRY(hipEventRecord(ev0, w->stream));
+        dxg::launchTransport(p, blocks, block, w->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ev1, w->stream));
+        dxg::DevCounters c {};
+        HIP_TRY(hipMemcpyAsync(&c, w->d_counters.ptr, sizeof c, hipMemcpyDeviceToHost, w->stream));
+        HIP_TRY(hipStreamSynchronize(w->stream));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+        acc.kernel_ms += ms;
+        acc.kernel_launches += 1;
+        acc.histories += c.histories;
+        acc.steps += c.steps;
+        acc.real_events += c.real_events;
+        acc.deposits += c.deposits;
+        acc.emitted_energy += c.emitted;
+        acc.escaped_energy += c.escaped;
+        acc.deposited_energy += c.deposited;
+        w->histories += p.h_end - p.h_begin;
+        if (progress)
+            progress->n.store(p.h_end);
+    }
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    w->counters = acc;
+    if (progress && status == DXG_OK)
+        progress->setMessage("Simulation finished");
+    return status;
+}
+
+int dxg_read_energy_scores(const dxg_world* w, double* sum_e, double* sum_e2, uint64_t* n_events)
+{
+    if (!w || !w->built)
+        return fail(DXG_ERR_STATE, "dxg_read_energy_scores: world not built");
+    const uint64_t n = w->size();
+    std::vector<double> t(4 * n);
+    HIP_TRY(hipMemcpyAsync(t.data(), w->d_tally.ptr, t.size() * sizeof(double), hipMemcpyDeviceToHost, w->stream));
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    for (uint64_t i = 0; i < n; ++i) {
+        if (sum_e)
+            sum_e[i] = t[4 * i];
+        if (sum_e2)
+            sum_e2[i] = t[4 * i + 1];
+        if (n_events) {
+            uint64_t c;
+            std::memcpy(&c, &t[4 * i + 2], sizeof c);
+            n_events[i] = c;
+        }
+    }
+    return DXG_OK;
+}
+
+int dxg_read_dose_scores(const dxg_world* w, double* dose_mgy, double* variance_mgy2, uint64_t* n_events)
+{
+    if (!w || !w->built)
+        return fail(DXG_ERR_STATE, "dxg_read_dose_scores: world not built");
+    const uint64_t n = w->size();
+    std::vector<double> e(n), e2(n);
+    int rc = dxg_read_energy_scores(w, e.data(), e2.data(), n_events);
+    if (rc != DXG_OK)
+        return rc;
+    const double voxel_cm3 = w->spacing[0] * w->spacing[1] * w->spacing[2];
+    const double H = static_cast<double>(w->histories);
+    for (uint64_t i = 0; i < n; ++i) {
+        const double mass_kg = w->density[i] * voxel_cm3 * 1e-3;
+        const double f = mass_kg > 0 ? dxg::kKeVToJ / mass_kg * 1e3 : 0.0;   // keV -> mGy
+        if (dose_mgy)
+            dose_mgy[i] = e[i] * f;
+        if (variance_mgy2) {
+            double v = 0;
+            if (H > 1)
+                v = std::max(0.0, H / (H - 1.0) * (e2[i] - e[i] * e[i] / H));
+            variance_mgy2[i] = v * f * f;
+        }
+    }
+    return DXG_OK;
+}
+
+int dxg_get_counters(const dxg_world* w, dxg_counters* out)
+{
+    if (!w || !out)
+        return fail(DXG_ERR_ARGUMENT, "dxg_get_counters: null argument");
+    *out = w->counters;
+    return DXG_OK;
+}
+
+int dxg_progress_create(dxg_progress** out)
+{
+    if (!out)
+        return fail(DXG_ERR_ARGUMENT, "dxg_progress_create: null output pointer");
+    *out = new dxg_progress();
+    return DXG_OK;
+}
+
+void dxg_progress_destroy(dxg_progress* p) { delete p; }
+
+void dxg_progress_request_stop(dxg_progress* p)
+{
+    if (p)
+        p->stop.store(true);
+}
+
+int dxg_progress_continue(const dxg_progress* p) { return p && !p->stop.load() ? 1 : 0; }
+
+void dxg_progress_get(const dxg_progress* p, uint64_t* n, uint64_t* total)
+{
+    if (n)
+        *n = p ? p->n.load() : 0;
+    if (total)
+        *total = p ? p->total.load() : 0;
+}
+
+void dxg_progress_message(const dxg_progress* p, char* buf, uint64_t cap)
+{
+    if (!buf || cap == 0)
+        return;
+    std::string m;
+    if (p) {
+        std::lock_guard<std::mutex> lk(p->msg_mutex);
+        m = p->msg;
+    }
+    const size_t k = std::min<size_t>(m.size(), cap - 1);
+    std::memcpy(buf, m.data(), k);
+    buf[k] = '\0';
+}
+
+int dxg_material_attenuation(const dxg_material* m, const double* energy_kev, uint64_t n, double* out)
+{
+    if (!m || (!energy_kev && n) || (!out && n))
+        return fail(DXG_ERR_ARGUMENT, "dxg_material_attenuation: null argument");
+    std::string err;
+    const dxg::PhysicsList* pl = physics(err);
+    if (!pl)
+        return fail(DXG_ERR_PHYSICS, err);
+    dxg::MixMaterial mm;
+    if (!dxg::makeMaterial(*pl, m->Z, m->mass_fraction, m->n, mm, err))
+        return fail(DXG_ERR_MATERIAL, err);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (!(energy_kev[i] > 0))
+            return fail(DXG_ERR_ARGUMENT, "energy must be positive");
+        double pe, inc, coh;
+        mm.attenuation(*pl, energy_kev[i], pe, inc, coh);
+        out[4 * i] = pe;
+        out[4 * i + 1] = inc;
+        out[4 * i + 2] = coh;
+        out[4 * i + 3] = pe + inc + coh;
+    }
+    return DXG_OK;
+}
+
+int dxg_debug_traverse(const dxg_world* w, uint64_t n_rays, const float* pos, const float* dir, uint64_t n_steps,
+                       const float* steps, int64_t* out_index, int32_t* out_material)
+{
+    if (!w || !w->built)
+        return fail(DXG_ERR_STATE, "dxg_debug_traverse: world not built");
+    if (n_rays == 0 || n_steps == 0)
+        return DXG_OK;
+    if (!pos || !dir || !steps || !out_index || !out_material)
+        return fail(DXG_ERR_ARGUMENT, "dxg_debug_traverse: null argument");
+    HIP_TRY(hipSetDevice(w->device));
+    DevBuf<float> dp, dd, ds;
+    DevBuf<int64_t> di;
+    DevBuf<int32_t> dm;
+    HIP_TRY(dp.upload(std::vector<float>(pos, pos + 3 * n_rays), w->stream));
+    HIP_TRY(dd.upload(std::vector<float>(dir, dir + 3 * n_rays), w->stream));
+    HIP_TRY(ds.upload(std::vector<float>(steps, steps + n_rays * n_steps), w->stream));
+    HIP_TRY(di.alloc(n_rays * n_steps));
+    HIP_TRY(dm.alloc(n_rays * n_steps));
+    dxg::DevParams p;
+    fillParams(w, p);
+    dxg::launchTraverse(p, n_rays, dp.ptr, dd.ptr, n_steps, ds.ptr, di.ptr, dm.ptr, w->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out_index, di.ptr, n_rays * n_steps * sizeof(int64_t), hipMemcpyDeviceToHost, w->stream));
+    HIP_TRY(hipMemcpyAsync(out_material, dm.ptr, n_rays * n_steps * sizeof(int32_t), hipMemcpyDeviceToHost, w->stream));
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    return DXG_OK;
+}
+
+}  // extern "C"

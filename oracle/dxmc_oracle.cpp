@@ -1,0 +1,881 @@
+// Oracle for the voxel photon-transport path.  TEST INFRASTRUCTURE ONLY:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it,
+// as the checker or as the timed CPU baseline; the product never links it.
+//
+// It is an FP64, multi-threaded (std::thread) C++ restatement of the DXMClib
+// voxel dose path that OpenDXMC drives from SimulationPipeline::worker<C>
+// (simulationpipeline.cpp:124-235).  DXMClib itself is not in the reference
+// tree (fetched at configure time, CMakeLists.txt:71-76), so every algorithm
+// here is written from the published method named beside it:
+//   * Material::byWeight mixture rule (simulationpipeline.cpp:133-145):
+//     mass-fraction weighted element cross sections.
+//   * Woodcock (delta) tracking through AAVoxelGrid (Woodcock et al. 1965),
+//     majorant mu_max(E) = max_m mu_m(E) * rho_max_m, evaluated exactly at E.
+//   * Photoelectric absorption (KERMA: the photon energy is scored locally).
+//   * Compton: Klein-Nishina by the Butcher & Messel (1960) method; correction
+//     level 1 adds rejection on S(x,Z)/Z (incoherent scattering function).
+//   * Rayleigh: Thomson (level 0, exact inverse of the cubic CDF); level 1
+//     samples x^2 from F^2(x) and accepts with (1+cos^2)/2.
+//   * Scoring: per voxel sum E, sum E^2 and event count (EnergyScore), dose =
+//     E / (rho V) in mGy (DoseScore, simulationpipeline.cpp:174-234).
+//   * Exposures and particle sampling follow the geometry conventions of
+//     beamactorcontainer.cpp:44-69 (axis = cos0 x cos1, sin-parameterised
+//     collimation) and dxmc_specialization.cpp:21-90 (DX geometry).
+// Random numbers: xoshiro128+ per history, seeded by splitmix64(seed, history).
+// Physics data: the physics list written by opendxmc_amd/physics/gen_physics_list.py.
+//
+// Correction level 2 (impulse approximation + fluorescence) is not restated.
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr double NA = 6.02214076e23;
+constexpr double HC_KEV_ANG = 12.398419843320026;
+constexpr double ME_C2_KEV = 510.99895000;
+constexpr double PI = 3.14159265358979323846;
+constexpr double KEV_TO_J = 1.602176634e-16;
+constexpr double E_CUTOFF_KEV = 1.0;
+
+// ---------------------------------------------------------------- physics list
+struct Element {
+    double A = 0;
+    std::vector<double> inc, coh, F, S, pe_e, pe;
+};
+
+struct PhysicsList {
+    bool loaded = false;
+    std::vector<double> egrid, xgrid, log_egrid;
+    std::vector<Element> el;   // index Z-1
+};
+
+PhysicsList g_phys;
+std::mutex g_phys_mtx;
+thread_local std::string g_err;
+
+bool read_exact(std::ifstream& f, void* p, size_t n)
+{
+    f.read(static_cast<char*>(p), static_cast<std::streamsize>(n));
+    return static_cast<size_t>(f.gcount()) == n;
+}
+
+bool load_physics(const char* path)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        g_err = std::string("cannot open physics list ") + path;
+        return false;
+    }
+    char magic[8];
+    uint32_t hdr[3];
+    if (!read_exact(f, magic, 8) || std::memcmp(magic, "DXGPHYS1", 8) != 0 || !read_exact(f, hdr, sizeof hdr)) {
+        g_err = "bad physics list header";
+        return false;
+    }
+    PhysicsList p;
+    const uint32_t zmax = hdr[0], ne = hdr[1], nx = hdr[2];
+    p.egrid.resize(ne);
+    p.xgrid.resize(nx);
+    if (!read_exact(f, p.egrid.data(), ne * 8) || !read_exact(f, p.xgrid.data(), nx * 8)) {
+        g_err = "truncated physics list";
+        return false;
+    }
+    p.log_egrid.resize(ne);
+    for (uint32_t i = 0; i < ne; ++i)
+        p.log_egrid[i] = std::log(p.egrid[i]);
+    p.el.resize(zmax);
+    for (uint32_t z = 0; z < zmax; ++z) {
+        Element& e = p.el[z];
+        e.inc.resize(ne);
+        e.coh.resize(ne);
+        e.F.resize(nx);
+        e.S.resize(nx);
+        uint32_t npe = 0, nsh = 0;
+        bool ok = read_exact(f, &e.A, 8) && read_exact(f, e.inc.data(), ne * 8) && read_exact(f, e.coh.data(), ne * 8)
+            && read_exact(f, e.F.data(), nx * 8) && read_exact(f, e.S.data(), nx * 8) && read_exact(f, &npe, 4);
+        if (ok) {
+            e.pe_e.resize(npe);
+            e.pe.resize(npe);
+            ok = read_exact(f, e.pe_e.data(), npe * 8) && read_exact(f, e.pe.data(), npe * 8) && read_exact(f, &nsh, 4);
+        }
+        if (ok) {
+            std::vector<double> sh(nsh * 5);
+            ok = read_exact(f, sh.data(), sh.size() * 8);
+        }
+        if (!ok) {
+            g_err = "truncated physics list element block";
+            return false;
+        }
+    }
+    p.loaded = true;
+    g_phys = std::move(p);
+    return true;
+}
+
+// log-log interpolation on an ascending table; ties in e mark an edge.
+double loglog(const std::vector<double>& e, const std::vector<double>& v, double E)
+{
+    const size_t n = e.size();
+    if (E <= e[0])
+        return v[0];
+    if (E >= e[n - 1])
+        return v[n - 1];
+    // first index with e[k] > E
+    size_t hi = static_cast<size_t>(std::upper_bound(e.begin(), e.end(), E) - e.begin());
+    size_t lo = hi - 1;
+    const double e0 = e[lo], e1 = e[hi];
+    const double v0 = std::max(v[lo], 1e-300), v1 = std::max(v[hi], 1e-300);
+    const double t = std::log(E / e0) / std::log(e1 / e0);
+    return std::exp(std::log(v0) + t * (std::log(v1) - std::log(v0)));
+}
+
+double element_pe(int z, double E) { return loglog(g_phys.el[z - 1].pe_e, g_phys.el[z - 1].pe, E); }
+double element_inc(int z, double E) { return loglog(g_phys.egrid, g_phys.el[z - 1].inc, E); }
+double element_coh(int z, double E) { return loglog(g_phys.egrid, g_phys.el[z - 1].coh, E); }
+
+double lin_x(const std::vector<double>& xg, const std::vector<double>& v, double x)
+{
+    if (x <= xg[0])
+        return v[0];
+    if (x >= xg.back())
+        return v.back();
+    size_t hi = static_cast<size_t>(std::upper_bound(xg.begin(), xg.end(), x) - xg.begin());
+    size_t lo = hi - 1;
+    const double t = (x - xg[lo]) / (xg[hi] - xg[lo]);
+    return v[lo] + t * (v[hi] - v[lo]);
+}
+
+// ------------------------------------------------------------------ material
+struct Material {
+    std::vector<int> Z;
+    std::vector<double> atoms_per_gram;   // w_i N_A / A_i
+    double electrons_per_gram = 0;        // S(x -> inf) per gram
+    // Rayleigh level-1 sampling table: cumulative of F^2 in u = x^2
+    std::vector<double> u_grid, a_cum;
+
+    void attenuation(double E, double out[3]) const   // cm^2/g: pe, inc, coh
+    {
+        out[0] = out[1] = out[2] = 0;
+        for (size_t i = 0; i < Z.size(); ++i) {
+            const double f = atoms_per_gram[i] * 1e-24;
+            out[0] += f * element_pe(Z[i], E);
+            out[1] += f * element_inc(Z[i], E);
+            out[2] += f * element_coh(Z[i], E);
+        }
+    }
+    double S(double x) const
+    {
+        double s = 0;
+        for (size_t i = 0; i < Z.size(); ++i)
+            s += atoms_per_gram[i] * lin_x(g_phys.xgrid, g_phys.el[Z[i] - 1].S, x);
+        return s;
+    }
+    double F2(double x) const
+    {
+        double s = 0;
+        for (size_t i = 0; i < Z.size(); ++i) {
+            const double f = lin_x(g_phys.xgrid, g_phys.el[Z[i] - 1].F, x);
+            s += atoms_per_gram[i] * f * f;
+        }
+        return s;
+    }
+};
+
+constexpr double X_TABLE_MAX = 16.0;   // 1/Angstrom; covers E <= 198 keV
+constexpr int RAYLEIGH_FINE = 8192;
+
+bool make_material(const int* Z, const double* w, int n, Material& m)
+{
+    double sum = 0;
+    for (int i = 0; i < n; ++i) {
+        if (Z[i] < 1 || Z[i] > static_cast<int>(g_phys.el.size()) || !(w[i] >= 0) || !std::isfinite(w[i])) {
+            g_err = "material: bad element or weight";
+            return false;
+        }
+        sum += w[i];
+    }
+    if (!(sum > 0)) {
+        g_err = "material: weights sum to zero";
+        return false;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (w[i] <= 0)
+            continue;
+        m.Z.push_back(Z[i]);
+        const double wn = w[i] / sum;
+        m.atoms_per_gram.push_back(wn * NA / g_phys.el[Z[i] - 1].A);
+        m.electrons_per_gram += wn * NA / g_phys.el[Z[i] - 1].A * Z[i];
+    }
+    // Rayleigh: A(u) = int_0^u F^2(sqrt(u')) du' on a fine grid in sqrt(u)
+    m.u_grid.resize(RAYLEIGH_FINE + 1);
+    m.a_cum.resize(RAYLEIGH_FINE + 1);
+    double prev_f2 = m.F2(0.0);
+    m.u_grid[0] = 0;
+    m.a_cum[0] = 0;
+    for (int k = 1; k <= RAYLEIGH_FINE; ++k) {
+        const double x = X_TABLE_MAX * k / RAYLEIGH_FINE;
+        const double u = x * x;
+        const double f2 = m.F2(x);
+        m.u_grid[k] = u;
+        m.a_cum[k] = m.a_cum[k - 1] + 0.5 * (f2 + prev_f2) * (u - m.u_grid[k - 1]);
+        prev_f2 = f2;
+    }
+    return true;
+}
+
+// ----------------------------------------------------------------------- rng
+struct Rng {
+    uint32_t s[4];
+    static uint64_t splitmix(uint64_t& x)
+    {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    Rng(uint64_t seed, uint64_t history)
+    {
+        uint64_t x = seed ^ (history * 0xD1B54A32D192ED03ull);
+        const uint64_t a = splitmix(x), b = splitmix(x);
+        s[0] = static_cast<uint32_t>(a);
+        s[1] = static_cast<uint32_t>(a >> 32);
+        s[2] = static_cast<uint32_t>(b);
+        s[3] = static_cast<uint32_t>(b >> 32);
+        if ((s[0] | s[1] | s[2] | s[3]) == 0)
+            s[0] = 1;
+    }
+    uint32_t next()   // xoshiro128+
+    {
+        const uint32_t r = s[0] + s[3];
+        const uint32_t t = s[1] << 9;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = (s[3] << 11) | (s[3] >> 21);
+        return r;
+    }
+    // uniform in (0, 1]
+    double u() { return ((next() >> 8) + 1) * (1.0 / 16777216.0); }
+};
+
+// --------------------------------------------------------------- interfaces
+}  // namespace
+
+extern "C" {
+
+struct oracle_exposure {
+    double position[3];
+    double cosines[6];
+    double collimation_half_angles[2];
+    double weight;
+    uint32_t spectrum_index;
+    int32_t bowtie_index;
+    uint64_t n_particles;
+};
+
+struct oracle_spectrum {
+    const double* energy;
+    const double* weight;
+    uint64_t n;
+};
+
+struct oracle_bowtie {
+    const double* angle;
+    const double* weight;
+    uint64_t n;
+};
+
+struct oracle_material {
+    const int* Z;
+    const double* weight;
+    uint64_t n;
+};
+
+struct oracle_counters {
+    uint64_t histories;
+    uint64_t steps;          // Woodcock steps, real + virtual
+    uint64_t real_events;    // real interactions
+    uint64_t deposits;       // scored deposits
+    double emitted_energy;   // sum w E over emitted photons entering or missing the grid
+    double escaped_energy;   // sum w E leaving the grid box
+    double deposited_energy; // sum of scored w dE
+};
+
+}  // extern "C"
+
+namespace {
+
+struct Grid {
+    size_t dims[3];
+    double spacing[3];
+    double box_min[3], box_max[3];
+    std::vector<double> density;
+    std::vector<uint8_t> material;
+    std::vector<Material> mats;
+    std::vector<double> rho_max;   // per material
+};
+
+struct Spectrum {
+    std::vector<double> e, cdf;
+    double sample(double u) const
+    {
+        const size_t k = static_cast<size_t>(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
+        return e[std::min(k, e.size() - 1)];
+    }
+};
+
+struct Bowtie {
+    std::vector<double> a, w;
+    double eval(double ang) const
+    {
+        const double x = std::fabs(ang);
+        if (x <= a.front())
+            return w.front();
+        if (x >= a.back())
+            return w.back();
+        const size_t hi = static_cast<size_t>(std::upper_bound(a.begin(), a.end(), x) - a.begin());
+        const size_t lo = hi - 1;
+        const double t = (x - a[lo]) / (a[hi] - a[lo]);
+        return w[lo] + t * (w[hi] - w[lo]);
+    }
+};
+
+Bowtie make_bowtie(const oracle_bowtie& b)
+{
+    std::vector<std::pair<double, double>> p;
+    for (uint64_t i = 0; i < b.n; ++i)
+        p.emplace_back(std::fabs(b.angle[i]), b.weight[i]);
+    std::sort(p.begin(), p.end());
+    Bowtie r;
+    for (auto& q : p) {
+        r.a.push_back(q.first);
+        r.w.push_back(q.second);
+    }
+    // normalise to the central ray
+    const double w0 = r.eval(0.0);
+    for (auto& v : r.w)
+        v /= w0;
+    return r;
+}
+
+struct Particle {
+    double p[3], d[3], E, w;
+};
+
+bool box_entry(const Grid& g, Particle& ph)
+{
+    double t0 = 0, t1 = 1e300;
+    for (int a = 0; a < 3; ++a) {
+        if (ph.d[a] != 0) {
+            const double inv = 1.0 / ph.d[a];
+            double ta = (g.box_min[a] - ph.p[a]) * inv;
+            double tb = (g.box_max[a] - ph.p[a]) * inv;
+            if (ta > tb)
+                std::swap(ta, tb);
+            t0 = std::max(t0, ta);
+            t1 = std::min(t1, tb);
+        } else if (ph.p[a] < g.box_min[a] || ph.p[a] > g.box_max[a]) {
+            return false;
+        }
+    }
+    if (t0 > t1)
+        return false;
+    for (int a = 0; a < 3; ++a)
+        ph.p[a] += t0 * ph.d[a];
+    return true;
+}
+
+long long voxel(const Grid& g, const double p[3])
+{
+    long long idx[3];
+    for (int a = 0; a < 3; ++a) {
+        const double f = std::floor((p[a] - g.box_min[a]) / g.spacing[a]);
+        if (!(f >= 0) || f >= static_cast<double>(g.dims[a]))
+            return -1;
+        idx[a] = static_cast<long long>(f);
+    }
+    return idx[0] + static_cast<long long>(g.dims[0]) * (idx[1] + static_cast<long long>(g.dims[1]) * idx[2]);
+}
+
+void rotate(double d[3], double cost, double phi)
+{
+    const double sint = std::sqrt(std::max(0.0, 1.0 - cost * cost));
+    const double cp = std::cos(phi), sp = std::sin(phi);
+    const double dz = d[2];
+    double n[3];
+    if (std::fabs(dz) > 0.99999999) {
+        n[0] = sint * cp;
+        n[1] = sint * sp;
+        n[2] = cost * (dz > 0 ? 1.0 : -1.0);
+    } else {
+        const double t = std::sqrt(1.0 - dz * dz);
+        n[0] = d[0] * cost + sint * (d[0] * dz * cp - d[1] * sp) / t;
+        n[1] = d[1] * cost + sint * (d[1] * dz * cp + d[0] * sp) / t;
+        n[2] = dz * cost - t * sint * cp;
+    }
+    const double nn = 1.0 / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int a = 0; a < 3; ++a)
+        d[a] = n[a] * nn;
+}
+
+// Klein-Nishina by Butcher & Messel; returns cos(theta), sets eps = E'/E.
+double sample_kn(double k, Rng& r, double& eps)
+{
+    const double e0 = 1.0 / (1.0 + 2.0 * k);
+    const double a1 = std::log(1.0 + 2.0 * k);
+    const double a2 = 0.5 * (1.0 - e0 * e0);
+    for (;;) {
+        const double r1 = r.u(), r2 = r.u(), r3 = r.u();
+        if (r1 * (a1 + a2) < a1)
+            eps = std::exp(-a1 * r2);
+        else
+            eps = std::sqrt(e0 * e0 + (1.0 - e0 * e0) * r2);
+        const double t = (1.0 - eps) / (k * eps);
+        const double sin2 = t * (2.0 - t);
+        const double g = 1.0 - eps * sin2 / (1.0 + eps * eps);
+        if (r3 <= g)
+            return 1.0 - t;
+    }
+}
+
+double sample_thomson(Rng& r)
+{
+    const double q = 4.0 - 8.0 * r.u();
+    const double a = -0.5 * q + std::sqrt(0.25 * q * q + 1.0);
+    const double u = std::cbrt(a);
+    return std::clamp(u - 1.0 / u, -1.0, 1.0);
+}
+
+double sample_rayleigh_ff(const Material& m, double E, Rng& r)
+{
+    const double xmax = E / HC_KEV_ANG;
+    const double umax = xmax * xmax;
+    // A(umax)
+    const auto& ug = m.u_grid;
+    const auto& ac = m.a_cum;
+    auto a_of_u = [&](double u) {
+        if (u >= ug.back())
+            return ac.back();
+        const size_t hi = static_cast<size_t>(std::upper_bound(ug.begin(), ug.end(), u) - ug.begin());
+        const size_t lo = hi - 1;
+        const double t = (u - ug[lo]) / (ug[hi] - ug[lo]);
+        return ac[lo] + t * (ac[hi] - ac[lo]);
+    };
+    const double amax = a_of_u(umax);
+    for (;;) {
+        const double target = r.u() * amax;
+        const size_t hi = std::min(static_cast<size_t>(std::lower_bound(ac.begin(), ac.end(), target) - ac.begin()), ac.size() - 1);
+        const size_t lo = hi == 0 ? 0 : hi - 1;
+        double u = ug[hi];
+        if (hi > 0 && ac[hi] > ac[lo])
+            u = ug[lo] + (target - ac[lo]) / (ac[hi] - ac[lo]) * (ug[hi] - ug[lo]);
+        u = std::min(u, umax);
+        const double cost = 1.0 - 2.0 * u / umax;
+        if (r.u() * 2.0 <= 1.0 + cost * cost)
+            return cost;
+    }
+}
+
+struct Run {
+    const Grid* g;
+    std::vector<oracle_exposure> ex;
+    std::vector<Spectrum> spectra;
+    std::vector<Bowtie> bowties;
+    std::vector<uint64_t> ex_start;   // prefix sum of particles per exposure
+    uint64_t total = 0;
+    int level = 1;
+    uint64_t seed = 1;
+    uint64_t offset = 0;   // added to the history index that keys the RNG
+    double* sum_e;
+    double* sum_e2;
+    uint64_t* n_ev;
+};
+
+struct Acc {
+    uint64_t steps = 0, real = 0, deposits = 0;
+    double emitted = 0, escaped = 0, deposited = 0;
+};
+
+inline void atomic_add(double* p, double v) { std::atomic_ref<double>(*p).fetch_add(v, std::memory_order_relaxed); }
+inline void atomic_add(uint64_t* p, uint64_t v) { std::atomic_ref<uint64_t>(*p).fetch_add(v, std::memory_order_relaxed); }
+
+void score(const Run& R, long long i, double e, Acc& acc)
+{
+    atomic_add(R.sum_e + i, e);
+    atomic_add(R.sum_e2 + i, e * e);
+    atomic_add(R.n_ev + i, 1ull);
+    acc.deposits++;
+    acc.deposited += e;
+}
+
+void history(const Run& R, uint64_t h, Acc& acc)
+{
+    const Grid& g = *R.g;
+    Rng rng(R.seed, h + R.offset);
+    const size_t ei = static_cast<size_t>(std::upper_bound(R.ex_start.begin(), R.ex_start.end(), h) - R.ex_start.begin()) - 1;
+    const oracle_exposure& ex = R.ex[ei];
+
+    // ---- sample the source particle (Exposure::sampleParticle)
+    Particle ph;
+    const double ax = ex.collimation_half_angles[0] * (2.0 * rng.u() - 1.0);
+    const double ay = ex.collimation_half_angles[1] * (2.0 * rng.u() - 1.0);
+    const double sx = std::sin(ax), sy = std::sin(ay);
+    const double sz = std::sqrt(std::max(0.0, 1.0 - sx * sx - sy * sy));
+    const double* c0 = ex.cosines;
+    const double* c1 = ex.cosines + 3;
+    const double n[3] = { c0[1] * c1[2] - c0[2] * c1[1], c0[2] * c1[0] - c0[0] * c1[2], c0[0] * c1[1] - c0[1] * c1[0] };
+    double dl = 0;
+    for (int a = 0; a < 3; ++a) {
+        ph.d[a] = c0[a] * sx + c1[a] * sy + n[a] * sz;
+        dl += ph.d[a] * ph.d[a];
+    }
+    dl = 1.0 / std::sqrt(dl);
+    for (int a = 0; a < 3; ++a) {
+        ph.d[a] *= dl;
+        ph.p[a] = ex.position[a];
+    }
+    ph.E = R.spectra[ex.spectrum_index].sample(rng.u());
+    ph.w = ex.weight;
+    if (ex.bowtie_index >= 0)
+        ph.w *= R.bowties[static_cast<size_t>(ex.bowtie_index)].eval(ax);
+    acc.emitted += ph.w * ph.E;
+
+    if (!box_entry(g, ph)) {
+        acc.escaped += ph.w * ph.E;
+        return;
+    }
+
+    const size_t nm = g.mats.size();
+    std::vector<double> mu_pe(nm), mu_inc(nm), mu_coh(nm);
+    double maj = 0, last_E = -1;
+    for (;;) {
+        if (ph.E != last_E) {
+            maj = 0;
+            for (size_t m = 0; m < nm; ++m) {
+                double a[3];
+                g.mats[m].attenuation(ph.E, a);
+                mu_pe[m] = a[0];
+                mu_inc[m] = a[1];
+                mu_coh[m] = a[2];
+                maj = std::max(maj, (a[0] + a[1] + a[2]) * g.rho_max[m]);
+            }
+            last_E = ph.E;
+        }
+        const double s = -std::log(rng.u()) / maj;
+        for (int a = 0; a < 3; ++a)
+            ph.p[a] += s * ph.d[a];
+        acc.steps++;
+        const long long vi = voxel(g, ph.p);
+        if (vi < 0) {
+            acc.escaped += ph.w * ph.E;
+            return;
+        }
+        const uint8_t m = g.material[static_cast<size_t>(vi)];
+        const double rho = g.density[static_cast<size_t>(vi)];
+        const double mtot = mu_pe[m] + mu_inc[m] + mu_coh[m];
+        if (rng.u() * maj > mtot * rho)
+            continue;   // virtual collision
+        acc.real++;
+        const double r = rng.u() * mtot;
+        if (r < mu_pe[m]) {
+            score(R, vi, ph.w * ph.E, acc);
+            return;
+        } else if (r < mu_pe[m] + mu_inc[m]) {
+            const double k = ph.E / ME_C2_KEV;
+            double eps = 1, cost = 1;
+            for (;;) {
+                cost = sample_kn(k, rng, eps);
+                if (R.level == 0)
+                    break;
+                const double x = ph.E / HC_KEV_ANG * std::sqrt(0.5 * (1.0 - cost));
+                if (rng.u() * g.mats[m].electrons_per_gram <= g.mats[m].S(x))
+                    break;
+            }
+            const double Enew = ph.E * eps;
+            score(R, vi, ph.w * (ph.E - Enew), acc);
+            ph.E = Enew;
+            rotate(ph.d, cost, 2.0 * PI * rng.u());
+            if (ph.E < E_CUTOFF_KEV) {
+                score(R, vi, ph.w * ph.E, acc);
+                return;
+            }
+        } else {
+            const double cost = R.level == 0 ? sample_thomson(rng) : sample_rayleigh_ff(g.mats[m], ph.E, rng);
+            rotate(ph.d, cost, 2.0 * PI * rng.u());
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error(void) { return g_err.c_str(); }
+
+int oracle_load_physics(const char* path)
+{
+    std::lock_guard<std::mutex> lk(g_phys_mtx);
+    return load_physics(path) ? 0 : -1;
+}
+
+// Mass attenuation (cm^2/g) of a mixture: out[4*i + {0,1,2,3}] = pe, inc, coh, total.
+int oracle_material_attenuation(const oracle_material* mat, const double* energy, uint64_t n, double* out)
+{
+    if (!g_phys.loaded) {
+        g_err = "physics list not loaded";
+        return -1;
+    }
+    Material m;
+    if (!make_material(mat->Z, mat->weight, static_cast<int>(mat->n), m))
+        return -2;
+    for (uint64_t i = 0; i < n; ++i) {
+        double a[3];
+        m.attenuation(energy[i], a);
+        out[4 * i + 0] = a[0];
+        out[4 * i + 1] = a[1];
+        out[4 * i + 2] = a[2];
+        out[4 * i + 3] = a[0] + a[1] + a[2];
+    }
+    return 0;
+}
+
+// Material S(x) and F^2(x) per gram (for sampling tests).
+int oracle_material_sf(const oracle_material* mat, const double* x, uint64_t n, double* s_out, double* f2_out, double* electrons_per_gram)
+{
+    if (!g_phys.loaded)
+        return -1;
+    Material m;
+    if (!make_material(mat->Z, mat->weight, static_cast<int>(mat->n), m))
+        return -2;
+    for (uint64_t i = 0; i < n; ++i) {
+        s_out[i] = m.S(x[i]);
+        f2_out[i] = m.F2(x[i]);
+    }
+    *electrons_per_gram = m.electrons_per_gram;
+    return 0;
+}
+
+// Draw n samples of cos(theta) from the interaction samplers (statistical tests).
+// kind: 0 = Klein-Nishina, 1 = KN x S(x), 2 = Thomson, 3 = Thomson x F^2
+int oracle_sample_angles(const oracle_material* mat, double E, int kind, uint64_t n, uint64_t seed, double* cost_out, double* eps_out)
+{
+    if (!g_phys.loaded)
+        return -1;
+    Material m;
+    if (!make_material(mat->Z, mat->weight, static_cast<int>(mat->n), m))
+        return -2;
+    for (uint64_t i = 0; i < n; ++i) {
+        Rng r(seed, i);
+        double eps = 1, c = 1;
+        if (kind == 0) {
+            c = sample_kn(E / ME_C2_KEV, r, eps);
+        } else if (kind == 1) {
+            for (;;) {
+                c = sample_kn(E / ME_C2_KEV, r, eps);
+                const double x = E / HC_KEV_ANG * std::sqrt(0.5 * (1.0 - c));
+                if (r.u() * m.electrons_per_gram <= m.S(x))
+                    break;
+            }
+        } else if (kind == 2) {
+            c = sample_thomson(r);
+        } else {
+            c = sample_rayleigh_ff(m, E, r);
+        }
+        cost_out[i] = c;
+        if (eps_out)
+            eps_out[i] = eps;
+    }
+    return 0;
+}
+
+// The xoshiro128+ stream of one history (for RNG cross-checks).
+void oracle_rng_stream(uint64_t seed, uint64_t history, uint64_t n, uint32_t* out)
+{
+    Rng r(seed, history);
+    for (uint64_t i = 0; i < n; ++i)
+        out[i] = r.next();
+}
+
+// FP32 deterministic traversal with the exact operation order of the HIP
+// kernel: box entry by the slab method, then p = fmaf(s, d, p) per step and
+// ix = (int)floorf((p - box_min) * inv_spacing).  out_index = -1 outside.
+int oracle_traverse_f32(const uint64_t dims[3], const float spacing[3], const uint8_t* material,
+    uint64_t n_rays, const float* pos, const float* dir, uint64_t n_steps, const float* steps,
+    int64_t* out_index, int32_t* out_material)
+{
+    float bmin[3], bmax[3], inv[3];
+    for (int a = 0; a < 3; ++a) {
+        bmin[a] = -0.5f * (static_cast<float>(dims[a]) * spacing[a]);
+        bmax[a] = -bmin[a];
+        inv[a] = 1.0f / spacing[a];
+    }
+    for (uint64_t r = 0; r < n_rays; ++r) {
+        float p[3] = { pos[3 * r], pos[3 * r + 1], pos[3 * r + 2] };
+        const float d[3] = { dir[3 * r], dir[3 * r + 1], dir[3 * r + 2] };
+        // slab entry (same formula as the kernel)
+        float t0 = 0.0f, t1 = 3.0e38f;
+        bool hit = true;
+        for (int a = 0; a < 3; ++a) {
+            const float id = 1.0f / d[a];
+            float ta = (bmin[a] - p[a]) * id;
+            float tb = (bmax[a] - p[a]) * id;
+            const float lo = std::fmin(ta, tb), hi = std::fmax(ta, tb);
+            t0 = std::fmax(t0, lo);
+            t1 = std::fmin(t1, hi);
+        }
+        if (!(t0 <= t1))
+            hit = false;
+        if (hit)
+            for (int a = 0; a < 3; ++a)
+                p[a] = std::fmaf(t0, d[a], p[a]);
+        for (uint64_t k = 0; k < n_steps; ++k) {
+            int64_t vi = -1;
+            int32_t mm = -1;
+            if (hit) {
+                for (int a = 0; a < 3; ++a)
+                    p[a] = std::fmaf(steps[r * n_steps + k], d[a], p[a]);
+                const int ix = static_cast<int>(std::floor((p[0] - bmin[0]) * inv[0]));
+                const int iy = static_cast<int>(std::floor((p[1] - bmin[1]) * inv[1]));
+                const int iz = static_cast<int>(std::floor((p[2] - bmin[2]) * inv[2]));
+                if (ix >= 0 && iy >= 0 && iz >= 0 && ix < static_cast<int>(dims[0]) && iy < static_cast<int>(dims[1]) && iz < static_cast<int>(dims[2])) {
+                    vi = ix + static_cast<int64_t>(dims[0]) * (iy + static_cast<int64_t>(dims[1]) * iz);
+                    mm = material[vi];
+                }
+            }
+            out_index[r * n_steps + k] = vi;
+            out_material[r * n_steps + k] = mm;
+        }
+    }
+    return 0;
+}
+
+// Run all exposures of one beam through the grid (Transport::operator()).
+// Tallies are accumulated (not cleared): sum_e, sum_e2 in keV, n_ev counts.
+int oracle_transport(const uint64_t dims[3], const double spacing[3], const double* density, const uint8_t* material,
+    const oracle_material* mats, uint64_t n_mats,
+    const oracle_exposure* exposures, uint64_t n_exposures,
+    const oracle_spectrum* spectra, uint64_t n_spectra,
+    const oracle_bowtie* bowties, uint64_t n_bowties,
+    int correction_level, uint64_t seed, uint64_t history_offset, int n_threads,
+    double* sum_e, double* sum_e2, uint64_t* n_ev, oracle_counters* counters)
+{
+    if (!g_phys.loaded) {
+        g_err = "physics list not loaded";
+        return -1;
+    }
+    if (correction_level < 0 || correction_level > 1) {
+        g_err = "correction level must be 0 or 1";
+        return -2;
+    }
+    Grid g;
+    size_t N = 1;
+    for (int a = 0; a < 3; ++a) {
+        g.dims[a] = dims[a];
+        g.spacing[a] = spacing[a];
+        g.box_min[a] = -0.5 * dims[a] * spacing[a];
+        g.box_max[a] = 0.5 * dims[a] * spacing[a];
+        N *= dims[a];
+    }
+    g.density.assign(density, density + N);
+    g.material.assign(material, material + N);
+    for (uint64_t i = 0; i < n_mats; ++i) {
+        Material m;
+        if (!make_material(mats[i].Z, mats[i].weight, static_cast<int>(mats[i].n), m))
+            return -3;
+        g.mats.push_back(std::move(m));
+    }
+    g.rho_max.assign(n_mats, 0.0);
+    for (size_t i = 0; i < N; ++i) {
+        if (g.material[i] >= n_mats) {
+            g_err = "material index out of range";
+            return -4;
+        }
+        g.rho_max[g.material[i]] = std::max(g.rho_max[g.material[i]], g.density[i]);
+    }
+    Run R;
+    R.g = &g;
+    R.ex.assign(exposures, exposures + n_exposures);
+    for (uint64_t i = 0; i < n_spectra; ++i) {
+        Spectrum s;
+        double tot = 0;
+        for (uint64_t k = 0; k < spectra[i].n; ++k)
+            tot += spectra[i].weight[k];
+        double c = 0;
+        for (uint64_t k = 0; k < spectra[i].n; ++k) {
+            c += spectra[i].weight[k] / tot;
+            s.e.push_back(spectra[i].energy[k]);
+            s.cdf.push_back(c);
+        }
+        s.cdf.back() = 1.0;
+        R.spectra.push_back(std::move(s));
+    }
+    for (uint64_t i = 0; i < n_bowties; ++i)
+        R.bowties.push_back(make_bowtie(bowties[i]));
+    for (auto& e : R.ex) {
+        if (e.spectrum_index >= n_spectra || (e.bowtie_index >= 0 && static_cast<uint64_t>(e.bowtie_index) >= n_bowties)) {
+            g_err = "exposure references a missing spectrum or bowtie";
+            return -5;
+        }
+    }
+    uint64_t acc_n = 0;
+    for (auto& e : R.ex) {
+        R.ex_start.push_back(acc_n);
+        acc_n += e.n_particles;
+    }
+    R.total = acc_n;
+    R.level = correction_level;
+    R.seed = seed;
+    R.offset = history_offset;
+    R.sum_e = sum_e;
+    R.sum_e2 = sum_e2;
+    R.n_ev = n_ev;
+    // ex_start must be searchable by global history index h in [0, total)
+    const int nt = std::max(1, n_threads);
+    std::vector<Acc> accs(static_cast<size_t>(nt));
+    std::atomic<uint64_t> next { 0 };
+    constexpr uint64_t CHUNK = 4096;
+    auto worker = [&](int t) {
+        for (;;) {
+            const uint64_t b = next.fetch_add(CHUNK);
+            if (b >= R.total)
+                break;
+            const uint64_t e = std::min(R.total, b + CHUNK);
+            for (uint64_t h = b; h < e; ++h)
+                history(R, h, accs[static_cast<size_t>(t)]);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back(worker, t);
+    for (auto& t : th)
+        t.join();
+    if (counters) {
+        oracle_counters c {};
+        c.histories = R.total;
+        for (auto& a : accs) {
+            c.steps += a.steps;
+            c.real_events += a.real;
+            c.deposits += a.deposits;
+            c.emitted_energy += a.emitted;
+            c.escaped_energy += a.escaped;
+            c.deposited_energy += a.deposited;
+        }
+        *counters = c;
+    }
+    return 0;
+}
+
+}  // extern "C"

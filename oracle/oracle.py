@@ -1,0 +1,185 @@
+"""ctypes binding of the oracle (liboracle_dxmc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or as the timed CPU baseline.
+The product never imports it.  The exposure, spectrum and bowtie records share
+the plain-C layout of include/dxmc_hip.h, so one numpy record array feeds both.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle_dxmc.so"
+PHYSICS_LIST = HERE.parent / "opendxmc_amd" / "physics" / "physics_list.bin"
+
+u64 = C.c_uint64
+PD = C.POINTER(C.c_double)
+
+
+class Material(C.Structure):
+    _fields_ = [("Z", C.POINTER(C.c_int)), ("weight", PD), ("n", u64)]
+
+
+class Spectrum(C.Structure):
+    _fields_ = [("energy", PD), ("weight", PD), ("n", u64)]
+
+
+class Bowtie(C.Structure):
+    _fields_ = [("angle", PD), ("weight", PD), ("n", u64)]
+
+
+class Counters(C.Structure):
+    _fields_ = [("histories", u64), ("steps", u64), ("real_events", u64), ("deposits", u64),
+                ("emitted_energy", C.c_double), ("escaped_energy", C.c_double), ("deposited_energy", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} missing: build the oracle (make -C oracle)")
+        h = C.CDLL(str(LIB_PATH))
+        h.oracle_last_error.restype = C.c_char_p
+        h.oracle_load_physics.argtypes = [C.c_char_p]
+        h.oracle_material_attenuation.argtypes = [C.POINTER(Material), PD, u64, PD]
+        h.oracle_material_sf.argtypes = [C.POINTER(Material), PD, u64, PD, PD, PD]
+        h.oracle_sample_angles.argtypes = [C.POINTER(Material), C.c_double, C.c_int, u64, u64, PD, PD]
+        h.oracle_rng_stream.argtypes = [u64, u64, u64, C.POINTER(C.c_uint32)]
+        h.oracle_rng_stream.restype = None
+        h.oracle_traverse_f32.argtypes = [C.POINTER(u64), C.POINTER(C.c_float), C.POINTER(C.c_uint8), u64,
+                                          C.POINTER(C.c_float), C.POINTER(C.c_float), u64, C.POINTER(C.c_float),
+                                          C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+        h.oracle_transport.argtypes = [C.POINTER(u64), PD, PD, C.POINTER(C.c_uint8), C.POINTER(Material), u64,
+                                       C.c_void_p, u64, C.POINTER(Spectrum), u64, C.POINTER(Bowtie), u64,
+                                       C.c_int, u64, u64, C.c_int, PD, PD, C.POINTER(u64), C.POINTER(Counters)]
+        if h.oracle_load_physics(str(PHYSICS_LIST).encode()) != 0:
+            raise RuntimeError(h.oracle_last_error().decode())
+        _lib = h
+    return _lib
+
+
+def _err() -> str:
+    return (lib().oracle_last_error() or b"").decode()
+
+
+class _Keep:
+    """Holds ctypes arrays alive for the duration of a call."""
+
+    def __init__(self):
+        self.items = []
+
+    def keep(self, x):
+        self.items.append(x)
+        return x
+
+
+def material(comp: dict[int, float], keep: _Keep) -> Material:
+    z = keep.keep(np.array(list(comp.keys()), dtype=np.int32))
+    w = keep.keep(np.array(list(comp.values()), dtype=np.float64))
+    return Material(z.ctypes.data_as(C.POINTER(C.c_int)), w.ctypes.data_as(PD), len(comp))
+
+
+def attenuation(comp: dict[int, float], energy) -> np.ndarray:
+    """(n, 4) array of pe, incoherent, coherent, total mass attenuation (cm2/g)."""
+    k = _Keep()
+    e = np.ascontiguousarray(np.atleast_1d(energy), dtype=np.float64)
+    out = np.zeros((len(e), 4))
+    m = material(comp, k)
+    if lib().oracle_material_attenuation(C.byref(m), e.ctypes.data_as(PD), len(e), out.ctypes.data_as(PD)) != 0:
+        raise RuntimeError(_err())
+    return out
+
+
+def scatter_functions(comp: dict[int, float], x) -> tuple[np.ndarray, np.ndarray, float]:
+    k = _Keep()
+    xs = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+    s = np.zeros(len(xs))
+    f2 = np.zeros(len(xs))
+    epg = C.c_double()
+    m = material(comp, k)
+    if lib().oracle_material_sf(C.byref(m), xs.ctypes.data_as(PD), len(xs), s.ctypes.data_as(PD),
+                                f2.ctypes.data_as(PD), C.byref(epg)) != 0:
+        raise RuntimeError(_err())
+    return s, f2, epg.value
+
+
+def sample_angles(comp: dict[int, float], energy: float, kind: int, n: int, seed: int = 1):
+    k = _Keep()
+    m = material(comp, k)
+    c = np.zeros(n)
+    eps = np.zeros(n)
+    if lib().oracle_sample_angles(C.byref(m), energy, kind, n, seed, c.ctypes.data_as(PD), eps.ctypes.data_as(PD)) != 0:
+        raise RuntimeError(_err())
+    return c, eps
+
+
+def rng_stream(seed: int, history: int, n: int) -> np.ndarray:
+    out = np.zeros(n, dtype=np.uint32)
+    lib().oracle_rng_stream(seed, history, n, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def traverse_f32(dims, spacing, material_arr, pos, dirs, steps):
+    d = np.ascontiguousarray(dims, dtype=np.uint64)
+    sp = np.ascontiguousarray(spacing, dtype=np.float32)
+    mat = np.ascontiguousarray(material_arr, dtype=np.uint8)
+    p = np.ascontiguousarray(pos, dtype=np.float32)
+    dr = np.ascontiguousarray(dirs, dtype=np.float32)
+    st = np.ascontiguousarray(steps, dtype=np.float32)
+    n_rays, n_steps = st.shape
+    idx = np.zeros((n_rays, n_steps), dtype=np.int64)
+    mm = np.zeros((n_rays, n_steps), dtype=np.int32)
+    lib().oracle_traverse_f32(d.ctypes.data_as(C.POINTER(u64)), sp.ctypes.data_as(C.POINTER(C.c_float)),
+                              mat.ctypes.data_as(C.POINTER(C.c_uint8)), n_rays,
+                              p.ctypes.data_as(C.POINTER(C.c_float)), dr.ctypes.data_as(C.POINTER(C.c_float)),
+                              n_steps, st.ctypes.data_as(C.POINTER(C.c_float)),
+                              idx.ctypes.data_as(C.POINTER(C.c_int64)), mm.ctypes.data_as(C.POINTER(C.c_int32)))
+    return idx, mm
+
+
+def transport(dims, spacing, density, material_arr, materials: list[dict[int, float]], exposures: np.ndarray,
+              spectra: list[tuple[np.ndarray, np.ndarray]], bowties: list[tuple[np.ndarray, np.ndarray]] = (),
+              level: int = 1, seed: int = 1, history_offset: int = 0, threads: int = 8):
+    """Run one beam; returns (sum_e, sum_e2, n_events, counters dict)."""
+    k = _Keep()
+    d = k.keep(np.ascontiguousarray(dims, dtype=np.uint64))
+    sp = k.keep(np.ascontiguousarray(spacing, dtype=np.float64))
+    dens = k.keep(np.ascontiguousarray(density, dtype=np.float64).ravel())
+    mat = k.keep(np.ascontiguousarray(material_arr, dtype=np.uint8).ravel())
+    mats = (Material * len(materials))(*[material(m, k) for m in materials])
+    ex = k.keep(np.ascontiguousarray(exposures))
+    specs = []
+    for e, w in spectra:
+        e = k.keep(np.ascontiguousarray(e, dtype=np.float64))
+        w = k.keep(np.ascontiguousarray(w, dtype=np.float64))
+        specs.append(Spectrum(e.ctypes.data_as(PD), w.ctypes.data_as(PD), len(e)))
+    spec_arr = (Spectrum * len(specs))(*specs)
+    bows = []
+    for a, w in bowties:
+        a = k.keep(np.ascontiguousarray(a, dtype=np.float64))
+        w = k.keep(np.ascontiguousarray(w, dtype=np.float64))
+        bows.append(Bowtie(a.ctypes.data_as(PD), w.ctypes.data_as(PD), len(a)))
+    bow_arr = (Bowtie * max(1, len(bows)))(*bows)
+    n = int(np.prod(dims))
+    se = np.zeros(n)
+    se2 = np.zeros(n)
+    nev = np.zeros(n, dtype=np.uint64)
+    cnt = Counters()
+    rc = lib().oracle_transport(d.ctypes.data_as(C.POINTER(u64)), sp.ctypes.data_as(PD), dens.ctypes.data_as(PD),
+                                mat.ctypes.data_as(C.POINTER(C.c_uint8)), mats, len(materials),
+                                ex.ctypes.data_as(C.c_void_p), len(ex), spec_arr, len(specs), bow_arr, len(bows),
+                                level, seed, history_offset, threads, se.ctypes.data_as(PD), se2.ctypes.data_as(PD),
+                                nev.ctypes.data_as(C.POINTER(u64)), C.byref(cnt))
+    if rc != 0:
+        raise RuntimeError(f"oracle_transport failed ({rc}): {_err()}")
+    return se, se2, nev, cnt.as_dict()
