@@ -183,6 +183,13 @@ int dxg_material_attenuation(const dxg_material* m, const double* energy_kev, ui
 int dxg_debug_traverse(const dxg_world* w, uint64_t n_rays, const float* pos, const float* dir,
                        uint64_t n_steps, const float* steps, int64_t* out_index, int32_t* out_material);
 
+/* ---- Profiling aid (no counterpart in the reference): `iters` launches of a
+ * plain float4 copy kernel over `bytes` bytes (multiple of 16) between two
+ * fresh device buffers.  Under a hardware-counter profiler its FETCH_SIZE and
+ * WRITE_SIZE per launch calibrate the counters against a known traffic of
+ * `bytes` read and `bytes` written.  *ms receives the mean launch time. */
+int dxg_debug_device_copy(uint64_t bytes, int iters, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

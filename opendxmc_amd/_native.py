@@ -157,6 +157,7 @@ SIGNATURES = [
     ("dxg_ct_spiral_exposures", C.c_int, [C.POINTER(CTSpiralParams), C.POINTER(Exposure), u64, C.POINTER(u64)]),
     ("dxg_ct_sequential_exposures", C.c_int, [C.POINTER(CTSequentialParams), C.POINTER(Exposure), u64, C.POINTER(u64)]),
     ("dxg_cbct_exposures", C.c_int, [C.POINTER(CBCTParams), C.POINTER(Exposure), u64, C.POINTER(u64)]),
+    ("dxg_debug_device_copy", C.c_int, [u64, C.c_int, PD]),
     ("dxg_debug_traverse", C.c_int, [C.c_void_p, u64, C.POINTER(C.c_float), C.POINTER(C.c_float), u64,
                                      C.POINTER(C.c_float), C.POINTER(i64), C.POINTER(C.c_int32)]),
 ]

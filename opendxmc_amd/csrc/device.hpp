@@ -32,11 +32,34 @@ struct DevCounters {
     double pad;
 };
 
+// Packed voxel word: the FP32 density with its 8 low mantissa bits replaced
+// by the material id.  One 4-byte gather per Woodcock step yields both; the
+// truncated density is below the true one by at most 2^-15 relative, so the
+// majorant built from the true densities still bounds it.
+__host__ __device__ inline uint32_t packVoxel(float density, uint32_t material)
+{
+    union {
+        float f;
+        uint32_t u;
+    } v;
+    v.f = density;
+    return (v.u & 0xFFFFFF00u) | (material & 0xFFu);
+}
+
+// Small per-beam tables, staged into LDS by every block.  Word offsets into
+// one uint32 blob:
+//   spec_off[n_spec] spec_n[n_spec] bow_off[n_bow] bow_n[n_bow]
+//   spec_energy[S] spec_prob[S] spec_alias[S] bow_angle[B] bow_weight[B]
+struct BeamBlob {
+    int n_spec, n_bow, n_bins, n_nodes;
+    int o_spec_off, o_spec_n, o_bow_off, o_bow_n, o_energy, o_prob, o_alias, o_angle, o_weight;
+    int words;
+};
+
 // Everything one transport launch reads.  Passed by value as a kernel argument.
 struct DevParams {
-    // voxel grid, flat index x + nx (y + ny z)
-    const uint8_t* material;
-    const float* density;
+    // voxel grid of packed words, flat index x + nx (y + ny z)
+    const uint32_t* voxels;
     int nx, ny, nz, n_mat;
     float bmin[3];
     float bmax[3];
@@ -58,15 +81,9 @@ struct DevParams {
     const DevExposure* exposures;
     const unsigned long long* ex_start;   // first beam-history of each exposure, n_ex entries
     int n_ex;
-    const float* spec_energy;   // concatenated spectra
-    const float* spec_prob;     // alias acceptance probability
-    const int* spec_alias;      // alias index (relative to the spectrum)
-    const int* spec_off;        // per spectrum: first bin
-    const int* spec_n;          // per spectrum: number of bins
-    const float* bow_angle;     // concatenated bowtie nodes (ascending |angle|)
-    const float* bow_weight;
-    const int* bow_off;
-    const int* bow_n;
+    unsigned long long ppe;               // particles per exposure when uniform, else 0
+    const uint32_t* blob;                 // BeamBlob words
+    BeamBlob bb;
 
     unsigned long long h_begin;   // this launch covers beam histories [h_begin, h_end)
     unsigned long long h_end;
@@ -74,13 +91,20 @@ struct DevParams {
     unsigned long long rng_offset;   // added to the beam-history index that keys the RNG
     unsigned long long* pool;        // history pool counter, zero at launch
     DevCounters* counters;
+    int refill_threshold;            // dead lanes that trigger a wavefront refill
+    int lds_lut;                     // attenuation table staged in LDS
+    unsigned long long batch;        // histories a wavefront claims per pool atomic (>= 64)
 };
 
+// LDS bytes a transport block needs for these params.
+size_t transportLdsBytes(const DevParams& p);
 void launchTransport(const DevParams& p, int blocks, int threads, hipStream_t s);
 void launchTraverse(const DevParams& p, uint64_t n_rays, const float* pos, const float* dir, uint64_t n_steps,
                     const float* steps, int64_t* out_index, int32_t* out_material, hipStream_t s);
 void launchReset(unsigned long long* pool, DevCounters* counters, hipStream_t s);
 int transportBlockSize();
-const void* transportKernelSymbol();
+void launchCopy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
+// Resident blocks per CU of the transport kernel at the given LDS size.
+int transportBlocksPerCU(bool lds_lut, size_t lds_bytes);
 
 }  // namespace dxg

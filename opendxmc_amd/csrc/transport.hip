@@ -1,14 +1,20 @@
 // HIP/CDNA4 kernels of the voxel photon-transport engine.
 //
-// transport_kernel is a persistent history-pool kernel: every lane carries one
-// photon in registers (FP32 structure-of-registers state), and whenever lanes
-// of a wavefront lose their photon (absorbed, escaped, missed the grid) the
-// wavefront claims exactly that many new histories from a global pool counter
-// with ONE atomic and hands them to the dead lanes in lane order.  Each loop
-// iteration every live lane advances one Woodcock step, so the wavefront stays
-// full until the pool drains.  Deposits go straight to the FP64 per-voxel
-// tally with hardware float atomics; event counters are reduced across the
-// wavefront before a single atomic per wavefront at exit.
+// transport_kernel is a persistent history-pool kernel.  Every lane carries
+// one photon in registers (FP32 state).  When a wavefront has enough dead
+// lanes (absorbed, escaped, missed the grid) -- refill_threshold of them, or
+// all -- it claims exactly that many new histories from a global pool counter
+// with ONE atomic and hands them to the dead lanes in lane order, so the
+// expensive source-sampling path is paid once for many lanes.  Each loop
+// iteration every live lane advances one Woodcock step: one 4-byte gather of
+// the packed voxel word (density + material id), one LDS read of the
+// material's attenuation at the photon's energy bin, one compare.  Deposits go
+// straight to the FP64 per-voxel tally with hardware float atomics; event
+// counters are reduced across the wavefront before one atomic per wavefront.
+//
+// LDS holds, per block: the Woodcock majorant table, the per-material total
+// attenuation table (when it fits), and the beam's spectrum alias tables and
+// bowtie nodes.
 //
 // Algorithms (DXMClib's AAVoxelGrid::transport and interactions<5, C>, absent
 // from the reference tree; the caller is simulationpipeline.cpp:159-170):
@@ -162,10 +168,27 @@ __device__ inline float lerpTab(const float* tab, int j, float t)
     return __fmaf_rn(t, b - a, a);
 }
 
+__device__ inline float asFloat(uint32_t u) { return __uint_as_float(u); }
+
 }  // namespace
 
+template <bool LDS_LUT>
 __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
 {
+    extern __shared__ uint32_t lds[];
+    float* s_maj = reinterpret_cast<float*>(lds);
+    float* s_mu = s_maj + kLutNd;
+    uint32_t* s_blob = lds + kLutNd + (LDS_LUT ? P.n_mat * kLutNd : 0);
+    for (int i = threadIdx.x; i < kLutNd; i += blockDim.x)
+        s_maj[i] = P.majorant[i];
+    if (LDS_LUT)
+        for (int i = threadIdx.x; i < P.n_mat * kLutNd; i += blockDim.x)
+            s_mu[i] = P.mu_tot[i];
+    for (int i = threadIdx.x; i < P.bb.words; i += blockDim.x)
+        s_blob[i] = P.blob[i];
+    __syncthreads();
+    const float* mu_tab = LDS_LUT ? s_mu : P.mu_tot;
+
     const int lane = static_cast<int>(__lane_id());
     float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 1, E = 0, w = 0;
     float maj = 1, inv_maj = 1, et = 0;
@@ -175,35 +198,54 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
 
     unsigned long long n_steps = 0, n_real = 0, n_dep = 0, n_hist = 0;
     double e_emit = 0, e_esc = 0, e_dep = 0;
+    // wavefront-private batch of pool histories [cur, end), claimed P.batch at a time
+    unsigned long long cur = 0, end = 0;
 
     for (;;) {
-        // ---- refill: dead lanes of this wavefront claim new histories together
+        // ---- refill: once enough lanes are dead, they take the next histories together
         const bool need = !alive && !done;
-        const unsigned long long mask = __ballot(need);
-        if (mask) {
-            const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-            unsigned long long base = 0;
-            if (lane == leader)
-                base = atomicAdd(P.pool, static_cast<unsigned long long>(__popcll(mask)));
-            base = __shfl(base, leader, 64);
+        const unsigned long long need_mask = __ballot(need);
+        const int n_need = __popcll(need_mask);
+        if (n_need > 0 && (n_need >= P.refill_threshold || __ballot(alive) == 0)) {
+            const unsigned long long rank = __popcll(need_mask & ((1ull << lane) - 1ull));
+            const unsigned long long avail = end - cur;
+            unsigned long long mine;
+            if (avail >= static_cast<unsigned long long>(n_need)) {
+                mine = cur + rank;
+                cur += n_need;
+            } else {
+                // the rest of this batch first, then a fresh batch from the pool
+                unsigned long long base = 0;
+                if (lane == __ffsll(static_cast<long long>(need_mask)) - 1)
+                    base = atomicAdd(P.pool, static_cast<unsigned long long>(P.batch));
+                base = __shfl(base, __ffsll(static_cast<long long>(need_mask)) - 1, 64);
+                mine = rank < avail ? cur + rank : base + (rank - avail);
+                cur = base + (n_need - avail);
+                end = base + P.batch;
+            }
             if (need) {
-                const unsigned long long rank = __popcll(mask & ((1ull << lane) - 1ull));
-                const unsigned long long h = P.h_begin + base + rank;
+                const unsigned long long h = P.h_begin + mine;
                 if (h >= P.h_end) {
                     done = true;
                 } else {
                     ++n_hist;
                     rngSeed(rng, P.seed, h + P.rng_offset);
                     // exposure containing beam history h
-                    int lo = 0, hi = P.n_ex - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (P.ex_start[mid] <= h)
-                            lo = mid;
-                        else
-                            hi = mid - 1;
+                    int ei;
+                    if (P.ppe) {
+                        ei = static_cast<int>(min(h / P.ppe, static_cast<unsigned long long>(P.n_ex - 1)));
+                    } else {
+                        int lo = 0, hi = P.n_ex - 1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (P.ex_start[mid] <= h)
+                                lo = mid;
+                            else
+                                hi = mid - 1;
+                        }
+                        ei = lo;
                     }
-                    const DevExposure& ex = P.exposures[lo];
+                    const DevExposure& ex = P.exposures[ei];
                     // Exposure::sampleParticle
                     const float ax = ex.half_angle[0] * (2.0f * rngU(rng) - 1.0f);
                     const float ay = ex.half_angle[1] * (2.0f * rngU(rng) - 1.0f);
@@ -219,32 +261,34 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
                     px = ex.pos[0];
                     py = ex.pos[1];
                     pz = ex.pos[2];
-                    {   // spectrum: Walker alias with one uniform
-                        const int off = P.spec_off[ex.spectrum];
-                        const int nb = P.spec_n[ex.spectrum];
+                    {   // spectrum: Walker alias with one uniform, from LDS
+                        const int off = static_cast<int>(s_blob[P.bb.o_spec_off + ex.spectrum]);
+                        const int nb = static_cast<int>(s_blob[P.bb.o_spec_n + ex.spectrum]);
                         const float f = rngU(rng) * static_cast<float>(nb);
                         int k = min(static_cast<int>(f), nb - 1);
                         const float frac = f - static_cast<float>(k);
-                        if (frac >= P.spec_prob[off + k])
-                            k = P.spec_alias[off + k];
-                        E = P.spec_energy[off + k];
+                        if (frac >= asFloat(s_blob[P.bb.o_prob + off + k]))
+                            k = static_cast<int>(s_blob[P.bb.o_alias + off + k]);
+                        E = asFloat(s_blob[P.bb.o_energy + off + k]);
                     }
                     w = ex.weight;
-                    if (ex.bowtie >= 0) {
-                        const int off = P.bow_off[ex.bowtie];
-                        const int nb = P.bow_n[ex.bowtie];
+                    if (ex.bowtie >= 0) {   // piecewise linear in |angle|, from LDS
+                        const int off = static_cast<int>(s_blob[P.bb.o_bow_off + ex.bowtie]);
+                        const int nb = static_cast<int>(s_blob[P.bb.o_bow_n + ex.bowtie]);
+                        const float* ang = reinterpret_cast<const float*>(s_blob + P.bb.o_angle) + off;
+                        const float* wt = reinterpret_cast<const float*>(s_blob + P.bb.o_weight) + off;
                         const float a = fabsf(ax);
                         float bw;
-                        if (a <= P.bow_angle[off]) {
-                            bw = P.bow_weight[off];
-                        } else if (a >= P.bow_angle[off + nb - 1]) {
-                            bw = P.bow_weight[off + nb - 1];
+                        if (a <= ang[0]) {
+                            bw = wt[0];
+                        } else if (a >= ang[nb - 1]) {
+                            bw = wt[nb - 1];
                         } else {
-                            int k = off;
-                            while (P.bow_angle[k + 1] < a)
+                            int k = 0;
+                            while (ang[k + 1] < a)
                                 ++k;
-                            const float t = (a - P.bow_angle[k]) / (P.bow_angle[k + 1] - P.bow_angle[k]);
-                            bw = P.bow_weight[k] + t * (P.bow_weight[k + 1] - P.bow_weight[k]);
+                            const float t = (a - ang[k]) / (ang[k + 1] - ang[k]);
+                            bw = wt[k] + t * (wt[k + 1] - wt[k]);
                         }
                         w *= bw;
                     }
@@ -254,7 +298,7 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
                         const float f = (E - kLutE0f) * kLutInvDEf;
                         ej = min(static_cast<int>(f), kLutNd - 2);
                         et = f - static_cast<float>(ej);
-                        maj = P.majorant[ej];
+                        maj = s_maj[ej];
                         inv_maj = 1.0f / maj;
                     } else {
                         e_esc += static_cast<double>(w) * static_cast<double>(E);
@@ -282,9 +326,10 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
             alive = false;
             continue;
         }
-        const int m = P.material[vi];
-        const float rho = P.density[vi];
-        const float mu = lerpTab(P.mu_tot + m * kLutNd, ej, et) * rho;
+        const uint32_t word = P.voxels[vi];
+        const int m = static_cast<int>(word & 0xFFu);
+        const float rho = asFloat(word & 0xFFFFFF00u);
+        const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
         if (rngU(rng) * maj > mu)
             continue;   // virtual collision
 
@@ -354,7 +399,7 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
             const float f = (E - kLutE0f) * kLutInvDEf;
             ej = min(static_cast<int>(f), kLutNd - 2);
             et = f - static_cast<float>(ej);
-            maj = P.majorant[ej];
+            maj = s_maj[ej];
             inv_maj = 1.0f / maj;
         } else {
             // Rayleigh
@@ -425,7 +470,7 @@ __global__ void traverse_kernel(const DevParams P, uint64_t n_rays, const float*
             const int v = voxelOf(P, px, py, pz);
             if (v >= 0) {
                 vi = v;
-                mm = P.material[v];
+                mm = static_cast<int32_t>(P.voxels[v] & 0xFFu);
             }
         }
         out_index[r * n_steps + k] = vi;
@@ -448,9 +493,28 @@ __global__ void reset_kernel(unsigned long long* pool, DevCounters* c)
     }
 }
 
+size_t transportLdsBytes(const DevParams& p)
+{
+    return sizeof(uint32_t) * (static_cast<size_t>(kLutNd) + (p.lds_lut ? static_cast<size_t>(p.n_mat) * kLutNd : 0)
+                               + static_cast<size_t>(p.bb.words));
+}
+
 void launchTransport(const DevParams& p, int blocks, int threads, hipStream_t s)
 {
-    hipLaunchKernelGGL(transport_kernel, dim3(static_cast<unsigned>(blocks)), dim3(static_cast<unsigned>(threads)), 0, s, p);
+    const size_t lds = transportLdsBytes(p);
+    if (p.lds_lut)
+        hipLaunchKernelGGL(transport_kernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(static_cast<unsigned>(threads)), lds, s, p);
+    else
+        hipLaunchKernelGGL(transport_kernel<false>, dim3(static_cast<unsigned>(blocks)), dim3(static_cast<unsigned>(threads)), lds, s, p);
+}
+
+int transportBlocksPerCU(bool lds_lut, size_t lds_bytes)
+{
+    int n = 0;
+    const void* fn = lds_lut ? reinterpret_cast<const void*>(&transport_kernel<true>) : reinterpret_cast<const void*>(&transport_kernel<false>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kBlock, lds_bytes) != hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
 }
 
 void launchTraverse(const DevParams& p, uint64_t n_rays, const float* pos, const float* dir, uint64_t n_steps,
@@ -466,6 +530,21 @@ void launchReset(unsigned long long* pool, DevCounters* counters, hipStream_t s)
 }
 
 int transportBlockSize() { return kBlock; }
-const void* transportKernelSymbol() { return reinterpret_cast<const void*>(&transport_kernel); }
+
+}  // namespace dxg
+
+namespace dxg {
+
+__global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n)
+{
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launchCopy(const void* src, void* dst, uint64_t bytes, hipStream_t s)
+{
+    const uint64_t n = bytes / sizeof(float4);
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s, static_cast<const float4*>(src), static_cast<float4*>(dst), n);
+}
 
 }  // namespace dxg

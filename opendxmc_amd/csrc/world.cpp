@@ -91,8 +91,7 @@ struct dxg_world {
 
     int device = 0;
     hipStream_t stream = nullptr;
-    DevBuf<uint8_t> d_material;
-    DevBuf<float> d_density;
+    DevBuf<uint32_t> d_voxels;
     DevBuf<float> d_mu_tot;
     DevBuf<float4> d_mu_parts;
     DevBuf<float> d_majorant, d_s, d_rfwd, d_rinv;
@@ -102,8 +101,7 @@ struct dxg_world {
     // per-run beam tables
     DevBuf<dxg::DevExposure> d_ex;
     DevBuf<unsigned long long> d_ex_start;
-    DevBuf<float> d_spec_e, d_spec_p, d_bow_a, d_bow_w;
-    DevBuf<int> d_spec_alias, d_spec_off, d_spec_n, d_bow_off, d_bow_n;
+    DevBuf<uint32_t> d_blob;
 
     uint64_t histories = 0;
     dxg_counters counters {};
@@ -131,8 +129,7 @@ namespace {
 void fillParams(const dxg_world* w, dxg::DevParams& p)
 {
     std::memset(&p, 0, sizeof p);
-    p.material = w->d_material.ptr;
-    p.density = w->d_density.ptr;
+    p.voxels = w->d_voxels.ptr;
     p.nx = static_cast<int>(w->dims[0]);
     p.ny = static_cast<int>(w->dims[1]);
     p.nz = static_cast<int>(w->dims[2]);
@@ -286,11 +283,10 @@ int dxg_world_build(dxg_world* w)
         HIP_TRY(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
     dxg::buildTables(*pl, w->mats, w->rho_max, w->tables);
     const uint64_t n = w->size();
-    std::vector<float> dens32(n);
+    std::vector<uint32_t> packed(n);
     for (uint64_t i = 0; i < n; ++i)
-        dens32[i] = static_cast<float>(w->density[i]);
-    HIP_TRY(w->d_material.upload(w->material, w->stream));
-    HIP_TRY(w->d_density.upload(dens32, w->stream));
+        packed[i] = dxg::packVoxel(static_cast<float>(w->density[i]), w->material[i]);
+    HIP_TRY(w->d_voxels.upload(packed, w->stream));
     HIP_TRY(w->d_mu_tot.upload(w->tables.mu_tot, w->stream));
     {
         std::vector<float4> parts(w->tables.mu_parts.size() / 4);
@@ -437,48 +433,89 @@ int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress
         return fail(DXG_ERR_ARGUMENT, "beam has zero particles");
     HIP_TRY(w->d_ex.upload(ex, w->stream));
     HIP_TRY(w->d_ex_start.upload(ex_start, w->stream));
-    HIP_TRY(w->d_spec_e.upload(se, w->stream));
-    HIP_TRY(w->d_spec_p.upload(sp, w->stream));
-    HIP_TRY(w->d_spec_alias.upload(sa, w->stream));
-    HIP_TRY(w->d_spec_off.upload(soff, w->stream));
-    HIP_TRY(w->d_spec_n.upload(sn, w->stream));
-    HIP_TRY(w->d_bow_a.upload(ba, w->stream));
-    HIP_TRY(w->d_bow_w.upload(bw, w->stream));
-    if (boff.empty()) {
-        boff.push_back(0);
-        bn.push_back(1);
+    // ---- the small per-beam tables travel as one blob staged into LDS
+    dxg::BeamBlob bb {};
+    bb.n_spec = static_cast<int>(soff.size());
+    bb.n_bow = static_cast<int>(boff.size());
+    bb.n_bins = static_cast<int>(se.size());
+    bb.n_nodes = static_cast<int>(ba.size());
+    bb.o_spec_off = 0;
+    bb.o_spec_n = bb.n_spec;
+    bb.o_bow_off = 2 * bb.n_spec;
+    bb.o_bow_n = 2 * bb.n_spec + bb.n_bow;
+    bb.o_energy = 2 * (bb.n_spec + bb.n_bow);
+    bb.o_prob = bb.o_energy + bb.n_bins;
+    bb.o_alias = bb.o_prob + bb.n_bins;
+    bb.o_angle = bb.o_alias + bb.n_bins;
+    bb.o_weight = bb.o_angle + bb.n_nodes;
+    bb.words = bb.o_weight + bb.n_nodes;
+    std::vector<uint32_t> blob(static_cast<size_t>(bb.words));
+    auto fbits = [](float f) {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return u;
+    };
+    for (int i = 0; i < bb.n_spec; ++i) {
+        blob[static_cast<size_t>(bb.o_spec_off + i)] = static_cast<uint32_t>(soff[static_cast<size_t>(i)]);
+        blob[static_cast<size_t>(bb.o_spec_n + i)] = static_cast<uint32_t>(sn[static_cast<size_t>(i)]);
     }
-    HIP_TRY(w->d_bow_off.upload(boff, w->stream));
-    HIP_TRY(w->d_bow_n.upload(bn, w->stream));
+    for (int i = 0; i < bb.n_bow; ++i) {
+        blob[static_cast<size_t>(bb.o_bow_off + i)] = static_cast<uint32_t>(boff[static_cast<size_t>(i)]);
+        blob[static_cast<size_t>(bb.o_bow_n + i)] = static_cast<uint32_t>(bn[static_cast<size_t>(i)]);
+    }
+    for (int i = 0; i < bb.n_bins; ++i) {
+        blob[static_cast<size_t>(bb.o_energy + i)] = fbits(se[static_cast<size_t>(i)]);
+        blob[static_cast<size_t>(bb.o_prob + i)] = fbits(sp[static_cast<size_t>(i)]);
+        blob[static_cast<size_t>(bb.o_alias + i)] = static_cast<uint32_t>(sa[static_cast<size_t>(i)]);
+    }
+    for (int i = 0; i < bb.n_nodes; ++i) {
+        blob[static_cast<size_t>(bb.o_angle + i)] = fbits(ba[static_cast<size_t>(i)]);
+        blob[static_cast<size_t>(bb.o_weight + i)] = fbits(bw[static_cast<size_t>(i)]);
+    }
+    HIP_TRY(w->d_blob.upload(blob, w->stream));
 
     dxg::DevParams p;
     fillParams(w, p);
     p.exposures = w->d_ex.ptr;
     p.ex_start = w->d_ex_start.ptr;
     p.n_ex = static_cast<int>(beam->n_exposures);
-    p.spec_energy = w->d_spec_e.ptr;
-    p.spec_prob = w->d_spec_p.ptr;
-    p.spec_alias = w->d_spec_alias.ptr;
-    p.spec_off = w->d_spec_off.ptr;
-    p.spec_n = w->d_spec_n.ptr;
-    p.bow_angle = w->d_bow_a.ptr;
-    p.bow_weight = w->d_bow_w.ptr;
-    p.bow_off = w->d_bow_off.ptr;
-    p.bow_n = w->d_bow_n.ptr;
+    p.ppe = beam->exposures[0].n_particles;
+    for (uint64_t i = 1; i < beam->n_exposures; ++i)
+        if (beam->exposures[i].n_particles != p.ppe)
+            p.ppe = 0;
+    p.blob = w->d_blob.ptr;
+    p.bb = bb;
     p.seed = seed;
     p.rng_offset = w->histories;
+    p.refill_threshold = 16;
+    if (const char* env = std::getenv("DXG_REFILL"))
+        p.refill_threshold = std::clamp(std::atoi(env), 1, 64);
+    p.batch = 256;
+    if (const char* env = std::getenv("DXG_BATCH"))
+        p.batch = std::max<unsigned long long>(64, std::strtoull(env, nullptr, 10));
+
+    // ---- LDS plan: the attenuation table joins the LDS tables while a block
+    // still fits 3 to a CU (<= 48 KiB); larger material sets read it from L1/L2
+    constexpr size_t kLdsLutLimit = 48 * 1024;
+    p.lds_lut = 1;
+    if (dxg::transportLdsBytes(p) > kLdsLutLimit)
+        p.lds_lut = 0;
+    const size_t lds_bytes = dxg::transportLdsBytes(p);
+    if (lds_bytes > 64 * 1024)
+        return fail(DXG_ERR_ARGUMENT, "beam tables (spectra + bowties) exceed the LDS budget");
 
     // ---- launch geometry: fill every CU to the occupancy limit
-    int cus = 0, per_cu = 0;
+    int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, w->device));
     const int block = dxg::transportBlockSize();
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dxg::transportKernelSymbol(), block, 0));
-    per_cu = std::max(per_cu, 1);
+    const int per_cu = dxg::transportBlocksPerCU(p.lds_lut != 0, lds_bytes);
     int blocks = cus * per_cu;
     if (const char* env = std::getenv("DXG_BLOCKS"))
         blocks = std::max(1, std::atoi(env));
     const uint64_t lanes = static_cast<uint64_t>(blocks) * block;
-    uint64_t chunk = std::max<uint64_t>(lanes * 64, 1ull << 20);
+    // One launch covers up to 256 histories per lane: the drain at the end of a
+    // launch is paid rarely, and progress/cancel still act every few tens of ms.
+    uint64_t chunk = std::max<uint64_t>(lanes * 256, 1ull << 20);
     if (const char* env = std::getenv("DXG_CHUNK"))
         chunk = std::max<uint64_t>(1, std::strtoull(env, nullptr, 10));
 
@@ -676,6 +713,36 @@ int dxg_debug_traverse(const dxg_world* w, uint64_t n_rays, const float* pos, co
     HIP_TRY(hipMemcpyAsync(out_index, di.ptr, n_rays * n_steps * sizeof(int64_t), hipMemcpyDeviceToHost, w->stream));
     HIP_TRY(hipMemcpyAsync(out_material, dm.ptr, n_rays * n_steps * sizeof(int32_t), hipMemcpyDeviceToHost, w->stream));
     HIP_TRY(hipStreamSynchronize(w->stream));
+    return DXG_OK;
+}
+
+int dxg_debug_device_copy(uint64_t bytes, int iters, double* ms)
+{
+    if (bytes == 0 || bytes % 16 != 0 || iters < 1)
+        return fail(DXG_ERR_ARGUMENT, "dxg_debug_device_copy: bytes must be a positive multiple of 16, iters >= 1");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(DXG_ERR_DEVICE, "no HIP device is visible");
+    DevBuf<uint8_t> a, b;
+    HIP_TRY(a.alloc(bytes));
+    HIP_TRY(b.alloc(bytes));
+    HIP_TRY(hipMemset(a.ptr, 1, bytes));
+    HIP_TRY(hipDeviceSynchronize());
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, nullptr));
+    for (int i = 0; i < iters; ++i)
+        dxg::launchCopy(a.ptr, b.ptr, bytes, nullptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, nullptr));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms)
+        *ms = t / iters;
     return DXG_OK;
 }
 
