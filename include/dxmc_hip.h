@@ -102,6 +102,7 @@ typedef struct dxg_counters {
     double deposited_energy;   /* sum of scored w*dE (keV) */
     double kernel_ms;          /* summed duration of the transport kernels */
     uint64_t kernel_launches;
+    uint64_t uncollided;       /* histories leaving the grid with no real interaction */
 } dxg_counters;
 
 const char* dxg_last_error(void);

@@ -29,7 +29,7 @@ struct DevCounters {
     double emitted;
     double escaped;
     double deposited;
-    double pad;
+    unsigned long long uncollided;   // histories that left the grid with no real interaction
 };
 
 // Packed voxel word: the FP32 density with its 8 low mantissa bits replaced

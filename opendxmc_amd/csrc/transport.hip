@@ -194,9 +194,9 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
     float maj = 1, inv_maj = 1, et = 0;
     int ej = 0;
     Rng rng { 1, 2, 3, 4 };
-    bool alive = false, done = false;
+    bool alive = false, done = false, collided = false;
 
-    unsigned long long n_steps = 0, n_real = 0, n_dep = 0, n_hist = 0;
+    unsigned long long n_steps = 0, n_real = 0, n_dep = 0, n_hist = 0, n_unc = 0;
     double e_emit = 0, e_esc = 0, e_dep = 0;
     // wavefront-private batch of pool histories [cur, end), claimed P.batch at a time
     unsigned long long cur = 0, end = 0;
@@ -229,6 +229,7 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
                     done = true;
                 } else {
                     ++n_hist;
+                    collided = false;
                     rngSeed(rng, P.seed, h + P.rng_offset);
                     // exposure containing beam history h
                     int ei;
@@ -323,6 +324,7 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
         const int vi = voxelOf(P, px, py, pz);
         if (vi < 0) {
             e_esc += static_cast<double>(w) * static_cast<double>(E);
+            n_unc += collided ? 0ull : 1ull;
             alive = false;
             continue;
         }
@@ -335,6 +337,7 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
 
         // ---- real interaction
         ++n_real;
+        collided = true;
         const float4 a = P.mu_parts[m * kLutNd + ej];
         const float4 b = P.mu_parts[m * kLutNd + ej + 1];
         const float c_pe = __fmaf_rn(et, b.x - a.x, a.x);
@@ -435,6 +438,7 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
     n_real = waveSum(n_real);
     n_dep = waveSum(n_dep);
     n_hist = waveSum(n_hist);
+    n_unc = waveSum(n_unc);
     e_emit = waveSum(e_emit);
     e_esc = waveSum(e_esc);
     e_dep = waveSum(e_dep);
@@ -443,6 +447,7 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
         atomicAdd(&P.counters->real_events, n_real);
         atomicAdd(&P.counters->deposits, n_dep);
         atomicAdd(&P.counters->histories, n_hist);
+        atomicAdd(&P.counters->uncollided, n_unc);
         atomicAdd(&P.counters->emitted, e_emit);
         atomicAdd(&P.counters->escaped, e_esc);
         atomicAdd(&P.counters->deposited, e_dep);
@@ -489,7 +494,7 @@ __global__ void reset_kernel(unsigned long long* pool, DevCounters* c)
         c->emitted = 0;
         c->escaped = 0;
         c->deposited = 0;
-        c->pad = 0;
+        c->uncollided = 0;
     }
 }
 

@@ -556,6 +556,7 @@ int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress
         acc.emitted_energy += c.emitted;
         acc.escaped_energy += c.escaped;
         acc.deposited_energy += c.deposited;
+        acc.uncollided += c.uncollided;
         w->histories += p.h_end - p.h_begin;
         if (progress)
             progress->n.store(p.h_end);

@@ -312,6 +312,7 @@ struct oracle_counters {
     double emitted_energy;   // sum w E over emitted photons entering or missing the grid
     double escaped_energy;   // sum w E leaving the grid box
     double deposited_energy; // sum of scored w dE
+    uint64_t uncollided;     // histories that left the grid without a real interaction
 };
 
 }  // extern "C"
@@ -505,7 +506,7 @@ struct Run {
 };
 
 struct Acc {
-    uint64_t steps = 0, real = 0, deposits = 0;
+    uint64_t steps = 0, real = 0, deposits = 0, uncollided = 0;
     double emitted = 0, escaped = 0, deposited = 0;
 };
 
@@ -561,6 +562,7 @@ void history(const Run& R, uint64_t h, Acc& acc)
     const size_t nm = g.mats.size();
     std::vector<double> mu_pe(nm), mu_inc(nm), mu_coh(nm);
     double maj = 0, last_E = -1;
+    bool collided = false;
     for (;;) {
         if (ph.E != last_E) {
             maj = 0;
@@ -581,6 +583,8 @@ void history(const Run& R, uint64_t h, Acc& acc)
         const long long vi = voxel(g, ph.p);
         if (vi < 0) {
             acc.escaped += ph.w * ph.E;
+            if (!collided)
+                acc.uncollided++;
             return;
         }
         const uint8_t m = g.material[static_cast<size_t>(vi)];
@@ -589,6 +593,7 @@ void history(const Run& R, uint64_t h, Acc& acc)
         if (rng.u() * maj > mtot * rho)
             continue;   // virtual collision
         acc.real++;
+        collided = true;
         const double r = rng.u() * mtot;
         if (r < mu_pe[m]) {
             score(R, vi, ph.w * ph.E, acc);
@@ -872,6 +877,7 @@ int oracle_transport(const uint64_t dims[3], const double spacing[3], const doub
             c.emitted_energy += a.emitted;
             c.escaped_energy += a.escaped;
             c.deposited_energy += a.deposited;
+            c.uncollided += a.uncollided;
         }
         *counters = c;
     }

@@ -34,7 +34,8 @@ class Bowtie(C.Structure):
 
 class Counters(C.Structure):
     _fields_ = [("histories", u64), ("steps", u64), ("real_events", u64), ("deposits", u64),
-                ("emitted_energy", C.c_double), ("escaped_energy", C.c_double), ("deposited_energy", C.c_double)]
+                ("emitted_energy", C.c_double), ("escaped_energy", C.c_double), ("deposited_energy", C.c_double),
+                ("uncollided", u64)]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -210,7 +210,29 @@ def test_variance_estimator_matches_seed_spread():
     assert 0.7 < ratio < 1.3
 
 
-def test_level2_is_reported_unsupported():
-    with pytest.raises(Exception) as e:
-        dx.AAVoxelGrid(correction=2)
-    assert "-7" in str(e.value)
+@pytest.mark.parametrize("level", [0, 1])
+def test_woodcock_beer_lambert(level):
+    """Uncollided fraction through water 2 cm / bone 1 cm / lung 3 cm equals
+    exp(-sum mu rho t) on the device (FP32 tables, binned majorant)."""
+    dims = (8, 8, 60)
+    mat = np.zeros(dims[::-1], dtype=np.uint8)
+    dens = np.zeros(dims[::-1])
+    mat[0:20], dens[0:20] = 1, 1.0
+    mat[20:30], dens[20:30] = 2, 1.92
+    mat[30:60], dens[30:60] = 3, 0.26
+    comps = [phantoms.AIR, phantoms.WATER, phantoms.BONE, phantoms.LUNG]
+    ph = phantoms.Phantom(dims, (0.1, 0.1, 0.1), dens.ravel(), mat.ravel(), comps, ["a", "w", "b", "l"])
+    _, grid = make_world(ph, level)
+    beam = dx.PencilBeam((0.05, 0.05, -10.0), (0, 0, 1), 60.0)
+    n = 4_000_000
+    beam.setNumberOfParticlesPerExposure(n)
+    world = dx.World()
+    world.items.append(grid)
+    assert dx.Transport(seed=21)(world, beam)
+    c = grid.counters()
+    mu = [dx.Material(m).attenuationValues([60.0])[0, 3] for m in comps]
+    expected = math.exp(-(mu[1] * 2.0 + mu[2] * 1.92 * 1.0 + mu[3] * 0.26 * 3.0))
+    got = c["uncollided"] / n
+    se = math.sqrt(expected * (1 - expected) / n)
+    print(f"uncollided {got:.6f} expected {expected:.6f} ({(got - expected) / se:+.2f} SE)", flush=True)
+    assert abs(got - expected) < 4 * se + 2e-4 * expected   # + FP32 table interpolation allowance
