@@ -75,10 +75,24 @@ def build_oracle() -> Path:
     return ORACLE_LIB
 
 
+HARNESS_SRC = ROOT / "tests" / "cpp" / "worker_harness.cpp"
+HARNESS = ROOT / "tests" / "cpp" / "_build" / "worker_harness"
+
+
+def build_harness() -> Path:
+    """The C++ reproduction of the reference's worker<C>, on the C++ facade."""
+    HARNESS.parent.mkdir(parents=True, exist_ok=True)
+    _run(["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", str(HARNESS_SRC),
+          f"-L{PKG}", "-ldxmc_hip", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../../opendxmc_amd",
+          "-o", str(HARNESS)])
+    return HARNESS
+
+
 def build_all() -> None:
     build_physics_list()
     build_library()
     build_oracle()
+    build_harness()
 
 
 if __name__ == "__main__":
