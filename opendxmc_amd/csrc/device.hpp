@@ -92,6 +92,7 @@ struct DevParams {
     unsigned long long* pool;        // history pool counter, zero at launch
     DevCounters* counters;
     int refill_threshold;            // dead lanes that trigger a wavefront refill
+    int interact_threshold;          // parked lanes that trigger the interaction phase
     int lds_lut;                     // attenuation table staged in LDS
     unsigned long long batch;        // histories a wavefront claims per pool atomic (>= 64)
 };

@@ -1,20 +1,32 @@
 // HIP/CDNA4 kernels of the voxel photon-transport engine.
 //
 // transport_kernel is a persistent history-pool kernel.  Every lane carries
-// one photon in registers (FP32 state).  When a wavefront has enough dead
-// lanes (absorbed, escaped, missed the grid) -- refill_threshold of them, or
-// all -- it claims exactly that many new histories from a global pool counter
-// with ONE atomic and hands them to the dead lanes in lane order, so the
-// expensive source-sampling path is paid once for many lanes.  Each loop
-// iteration every live lane advances one Woodcock step: one 4-byte gather of
-// the packed voxel word (density + material id), one LDS read of the
-// material's attenuation at the photon's energy bin, one compare.  Deposits go
-// straight to the FP64 per-voxel tally with hardware float atomics; event
-// counters are reduced across the wavefront before one atomic per wavefront.
+// one photon in registers (FP32 state) and is in one of four states:
+//   DEAD      needs a new history,
+//   TRACK     Woodcock tracking,
+//   INTERACT  parked at a real collision, waiting for its interaction,
+//   DONE      the history pool is exhausted.
+// Each loop iteration a wavefront runs exactly ONE phase, chosen from ballots
+// of the lane states, with the lanes of the other states masked off:
+//   * refill   once refill_threshold lanes are DEAD (or nothing else can run):
+//              they take consecutive ids from the wavefront's private batch of
+//              pool histories (one global atomic per `batch` histories) and
+//              sample their source particle together;
+//   * interact once interact_threshold lanes are parked (or no lane tracks):
+//              photoelectric / Compton / Rayleigh for all parked lanes at once;
+//   * step     otherwise: every TRACK lane advances one Woodcock step -- one
+//              4-byte gather of the packed voxel word (density + material id),
+//              one LDS read of the attenuation at the photon's energy bin.
+// The expensive branches therefore run with many lanes active instead of
+// stalling a full wavefront for a few lanes.  The per-history random streams
+// are consumed in the same order in every phase schedule, so results do not
+// depend on thresholds, launch geometry or lane placement.
 //
-// LDS holds, per block: the Woodcock majorant table, the per-material total
-// attenuation table (when it fits), and the beam's spectrum alias tables and
-// bowtie nodes.
+// Deposits go straight to the FP64 per-voxel tally with hardware float
+// atomics; event counters are summed across the wavefront before one atomic
+// per wavefront.  LDS holds, per block: the Woodcock majorant table, the
+// per-material total attenuation table (when it fits), and the beam's
+// spectrum alias tables and bowtie nodes.
 //
 // Algorithms (DXMClib's AAVoxelGrid::transport and interactions<5, C>, absent
 // from the reference tree; the caller is simulationpipeline.cpp:159-170):
@@ -39,6 +51,8 @@ constexpr float kInvMeC2 = 1.0f / 510.99895f;
 constexpr float kCutoff = 1.0f;
 constexpr float kTwoPi = 6.283185307179586f;
 constexpr int kBlock = 256;
+
+enum LaneState : int { kDead = 0, kTrack = 1, kInteract = 2, kDone = 3 };
 
 struct Rng {
     uint32_t s0, s1, s2, s3;
@@ -153,14 +167,6 @@ __device__ inline void tallyAdd(const DevParams& P, int idx, float e)
     atomicAdd(reinterpret_cast<unsigned long long*>(t + 2), 1ull);
 }
 
-template <typename T>
-__device__ inline T waveSum(T v)
-{
-    for (int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 __device__ inline float lerpTab(const float* tab, int j, float t)
 {
     const float a = tab[j];
@@ -189,46 +195,61 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
     __syncthreads();
     const float* mu_tab = LDS_LUT ? s_mu : P.mu_tot;
 
+    // per-wavefront FP64 energy sums (emitted, escaped, deposited) in LDS
+    __shared__ double s_energy[kBlock / 64][3];
     const int lane = static_cast<int>(__lane_id());
-    float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 1, E = 0, w = 0;
-    float maj = 1, inv_maj = 1, et = 0;
-    int ej = 0;
-    Rng rng { 1, 2, 3, 4 };
-    bool alive = false, done = false, collided = false;
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    if (lane < 3)
+        s_energy[wave][lane] = 0.0;
+    __syncthreads();
+    double* const e_wave = s_energy[wave];
 
+    float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 1, E = 0, w = 0;
+    float maj = 1, et = 0;
+    int ej = 0, vi = 0;
+    int state = kDead;
+    Rng rng { 1, 2, 3, 4 };
+    bool collided = false;
+
+    // wavefront-uniform event counters (popcounts of ballots, kept in SGPRs)
     unsigned long long n_steps = 0, n_real = 0, n_dep = 0, n_hist = 0, n_unc = 0;
-    double e_emit = 0, e_esc = 0, e_dep = 0;
     // wavefront-private batch of pool histories [cur, end), claimed P.batch at a time
     unsigned long long cur = 0, end = 0;
 
     for (;;) {
-        // ---- refill: once enough lanes are dead, they take the next histories together
-        const bool need = !alive && !done;
-        const unsigned long long need_mask = __ballot(need);
-        const int n_need = __popcll(need_mask);
-        if (n_need > 0 && (n_need >= P.refill_threshold || __ballot(alive) == 0)) {
-            const unsigned long long rank = __popcll(need_mask & ((1ull << lane) - 1ull));
+        const unsigned long long m_dead = __ballot(state == kDead);
+        const unsigned long long m_int = __ballot(state == kInteract);
+        const unsigned long long m_track = __ballot(state == kTrack);
+        const int n_dead = __popcll(m_dead), n_int = __popcll(m_int), n_track = __popcll(m_track);
+        if (n_dead + n_int + n_track == 0)
+            break;
+
+        // ------------------------------------------------------------ refill
+        if (n_dead > 0 && (n_dead >= P.refill_threshold || (n_track == 0 && n_int == 0))) {
+            const unsigned long long rank = __popcll(m_dead & ((1ull << lane) - 1ull));
             const unsigned long long avail = end - cur;
             unsigned long long mine;
-            if (avail >= static_cast<unsigned long long>(n_need)) {
+            if (avail >= static_cast<unsigned long long>(n_dead)) {
                 mine = cur + rank;
-                cur += n_need;
+                cur += n_dead;
             } else {
                 // the rest of this batch first, then a fresh batch from the pool
+                const int leader = __ffsll(static_cast<long long>(m_dead)) - 1;
                 unsigned long long base = 0;
-                if (lane == __ffsll(static_cast<long long>(need_mask)) - 1)
+                if (lane == leader)
                     base = atomicAdd(P.pool, static_cast<unsigned long long>(P.batch));
-                base = __shfl(base, __ffsll(static_cast<long long>(need_mask)) - 1, 64);
+                base = __shfl(base, leader, 64);
                 mine = rank < avail ? cur + rank : base + (rank - avail);
-                cur = base + (n_need - avail);
+                cur = base + (n_dead - avail);
                 end = base + P.batch;
             }
-            if (need) {
+            bool started = false;
+            if (state == kDead) {
                 const unsigned long long h = P.h_begin + mine;
                 if (h >= P.h_end) {
-                    done = true;
+                    state = kDone;
                 } else {
-                    ++n_hist;
+                    started = true;
                     collided = false;
                     rngSeed(rng, P.seed, h + P.rng_offset);
                     // exposure containing beam history h
@@ -293,155 +314,156 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
                         }
                         w *= bw;
                     }
-                    e_emit += static_cast<double>(w) * static_cast<double>(E);
-                    alive = boxEntry(P, px, py, pz, dx, dy, dz);
-                    if (alive) {
+                    const double we = static_cast<double>(w) * static_cast<double>(E);
+                    atomicAdd(&e_wave[0], we);
+                    if (boxEntry(P, px, py, pz, dx, dy, dz)) {
                         const float f = (E - kLutE0f) * kLutInvDEf;
                         ej = min(static_cast<int>(f), kLutNd - 2);
                         et = f - static_cast<float>(ej);
                         maj = s_maj[ej];
-                        inv_maj = 1.0f / maj;
+                        state = kTrack;
                     } else {
-                        e_esc += static_cast<double>(w) * static_cast<double>(E);
+                        atomicAdd(&e_wave[1], we);
                     }
                 }
             }
-        }
-        if (__ballot(alive) == 0) {
-            if (__ballot(!done) == 0)
-                break;
+            n_hist += __popcll(__ballot(started));
             continue;
         }
-        if (!alive)
-            continue;
 
-        // ---- one Woodcock step
-        const float s = -__logf(rngU(rng)) * inv_maj;
-        px = __fmaf_rn(s, dx, px);
-        py = __fmaf_rn(s, dy, py);
-        pz = __fmaf_rn(s, dz, pz);
-        ++n_steps;
-        const int vi = voxelOf(P, px, py, pz);
-        if (vi < 0) {
-            e_esc += static_cast<double>(w) * static_cast<double>(E);
-            n_unc += collided ? 0ull : 1ull;
-            alive = false;
-            continue;
-        }
-        const uint32_t word = P.voxels[vi];
-        const int m = static_cast<int>(word & 0xFFu);
-        const float rho = asFloat(word & 0xFFFFFF00u);
-        const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
-        if (rngU(rng) * maj > mu)
-            continue;   // virtual collision
-
-        // ---- real interaction
-        ++n_real;
-        collided = true;
-        const float4 a = P.mu_parts[m * kLutNd + ej];
-        const float4 b = P.mu_parts[m * kLutNd + ej + 1];
-        const float c_pe = __fmaf_rn(et, b.x - a.x, a.x);
-        const float c_inc = __fmaf_rn(et, b.y - a.y, a.y);
-        const float c_tot = __fmaf_rn(et, b.z - a.z, a.z);
-        const float r = rngU(rng) * c_tot;
-        if (r < c_pe) {
-            const float de = w * E;
-            tallyAdd(P, vi, de);
-            ++n_dep;
-            e_dep += static_cast<double>(de);
-            alive = false;
-            continue;
-        }
-        float cost;
-        if (r < c_inc) {
-            // Compton: Klein-Nishina (Butcher & Messel), level 1 rejects on S(x)/Z
-            const float k = E * kInvMeC2;
-            const float e0 = 1.0f / (1.0f + 2.0f * k);
-            const float a1 = __logf(1.0f + 2.0f * k);
-            const float a2 = 0.5f * (1.0f - e0 * e0);
-            float eps;
-            for (;;) {
-                const float r1 = rngU(rng), r2 = rngU(rng), r3 = rngU(rng);
-                if (r1 * (a1 + a2) < a1)
-                    eps = __expf(-a1 * r2);
-                else
-                    eps = sqrtf(e0 * e0 + (1.0f - e0 * e0) * r2);
-                const float t = (1.0f - eps) / (k * eps);
-                const float sin2 = t * (2.0f - t);
-                const float g = 1.0f - eps * sin2 / (1.0f + eps * eps);
-                if (r3 > g)
-                    continue;
-                cost = 1.0f - t;
-                if (P.level == 0)
-                    break;
-                const float x = E * kInvHc * sqrtf(fmaxf(0.0f, 0.5f * (1.0f - cost)));
-                const float fx = x * (static_cast<float>(kSfNd - 1) / kXMaxf);
-                float sv = 1.0f;
-                if (fx < static_cast<float>(kSfNd - 1)) {
-                    const int kk = static_cast<int>(fx);
-                    sv = lerpTab(P.s_table + m * kSfNd, kk, fx - static_cast<float>(kk));
+        // -------------------------------------------------------- interact
+        if (n_int > 0 && (n_int >= P.interact_threshold || n_track == 0)) {
+            int deps = 0;
+            double e_scored = 0.0;
+            if (state == kInteract) {
+                collided = true;
+                state = kTrack;
+                const int m = static_cast<int>(P.voxels[vi] & 0xFFu);
+                const float4 a = P.mu_parts[m * kLutNd + ej];
+                const float4 b = P.mu_parts[m * kLutNd + ej + 1];
+                const float c_pe = __fmaf_rn(et, b.x - a.x, a.x);
+                const float c_inc = __fmaf_rn(et, b.y - a.y, a.y);
+                const float c_tot = __fmaf_rn(et, b.z - a.z, a.z);
+                const float r = rngU(rng) * c_tot;
+                if (r < c_pe) {
+                    const float de = w * E;
+                    tallyAdd(P, vi, de);
+                    deps = 1;
+                    e_scored = de;
+                    state = kDead;
+                } else if (r < c_inc) {
+                    // Compton: Klein-Nishina (Butcher & Messel), level 1 rejects on S(x)/Z
+                    const float k = E * kInvMeC2;
+                    const float e0 = 1.0f / (1.0f + 2.0f * k);
+                    const float a1 = __logf(1.0f + 2.0f * k);
+                    const float a2 = 0.5f * (1.0f - e0 * e0);
+                    float eps, cost;
+                    for (;;) {
+                        const float r1 = rngU(rng), r2 = rngU(rng), r3 = rngU(rng);
+                        if (r1 * (a1 + a2) < a1)
+                            eps = __expf(-a1 * r2);
+                        else
+                            eps = sqrtf(e0 * e0 + (1.0f - e0 * e0) * r2);
+                        const float t = (1.0f - eps) / (k * eps);
+                        const float sin2 = t * (2.0f - t);
+                        const float g = 1.0f - eps * sin2 / (1.0f + eps * eps);
+                        if (r3 > g)
+                            continue;
+                        cost = 1.0f - t;
+                        if (P.level == 0)
+                            break;
+                        const float x = E * kInvHc * sqrtf(fmaxf(0.0f, 0.5f * (1.0f - cost)));
+                        const float fx = x * (static_cast<float>(kSfNd - 1) / kXMaxf);
+                        float sv = 1.0f;
+                        if (fx < static_cast<float>(kSfNd - 1)) {
+                            const int kk = static_cast<int>(fx);
+                            sv = lerpTab(P.s_table + m * kSfNd, kk, fx - static_cast<float>(kk));
+                        }
+                        if (rngU(rng) <= sv)
+                            break;
+                    }
+                    const float enew = E * eps;
+                    const float de = w * (E - enew);
+                    tallyAdd(P, vi, de);
+                    deps = 1;
+                    e_scored = de;
+                    E = enew;
+                    rotate(dx, dy, dz, cost, kTwoPi * rngU(rng));
+                    if (E < kCutoff) {
+                        const float rest = w * E;
+                        tallyAdd(P, vi, rest);
+                        deps = 2;
+                        e_scored += rest;
+                        state = kDead;
+                    } else {
+                        const float f = (E - kLutE0f) * kLutInvDEf;
+                        ej = min(static_cast<int>(f), kLutNd - 2);
+                        et = f - static_cast<float>(ej);
+                        maj = s_maj[ej];
+                    }
+                } else {
+                    // Rayleigh
+                    float cost;
+                    if (P.level == 0) {
+                        const float q = 4.0f - 8.0f * rngU(rng);
+                        const float aa = -0.5f * q + sqrtf(0.25f * q * q + 1.0f);
+                        const float u = cbrtf(aa);
+                        cost = fminf(1.0f, fmaxf(-1.0f, u - 1.0f / u));
+                    } else {
+                        const float xm = E * kInvHc;
+                        const float umax = xm * xm;
+                        const float fx = xm * (static_cast<float>(kSfNd - 1) / kXMaxf);
+                        float amax = 1.0f;
+                        if (fx < static_cast<float>(kSfNd - 1)) {
+                            const int kk = static_cast<int>(fx);
+                            amax = lerpTab(P.ray_fwd + m * kSfNd, kk, fx - static_cast<float>(kk));
+                        }
+                        for (;;) {
+                            const float fy = rngU(rng) * amax * static_cast<float>(kRayInvNd - 1);
+                            const int kk = min(static_cast<int>(fy), kRayInvNd - 2);
+                            const float u = fminf(umax, lerpTab(P.ray_inv + m * kRayInvNd, kk, fy - static_cast<float>(kk)) * (kXMaxf * kXMaxf));
+                            cost = 1.0f - 2.0f * u / umax;
+                            if (2.0f * rngU(rng) <= 1.0f + cost * cost)
+                                break;
+                        }
+                    }
+                    rotate(dx, dy, dz, cost, kTwoPi * rngU(rng));
                 }
-                if (rngU(rng) <= sv)
-                    break;
             }
-            const float enew = E * eps;
-            const float de = w * (E - enew);
-            tallyAdd(P, vi, de);
-            ++n_dep;
-            e_dep += static_cast<double>(de);
-            E = enew;
-            rotate(dx, dy, dz, cost, kTwoPi * rngU(rng));
-            if (E < kCutoff) {
-                const float rest = w * E;
-                tallyAdd(P, vi, rest);
-                ++n_dep;
-                e_dep += static_cast<double>(rest);
-                alive = false;
-                continue;
-            }
-            const float f = (E - kLutE0f) * kLutInvDEf;
-            ej = min(static_cast<int>(f), kLutNd - 2);
-            et = f - static_cast<float>(ej);
-            maj = s_maj[ej];
-            inv_maj = 1.0f / maj;
-        } else {
-            // Rayleigh
-            if (P.level == 0) {
-                const float q = 4.0f - 8.0f * rngU(rng);
-                const float aa = -0.5f * q + sqrtf(0.25f * q * q + 1.0f);
-                const float u = cbrtf(aa);
-                cost = fminf(1.0f, fmaxf(-1.0f, u - 1.0f / u));
+            if (deps > 0)
+                atomicAdd(&e_wave[2], e_scored);
+            n_real += n_int;
+            n_dep += __popcll(__ballot(deps > 0)) + __popcll(__ballot(deps > 1));
+            continue;
+        }
+
+        // ------------------------------------------------------------ step
+        bool uncollided_exit = false;
+        if (state == kTrack) {
+            const float s = __fdividef(-__logf(rngU(rng)), maj);
+            px = __fmaf_rn(s, dx, px);
+            py = __fmaf_rn(s, dy, py);
+            pz = __fmaf_rn(s, dz, pz);
+            vi = voxelOf(P, px, py, pz);
+            if (vi < 0) {
+                atomicAdd(&e_wave[1], static_cast<double>(w) * static_cast<double>(E));
+                uncollided_exit = !collided;
+                state = kDead;
             } else {
-                const float xm = E * kInvHc;
-                const float umax = xm * xm;
-                const float fx = xm * (static_cast<float>(kSfNd - 1) / kXMaxf);
-                float amax = 1.0f;
-                if (fx < static_cast<float>(kSfNd - 1)) {
-                    const int kk = static_cast<int>(fx);
-                    amax = lerpTab(P.ray_fwd + m * kSfNd, kk, fx - static_cast<float>(kk));
-                }
-                for (;;) {
-                    const float fy = rngU(rng) * amax * static_cast<float>(kRayInvNd - 1);
-                    const int kk = min(static_cast<int>(fy), kRayInvNd - 2);
-                    const float u = fminf(umax, lerpTab(P.ray_inv + m * kRayInvNd, kk, fy - static_cast<float>(kk)) * (kXMaxf * kXMaxf));
-                    cost = 1.0f - 2.0f * u / umax;
-                    if (2.0f * rngU(rng) <= 1.0f + cost * cost)
-                        break;
-                }
+                const uint32_t word = P.voxels[vi];
+                const int m = static_cast<int>(word & 0xFFu);
+                const float rho = asFloat(word & 0xFFFFFF00u);
+                const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
+                if (!(rngU(rng) * maj > mu))
+                    state = kInteract;   // real collision; virtual ones keep tracking
             }
-            rotate(dx, dy, dz, cost, kTwoPi * rngU(rng));
         }
+        n_steps += n_track;
+        n_unc += __popcll(__ballot(uncollided_exit));
     }
 
-    // ---- wavefront reduction of the counters, one atomic per wavefront
-    n_steps = waveSum(n_steps);
-    n_real = waveSum(n_real);
-    n_dep = waveSum(n_dep);
-    n_hist = waveSum(n_hist);
-    n_unc = waveSum(n_unc);
-    e_emit = waveSum(e_emit);
-    e_esc = waveSum(e_esc);
-    e_dep = waveSum(e_dep);
+    // ---- one atomic per wavefront and counter
+    const double e_emit = e_wave[0], e_esc = e_wave[1], e_dep = e_wave[2];
     if (lane == 0) {
         atomicAdd(&P.counters->steps, n_steps);
         atomicAdd(&P.counters->real_events, n_real);
@@ -498,6 +520,12 @@ __global__ void reset_kernel(unsigned long long* pool, DevCounters* c)
     }
 }
 
+__global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n)
+{
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
 size_t transportLdsBytes(const DevParams& p)
 {
     return sizeof(uint32_t) * (static_cast<size_t>(kLutNd) + (p.lds_lut ? static_cast<size_t>(p.n_mat) * kLutNd : 0)
@@ -534,22 +562,12 @@ void launchReset(unsigned long long* pool, DevCounters* counters, hipStream_t s)
     hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, pool, counters);
 }
 
-int transportBlockSize() { return kBlock; }
-
-}  // namespace dxg
-
-namespace dxg {
-
-__global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n)
-{
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-        dst[i] = src[i];
-}
-
 void launchCopy(const void* src, void* dst, uint64_t bytes, hipStream_t s)
 {
     const uint64_t n = bytes / sizeof(float4);
     hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s, static_cast<const float4*>(src), static_cast<float4*>(dst), n);
 }
+
+int transportBlockSize() { return kBlock; }
 
 }  // namespace dxg

@@ -490,6 +490,9 @@ int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress
     p.refill_threshold = 16;
     if (const char* env = std::getenv("DXG_REFILL"))
         p.refill_threshold = std::clamp(std::atoi(env), 1, 64);
+    p.interact_threshold = 24;
+    if (const char* env = std::getenv("DXG_INTERACT"))
+        p.interact_threshold = std::clamp(std::atoi(env), 1, 64);
     p.batch = 256;
     if (const char* env = std::getenv("DXG_BATCH"))
         p.batch = std::max<unsigned long long>(64, std::strtoull(env, nullptr, 10));

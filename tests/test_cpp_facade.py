@@ -36,7 +36,8 @@ def write_phantom(path: Path, ph: phantoms.Phantom) -> None:
 
 def write_bowtie(path: Path) -> dx.BowtieFilter:
     b = read_filters()[DEFAULT_CT_BOWTIE]
-    path.write_text("\n".join(f"{a!r} {w!r}" for a, w in zip(b.angle, b.weight)))
+    path.write_text("\n".join(f"{float(a)!r} {float(w)!r}" for a, w in zip(b.angle, b.weight)))
+    assert len(path.read_text().split()) == 2 * len(b.angle)
     return b
 
 
