@@ -73,6 +73,11 @@ struct DevParams {
     const float* s_table;
     const float* ray_fwd;
     const float* ray_inv;
+    // correction level 2 (see physics.hpp DeviceTables)
+    const float4* ia_shell;
+    const float* ia_inv;
+    const float* fl_cum;
+    const float* fl_line;
 
     // tallies: per voxel {sum w dE, sum (w dE)^2, events (u64 bits), unused}
     double* tally;
@@ -106,6 +111,6 @@ void launchReset(unsigned long long* pool, DevCounters* counters, hipStream_t s)
 int transportBlockSize();
 void launchCopy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 // Resident blocks per CU of the transport kernel at the given LDS size.
-int transportBlocksPerCU(bool lds_lut, size_t lds_bytes);
+int transportBlocksPerCU(bool lds_lut, int level, size_t lds_bytes);
 
 }  // namespace dxg

@@ -78,7 +78,7 @@ bool PhysicsList::load(const std::string& path, std::string& err)
     char magic[8] = {};
     f.read(magic, 8);
     uint32_t zmax = 0, ne = 0, nx = 0;
-    if (std::memcmp(magic, "DXGPHYS1", 8) != 0 || !readOne(f, zmax) || !readOne(f, ne) || !readOne(f, nx)) {
+    if (std::memcmp(magic, "DXGPHYS2", 8) != 0 || !readOne(f, zmax) || !readOne(f, ne) || !readOne(f, nx)) {
         err = "physics list '" + path + "' has a bad header";
         return false;
     }
@@ -92,14 +92,25 @@ bool PhysicsList::load(const std::string& path, std::string& err)
         log_e_[i] = std::log(e[i]);
     std::vector<ElementData> els(zmax);
     for (auto& el : els) {
-        std::vector<double> inc, coh, pe_e, pe, shells;
+        std::vector<double> inc, coh, pe_e, pe, shells, fl;
         uint32_t npe = 0, nsh = 0;
         bool ok = readOne(f, el.A) && readVec(f, inc, ne) && readVec(f, coh, ne) && readVec(f, el.F, nx)
             && readVec(f, el.S, nx) && readOne(f, npe) && readVec(f, pe_e, npe) && readVec(f, pe, npe)
-            && readOne(f, nsh) && readVec(f, shells, static_cast<size_t>(nsh) * 5);
-        if (!ok || el.A <= 0) {
+            && readVec(f, el.pe_k, npe) && readOne(f, nsh) && readVec(f, shells, static_cast<size_t>(nsh) * 6)
+            && readVec(f, fl, 7);
+        if (!ok || el.A <= 0 || nsh == 0) {
             err = "physics list element block truncated";
             return false;
+        }
+        for (uint32_t s = 0; s < nsh; ++s) {
+            const double* r = &shells[6 * s];   // n, l, occupancy, binding, zeff, n*
+            el.shells.push_back({ r[2], r[3], r[4] / r[5] });
+        }
+        el.k_edge = shells[3];
+        el.omega_k = fl[0];
+        for (int i = 0; i < 3; ++i) {
+            el.line_e[i] = fl[1 + i];
+            el.line_p[i] = fl[4 + i];
         }
         el.log_inc.resize(ne);
         el.log_coh.resize(ne);
@@ -123,6 +134,22 @@ double PhysicsList::photo(int z, double e) const
     const auto& el = element(z);
     return interpLogLog(el.pe_log_e, el.pe_log_v, std::log(e));
 }
+double PhysicsList::photoK(int z, double e) const
+{
+    const auto& el = element(z);
+    if (e <= el.k_edge)
+        return 0.0;
+    // the K part is zero below the edge; above it interpolate like the total
+    const auto it = std::upper_bound(el.pe_log_e.begin(), el.pe_log_e.end(), std::log(e));
+    const size_t b = std::min(static_cast<size_t>(it - el.pe_log_e.begin()), el.pe_log_e.size() - 1);
+    const size_t a = b == 0 ? 0 : b - 1;
+    const double va = el.pe_k[a], vb = el.pe_k[b];
+    if (va <= 0 || vb <= 0 || b == a)
+        return std::max(vb, 0.0);
+    const double t = (std::log(e) - el.pe_log_e[a]) / (el.pe_log_e[b] - el.pe_log_e[a]);
+    return std::exp(std::log(va) + t * (std::log(vb) - std::log(va)));
+}
+
 double PhysicsList::incoherent(int z, double e) const { return interpLogLog(log_e_, element(z).log_inc, std::log(e)); }
 double PhysicsList::coherent(int z, double e) const { return interpLogLog(log_e_, element(z).log_coh, std::log(e)); }
 double PhysicsList::formFactor(int z, double x) const { return interpLinear(x_, element(z).F, x); }
@@ -192,7 +219,8 @@ bool makeMaterial(const PhysicsList& pl, const int32_t* Z, const double* w, uint
     return true;
 }
 
-void buildTables(const PhysicsList& pl, const std::vector<MixMaterial>& mats, const std::vector<double>& rho_max, DeviceTables& t)
+void buildTables(const PhysicsList& pl, const std::vector<MixMaterial>& mats, const std::vector<double>& rho_max, int level,
+                 DeviceTables& t)
 {
     const int nm = static_cast<int>(mats.size());
     t.n_mat = nm;
@@ -280,6 +308,107 @@ void buildTables(const PhysicsList& pl, const std::vector<MixMaterial>& mats, co
             t.ray_inv[static_cast<size_t>(m) * kRayInvN + k] = static_cast<float>(u / (kXMax * kXMax));
         }
     }
+    if (level < 2)
+        return;
+
+    // ---- impulse approximation: per-material shells and the unit profile inverse CDF
+    t.ia_shell.assign(static_cast<size_t>(nm) * kIaShells * 4, 0.f);
+    for (int m = 0; m < nm; ++m) {
+        const auto sh = impulseShells(pl, mats[static_cast<size_t>(m)]);
+        double cum = 0;
+        for (int s = 0; s < kIaShells; ++s) {
+            float* r = &t.ia_shell[(static_cast<size_t>(m) * kIaShells + s) * 4];
+            if (s < static_cast<int>(sh.size())) {
+                cum += sh[static_cast<size_t>(s)].electron_fraction;
+                r[1] = static_cast<float>(sh[static_cast<size_t>(s)].binding_kev);
+                r[2] = static_cast<float>(sh[static_cast<size_t>(s)].p0_mec);
+            }
+            r[0] = static_cast<float>(std::min(cum, 1.0));
+        }
+        t.ia_shell[(static_cast<size_t>(m) * kIaShells + kIaShells - 1) * 4] = 1.0f;
+    }
+    t.ia_inv.assign(kIaInvN, 0.f);
+    for (int k = 0; k < kIaInvN; ++k) {
+        // invert G by bisection; the ends are clamped at +-8 (G(-8) ~ 1e-5)
+        const double u = std::clamp(static_cast<double>(k) / (kIaInvN - 1), 1e-6, 1.0 - 1e-6);
+        double lo = -8, hi = 8;
+        for (int it = 0; it < 80; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            (comptonProfileCdf(mid) < u ? lo : hi) = mid;
+        }
+        t.ia_inv[static_cast<size_t>(k)] = static_cast<float>(0.5 * (lo + hi));
+    }
+
+    // ---- K fluorescence: which element's K shell absorbed, per energy node
+    t.fl_cum.assign(static_cast<size_t>(nm) * kLutN * kFlMax, 0.f);
+    t.fl_line.assign(static_cast<size_t>(nm) * kFlMax * 8, 0.f);
+    for (int m = 0; m < nm; ++m) {
+        const MixMaterial& mat = mats[static_cast<size_t>(m)];
+        std::vector<size_t> fl;   // element slots with a trackable K line
+        for (size_t i = 0; i < mat.Z.size() && fl.size() < static_cast<size_t>(kFlMax); ++i) {
+            const ElementData& el = pl.element(mat.Z[i]);
+            if (el.omega_k > 0 && el.line_e[0] >= kCutoffKeV)
+                fl.push_back(i);
+        }
+        for (size_t k = 0; k < fl.size(); ++k) {
+            const ElementData& el = pl.element(mat.Z[fl[k]]);
+            float* r = &t.fl_line[(static_cast<size_t>(m) * kFlMax + k) * 8];
+            r[0] = static_cast<float>(el.omega_k);
+            r[1] = static_cast<float>(el.line_e[0]);
+            r[2] = static_cast<float>(el.line_e[1]);
+            r[3] = static_cast<float>(el.line_e[2]);
+            r[4] = static_cast<float>(el.line_p[0]);
+            r[5] = static_cast<float>(el.line_p[0] + el.line_p[1]);
+        }
+        for (int j = 0; j < kLutN; ++j) {
+            const double e = kLutE0 + j * kLutDE;
+            double total = 0;
+            for (size_t i = 0; i < mat.Z.size(); ++i)
+                total += mat.atoms_per_gram[i] * pl.photo(mat.Z[i], e);
+            double cum = 0;
+            for (int k = 0; k < kFlMax; ++k) {
+                if (k < static_cast<int>(fl.size()) && total > 0)
+                    cum += mat.atoms_per_gram[fl[static_cast<size_t>(k)]] * pl.photoK(mat.Z[fl[static_cast<size_t>(k)]], e) / total;
+                t.fl_cum[(static_cast<size_t>(m) * kLutN + j) * kFlMax + k] = static_cast<float>(std::min(cum, 1.0));
+            }
+        }
+    }
+}
+
+double comptonProfileCdf(double x)
+{
+    const double q = 1.0 + x * x;
+    const double prim = x / (4.0 * q * q) + 3.0 * x / (8.0 * q) + 0.375 * std::atan(x);
+    return 0.5 + prim / (0.375 * 3.14159265358979323846);
+}
+
+std::vector<IaShell> impulseShells(const PhysicsList& pl, const MixMaterial& m)
+{
+    struct S {
+        double e, b, p0;
+    };
+    std::vector<S> all;
+    double total = 0;
+    for (size_t i = 0; i < m.Z.size(); ++i)
+        for (const auto& s : pl.element(m.Z[i]).shells) {
+            const double e = m.atoms_per_gram[i] * s.occupancy;
+            all.push_back({ e, s.binding_kev, s.p0_au * kFineStructure });
+            total += e;
+        }
+    std::stable_sort(all.begin(), all.end(), [](const S& a, const S& b) { return a.b > b.b; });
+    std::vector<IaShell> out;
+    for (size_t i = 0; i < all.size() && i + 1 < static_cast<size_t>(kIaShells); ++i)
+        out.push_back({ all[i].e / total, all[i].b, all[i].p0 });
+    if (all.size() >= static_cast<size_t>(kIaShells)) {
+        double e = 0, p2 = 0, bmin = 1e300;
+        for (size_t i = kIaShells - 1; i < all.size(); ++i) {
+            e += all[i].e;
+            p2 += all[i].e * all[i].p0 * all[i].p0;
+            bmin = std::min(bmin, all[i].b);
+        }
+        out.push_back({ e / total, bmin, std::sqrt(p2 / e) });
+    }
+    return out;
 }
 
 const PhysicsList* sharedPhysics(std::string& err)

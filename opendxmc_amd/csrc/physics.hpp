@@ -33,6 +33,14 @@ struct ElementData {
     std::vector<double> log_inc, log_coh;     // on the common log-energy grid
     std::vector<double> F, S;                 // on the x grid
     std::vector<double> pe_log_e, pe_log_v;   // edge-resolved photoelectric table
+    std::vector<double> pe_k;                 // K-shell part at the same nodes (barn/atom)
+    struct Shell {
+        double occupancy, binding_kev, p0_au;   // p0 = zeff / n*: Compton profile momentum scale
+    };
+    std::vector<Shell> shells;
+    // K fluorescence: yield, line energies (Ka1, Ka2, Kb) and their probabilities
+    double omega_k = 0, line_e[3] = { 0, 0, 0 }, line_p[3] = { 0, 0, 0 };
+    double k_edge = 0;
 };
 
 class PhysicsList {
@@ -44,6 +52,7 @@ public:
 
     // barn/atom at energy E (keV), log-log interpolation
     double photo(int z, double e) const;
+    double photoK(int z, double e) const;   // K-shell part of photo()
     double incoherent(int z, double e) const;
     double coherent(int z, double e) const;
     double formFactor(int z, double x) const;
@@ -77,10 +86,32 @@ struct DeviceTables {
     std::vector<float> s_table;    // [m][k] S(x_k)/S(inf), x_k = k kXMax/(kSfN-1)
     std::vector<float> ray_fwd;    // [m][k] A(x_k)/A(kXMax), A(x) = int_0^{x^2} F^2 du
     std::vector<float> ray_inv;    // [m][k] u(y_k)/kXMax^2, y_k = k/(kRayInvN-1)
+    // correction level 2
+    std::vector<float> ia_shell;   // [m][kIaShells][4]: cumulative electron fraction, binding (keV), p0 (m_e c), 0
+    std::vector<float> ia_inv;     // [k] x(u_k) of the unit Compton profile (8/3pi)(1+x^2)^-3, u_k = k/(kIaInvN-1)
+    std::vector<float> fl_cum;     // [m][j][kFlMax]: cumulative P(K-shell absorption in fluorescing element) at E_j
+    std::vector<float> fl_line;    // [m][kFlMax][8]: omega_K, E_Ka1, E_Ka2, E_Kb, P(Ka1), P(Ka1)+P(Ka2), 0, 0
 };
 
+// Impulse approximation: the shells of a material (Material<5>: five).
+constexpr int kIaShells = 5;
+constexpr int kIaInvN = 2048;
+// Fluorescing elements per material whose K lines are tracked.
+constexpr int kFlMax = 12;
+constexpr double kFineStructure = 7.2973525693e-3;
+
+// Level-2 shell selection of one material: the four most tightly bound
+// shells and one shell lumping the rest (minimum binding, rms momentum scale).
+struct IaShell {
+    double electron_fraction, binding_kev, p0_mec;
+};
+std::vector<IaShell> impulseShells(const PhysicsList& pl, const MixMaterial& m);
+
+// Unit Compton profile CDF, G(x) = int_-inf^x (8/3pi)(1+t^2)^-3 dt.
+double comptonProfileCdf(double x);
+
 void buildTables(const PhysicsList& pl, const std::vector<MixMaterial>& mats, const std::vector<double>& rho_max,
-                 DeviceTables& t);
+                 int level, DeviceTables& t);
 
 // Locate the physics list next to the shared library (physics/physics_list.bin),
 // or at $DXG_PHYSICS_LIST.

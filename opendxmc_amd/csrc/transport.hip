@@ -51,6 +51,9 @@ constexpr float kInvMeC2 = 1.0f / 510.99895f;
 constexpr float kCutoff = 1.0f;
 constexpr float kTwoPi = 6.283185307179586f;
 constexpr int kBlock = 256;
+constexpr int kIaShellsd = 5;    // impulse-approximation shells per material
+constexpr int kIaInvNd = 2048;   // unit Compton profile inverse-CDF nodes
+constexpr int kFlMaxd = 12;      // fluorescing elements per material
 
 enum LaneState : int { kDead = 0, kTrack = 1, kInteract = 2, kDone = 3 };
 
@@ -176,9 +179,18 @@ __device__ inline float lerpTab(const float* tab, int j, float t)
 
 __device__ inline float asFloat(uint32_t u) { return __uint_as_float(u); }
 
+// Energy bin of the attenuation tables and the bin's majorant.
+__device__ inline void energyBin(float E, const float* s_maj, int& ej, float& et, float& maj)
+{
+    const float f = (E - kLutE0f) * kLutInvDEf;
+    ej = min(static_cast<int>(f), kLutNd - 2);
+    et = f - static_cast<float>(ej);
+    maj = s_maj[ej];
+}
+
 }  // namespace
 
-template <bool LDS_LUT>
+template <bool LDS_LUT, int LEVEL>
 __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
 {
     extern __shared__ uint32_t lds[];
@@ -346,43 +358,104 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
                 const float c_tot = __fmaf_rn(et, b.z - a.z, a.z);
                 const float r = rngU(rng) * c_tot;
                 if (r < c_pe) {
-                    const float de = w * E;
+                    float e_out = 0.0f;   // carried away by a K fluorescence photon (level 2)
+                    if (LEVEL == 2) {
+                        const float* cum = P.fl_cum + (static_cast<size_t>(m) * kLutNd + ej) * kFlMaxd;
+                        const float u = rngU(rng);
+                        int k = 0;
+                        for (; k < kFlMaxd; ++k)
+                            if (u < __fmaf_rn(et, cum[kFlMaxd + k] - cum[k], cum[k]))
+                                break;
+                        if (k < kFlMaxd) {
+                            const float* line = P.fl_line + (m * kFlMaxd + k) * 8;
+                            if (rngU(rng) < line[0]) {
+                                const float ul = rngU(rng);
+                                const float el = ul < line[4] ? line[1] : (ul < line[5] ? line[2] : line[3]);
+                                if (el >= kCutoff && el < E)
+                                    e_out = el;
+                            }
+                        }
+                    }
+                    const float de = w * (E - e_out);
                     tallyAdd(P, vi, de);
                     deps = 1;
                     e_scored = de;
-                    state = kDead;
+                    if (e_out > 0.0f) {
+                        // the fluorescence photon leaves the absorption point isotropically
+                        E = e_out;
+                        const float ct = 2.0f * rngU(rng) - 1.0f;
+                        const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+                        float sp, cp;
+                        __sincosf(kTwoPi * rngU(rng), &sp, &cp);
+                        dx = st * cp;
+                        dy = st * sp;
+                        dz = ct;
+                        energyBin(E, s_maj, ej, et, maj);
+                    } else {
+                        state = kDead;
+                    }
                 } else if (r < c_inc) {
-                    // Compton: Klein-Nishina (Butcher & Messel), level 1 rejects on S(x)/Z
+                    // Compton: Klein-Nishina (Butcher & Messel), level 1 rejects on S(x)/Z,
+                    // level 2 adds the impulse approximation: a shell drawn by electron
+                    // fraction, p_z from its Compton profile, the Doppler-shifted E'
                     const float k = E * kInvMeC2;
                     const float e0 = 1.0f / (1.0f + 2.0f * k);
                     const float a1 = __logf(1.0f + 2.0f * k);
                     const float a2 = 0.5f * (1.0f - e0 * e0);
-                    float eps, cost;
+                    float cost, enew;
                     for (;;) {
-                        const float r1 = rngU(rng), r2 = rngU(rng), r3 = rngU(rng);
-                        if (r1 * (a1 + a2) < a1)
-                            eps = __expf(-a1 * r2);
-                        else
-                            eps = sqrtf(e0 * e0 + (1.0f - e0 * e0) * r2);
-                        const float t = (1.0f - eps) / (k * eps);
-                        const float sin2 = t * (2.0f - t);
-                        const float g = 1.0f - eps * sin2 / (1.0f + eps * eps);
-                        if (r3 > g)
-                            continue;
-                        cost = 1.0f - t;
-                        if (P.level == 0)
-                            break;
-                        const float x = E * kInvHc * sqrtf(fmaxf(0.0f, 0.5f * (1.0f - cost)));
-                        const float fx = x * (static_cast<float>(kSfNd - 1) / kXMaxf);
-                        float sv = 1.0f;
-                        if (fx < static_cast<float>(kSfNd - 1)) {
-                            const int kk = static_cast<int>(fx);
-                            sv = lerpTab(P.s_table + m * kSfNd, kk, fx - static_cast<float>(kk));
+                        float eps;
+                        for (;;) {
+                            const float r1 = rngU(rng), r2 = rngU(rng), r3 = rngU(rng);
+                            if (r1 * (a1 + a2) < a1)
+                                eps = __expf(-a1 * r2);
+                            else
+                                eps = sqrtf(e0 * e0 + (1.0f - e0 * e0) * r2);
+                            const float t = (1.0f - eps) / (k * eps);
+                            const float sin2 = t * (2.0f - t);
+                            const float g = 1.0f - eps * sin2 / (1.0f + eps * eps);
+                            if (r3 > g)
+                                continue;
+                            cost = 1.0f - t;
+                            if (LEVEL == 0)
+                                break;
+                            const float x = E * kInvHc * sqrtf(fmaxf(0.0f, 0.5f * (1.0f - cost)));
+                            const float fx = x * (static_cast<float>(kSfNd - 1) / kXMaxf);
+                            float sv = 1.0f;
+                            if (fx < static_cast<float>(kSfNd - 1)) {
+                                const int kk = static_cast<int>(fx);
+                                sv = lerpTab(P.s_table + m * kSfNd, kk, fx - static_cast<float>(kk));
+                            }
+                            if (rngU(rng) <= sv)
+                                break;
                         }
-                        if (rngU(rng) <= sv)
+                        if (LEVEL < 2) {
+                            enew = E * eps;
+                            break;
+                        }
+                        const float us = rngU(rng);
+                        const float4* sh = P.ia_shell + m * kIaShellsd;
+                        float4 s = sh[0];
+                        for (int q = 1; q < kIaShellsd && us > s.x; ++q)
+                            s = sh[q];
+                        if (E <= s.y)
+                            continue;   // the shell cannot be ionised: draw again
+                        const float fy = rngU(rng) * static_cast<float>(kIaInvNd - 1);
+                        const int kk = min(static_cast<int>(fy), kIaInvNd - 2);
+                        const float pz = lerpTab(P.ia_inv, kk, fy - static_cast<float>(kk)) * s.z;
+                        // E'/mc^2 from p_z |q| = E E'(1-cos)/mc^2 - (E - E') (in m_e c units)
+                        const float K = 1.0f + k * (1.0f - cost);
+                        const float p2 = pz * pz;
+                        const float h = K - p2 * cost;
+                        const float den = K * K - p2;
+                        const float disc = h * h - den * (1.0f - p2);
+                        if (!(den > 0.0f) || !(disc >= 0.0f))
+                            continue;
+                        const float sq = sqrtf(disc);
+                        enew = k * (h + (pz >= 0.0f ? sq : -sq)) / den * (1.0f / kInvMeC2);
+                        if (enew > 0.0f && E - enew >= s.y)
                             break;
                     }
-                    const float enew = E * eps;
                     const float de = w * (E - enew);
                     tallyAdd(P, vi, de);
                     deps = 1;
@@ -404,7 +477,7 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
                 } else {
                     // Rayleigh
                     float cost;
-                    if (P.level == 0) {
+                    if (LEVEL == 0) {
                         const float q = 4.0f - 8.0f * rngU(rng);
                         const float aa = -0.5f * q + sqrtf(0.25f * q * q + 1.0f);
                         const float u = cbrtf(aa);
@@ -532,20 +605,33 @@ size_t transportLdsBytes(const DevParams& p)
                                + static_cast<size_t>(p.bb.words));
 }
 
+namespace {
+
+// The kernel instance for (LDS table placement, correction level).
+const void* transportInstance(bool lds_lut, int level)
+{
+    using K = void (*)(DevParams);
+    static const K table[2][3] = {
+        { &transport_kernel<false, 0>, &transport_kernel<false, 1>, &transport_kernel<false, 2> },
+        { &transport_kernel<true, 0>, &transport_kernel<true, 1>, &transport_kernel<true, 2> },
+    };
+    return reinterpret_cast<const void*>(table[lds_lut ? 1 : 0][level < 0 ? 0 : (level > 2 ? 2 : level)]);
+}
+
+}  // namespace
+
 void launchTransport(const DevParams& p, int blocks, int threads, hipStream_t s)
 {
     const size_t lds = transportLdsBytes(p);
-    if (p.lds_lut)
-        hipLaunchKernelGGL(transport_kernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(static_cast<unsigned>(threads)), lds, s, p);
-    else
-        hipLaunchKernelGGL(transport_kernel<false>, dim3(static_cast<unsigned>(blocks)), dim3(static_cast<unsigned>(threads)), lds, s, p);
+    void* args[] = { const_cast<DevParams*>(&p) };
+    (void)hipLaunchKernel(transportInstance(p.lds_lut != 0, p.level), dim3(static_cast<unsigned>(blocks)),
+                          dim3(static_cast<unsigned>(threads)), args, lds, s);
 }
 
-int transportBlocksPerCU(bool lds_lut, size_t lds_bytes)
+int transportBlocksPerCU(bool lds_lut, int level, size_t lds_bytes)
 {
     int n = 0;
-    const void* fn = lds_lut ? reinterpret_cast<const void*>(&transport_kernel<true>) : reinterpret_cast<const void*>(&transport_kernel<false>);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kBlock, lds_bytes) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transportInstance(lds_lut, level), kBlock, lds_bytes) != hipSuccess)
         return 1;
     return n > 0 ? n : 1;
 }

@@ -95,6 +95,8 @@ struct dxg_world {
     DevBuf<float> d_mu_tot;
     DevBuf<float4> d_mu_parts;
     DevBuf<float> d_majorant, d_s, d_rfwd, d_rinv;
+    DevBuf<float4> d_ia_shell;
+    DevBuf<float> d_ia_inv, d_fl_cum, d_fl_line;
     DevBuf<double> d_tally;
     DevBuf<unsigned long long> d_pool;
     DevBuf<dxg::DevCounters> d_counters;
@@ -147,6 +149,10 @@ void fillParams(const dxg_world* w, dxg::DevParams& p)
     p.s_table = w->d_s.ptr;
     p.ray_fwd = w->d_rfwd.ptr;
     p.ray_inv = w->d_rinv.ptr;
+    p.ia_shell = w->d_ia_shell.ptr;
+    p.ia_inv = w->d_ia_inv.ptr;
+    p.fl_cum = w->d_fl_cum.ptr;
+    p.fl_line = w->d_fl_line.ptr;
     p.tally = w->d_tally.ptr;
     p.pool = w->d_pool.ptr;
     p.counters = w->d_counters.ptr;
@@ -198,8 +204,6 @@ int dxg_world_create(dxg_world** out, int correction_level)
     if (!out)
         return fail(DXG_ERR_ARGUMENT, "dxg_world_create: null output pointer");
     *out = nullptr;
-    if (correction_level == 2)
-        return fail(DXG_ERR_UNSUPPORTED, "correction level 2 (impulse approximation, fluorescence) is not available in this build");
     if (correction_level < 0 || correction_level > 2)
         return fail(DXG_ERR_ARGUMENT, "correction level must be 0, 1 or 2");
     auto w = std::make_unique<dxg_world>();
@@ -281,7 +285,7 @@ int dxg_world_build(dxg_world* w)
     HIP_TRY(hipGetDevice(&w->device));
     if (!w->stream)
         HIP_TRY(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
-    dxg::buildTables(*pl, w->mats, w->rho_max, w->tables);
+    dxg::buildTables(*pl, w->mats, w->rho_max, w->level, w->tables);
     const uint64_t n = w->size();
     std::vector<uint32_t> packed(n);
     for (uint64_t i = 0; i < n; ++i)
@@ -297,6 +301,14 @@ int dxg_world_build(dxg_world* w)
     HIP_TRY(w->d_s.upload(w->tables.s_table, w->stream));
     HIP_TRY(w->d_rfwd.upload(w->tables.ray_fwd, w->stream));
     HIP_TRY(w->d_rinv.upload(w->tables.ray_inv, w->stream));
+    if (w->level == 2) {
+        std::vector<float4> shells(w->tables.ia_shell.size() / 4);
+        std::memcpy(shells.data(), w->tables.ia_shell.data(), w->tables.ia_shell.size() * sizeof(float));
+        HIP_TRY(w->d_ia_shell.upload(shells, w->stream));
+        HIP_TRY(w->d_ia_inv.upload(w->tables.ia_inv, w->stream));
+        HIP_TRY(w->d_fl_cum.upload(w->tables.fl_cum, w->stream));
+        HIP_TRY(w->d_fl_line.upload(w->tables.fl_line, w->stream));
+    }
     HIP_TRY(w->d_tally.alloc(4 * n));
     HIP_TRY(hipMemsetAsync(w->d_tally.ptr, 0, 4 * n * sizeof(double), w->stream));
     HIP_TRY(w->d_pool.alloc(1));
@@ -511,7 +523,7 @@ int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, w->device));
     const int block = dxg::transportBlockSize();
-    const int per_cu = dxg::transportBlocksPerCU(p.lds_lut != 0, lds_bytes);
+    const int per_cu = dxg::transportBlocksPerCU(p.lds_lut != 0, p.level, lds_bytes);
     int blocks = cus * per_cu;
     if (const char* env = std::getenv("DXG_BLOCKS"))
         blocks = std::max(1, std::atoi(env));

@@ -33,12 +33,19 @@ Models
 * Atomic masses: IUPAC standard atomic weights.
 
 Output format (little endian) -- read by the product's world builder and by the
-oracle:  magic "DXGPHYS1"; u32 zmax, n_energy, n_x; f64 energy[n_energy] (keV);
+oracle:  magic "DXGPHYS2"; u32 zmax, n_energy, n_x; f64 energy[n_energy] (keV);
 f64 x[n_x] (1/Angstrom); then for Z = 1..zmax: f64 A (g/mol); f64
 incoh[n_energy], coh[n_energy] (barn/atom); f64 F[n_x], S[n_x]; u32 n_pe;
 f64 pe_energy[n_pe], pe[n_pe] (barn/atom, energies ascending, an edge appears
-twice: value below then value above); u32 n_shell; per shell f64 n, l,
-occupancy, binding (keV), zeff.
+twice: value below then value above), pe_k[n_pe] (the K-shell part); u32
+n_shell; per shell f64 n, l, occupancy, binding (keV), zeff, n*; then f64
+omega_K, E_Ka1, E_Ka2, E_Kb (keV), p_Ka1, p_Ka2, p_Kb.
+
+Correction level 2 (impulse approximation) uses each shell's Compton profile
+in the hydrogenic 1s form J(p_z) = 8/(3 pi p0) (1 + (p_z/p0)^2)^-3 with the
+shell's momentum scale p0 = zeff/n* (atomic units): the exact profile for a
+1s orbital and, for other shells, the profile with the shell's virial mean
+kinetic energy.
 """
 from __future__ import annotations
 
@@ -324,7 +331,7 @@ def photo_table(z: int, egrid: np.ndarray):
     for e in merged:
         if any(abs(e - b) < 1e-12 for b in edges):
             below = photo_element(z, np.array([e * (1.0 - 1e-9)]))[0]
-            above = photo_element(z, np.array([e]))[0]
+            above = photo_element(z, np.array([e * (1.0 + 1e-9)]))[0]
             es += [e, e]
             vals += [below, above]
         else:
@@ -333,11 +340,48 @@ def photo_table(z: int, egrid: np.ndarray):
     return np.array(es), np.array(vals)
 
 
+def photo_k_table(z: int, pe_e: np.ndarray) -> np.ndarray:
+    """K-shell part of the photoelectric table at the nodes of photo_table:
+    zero below the K edge (the first of the doubled K-edge nodes)."""
+    n, l, occ, b, zs = shells(z)[0]
+    out = photo_subshell(pe_e * (1.0 + 1e-9), n, l, occ, b, zs, z)
+    dup = np.where((np.abs(pe_e - b) < 1e-12))[0]
+    if len(dup) == 2:
+        out[dup[0]] = 0.0
+    return out
+
+
+def fluorescence_yield_k(z: int) -> float:
+    """K-shell fluorescence yield from the semi-empirical fit
+    (omega/(1-omega))^(1/4) = 0.015 + 0.0327 Z - 0.64e-6 Z^3 (Burhop; Bambynek et al. 1972)."""
+    q = (0.015 + 0.0327 * z - 0.64e-6 * z ** 3) ** 4
+    return q / (1.0 + q)
+
+
+def k_lines(z: int):
+    """(omega_K, E_Ka1, E_Ka2, E_Kb, p_Ka1, p_Ka2, p_Kb): line energies as
+    differences of binding energies (K-L3, K-L2, K-M3), Ka2/Ka1 = 1/2 (2p1/2 to
+    2p3/2 occupancy) and Kb/Ka rising from 0.12 at Z = 20 to 0.23 at Z = 80."""
+    sh = shells(z)
+    bk = sh[0][3]
+    led = _interp_l_edges(z)
+    if led is None or z < 11:
+        return (0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    ka1 = bk - led[2]
+    ka2 = bk - led[1]
+    m3 = [b for (n, l, occ, b, zs) in sh if n == 3 and l == 1]
+    kb = bk - m3[0] if m3 else 0.0
+    rb = float(np.interp(z, [11, 20, 50, 80], [0.0, 0.12, 0.20, 0.23])) if m3 else 0.0
+    w1, w2, wb = 1.0, 0.5, 1.5 * rb
+    s = w1 + w2 + wb
+    return (fluorescence_yield_k(z), ka1, ka2, kb, w1 / s, w2 / s, wb / s)
+
+
 def write(path: Path, zmax: int = ZMAX) -> None:
     eg = energy_grid()
     xg = x_grid()
     with open(path, "wb") as f:
-        f.write(b"DXGPHYS1")
+        f.write(b"DXGPHYS2")
         f.write(struct.pack("<III", zmax, len(eg), len(xg)))
         f.write(eg.astype("<f8").tobytes())
         f.write(xg.astype("<f8").tobytes())
@@ -354,9 +398,11 @@ def write(path: Path, zmax: int = ZMAX) -> None:
             f.write(struct.pack("<I", len(pe_e)))
             f.write(pe_e.astype("<f8").tobytes())
             f.write(pe.astype("<f8").tobytes())
+            f.write(photo_k_table(z, pe_e).astype("<f8").tobytes())
             f.write(struct.pack("<I", len(sh)))
             for (n, l, occ, b, zs) in sh:
-                f.write(struct.pack("<ddddd", n, l, occ, b, zs))
+                f.write(struct.pack("<6d", n, l, occ, b, zs, N_STAR[n]))
+            f.write(struct.pack("<7d", *k_lines(z)))
 
 
 def default_path() -> Path:

@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
         h.oracle_material_attenuation.argtypes = [C.POINTER(Material), PD, u64, PD]
         h.oracle_material_sf.argtypes = [C.POINTER(Material), PD, u64, PD, PD, PD]
         h.oracle_sample_angles.argtypes = [C.POINTER(Material), C.c_double, C.c_int, u64, u64, PD, PD]
+        h.oracle_material_ia.argtypes = [C.POINTER(Material), PD, PD, PD]
         h.oracle_rng_stream.argtypes = [u64, u64, u64, C.POINTER(C.c_uint32)]
         h.oracle_rng_stream.restype = None
         h.oracle_traverse_f32.argtypes = [C.POINTER(u64), C.POINTER(C.c_float), C.POINTER(C.c_uint8), u64,
@@ -122,6 +123,17 @@ def sample_angles(comp: dict[int, float], energy: float, kind: int, n: int, seed
     if lib().oracle_sample_angles(C.byref(m), energy, kind, n, seed, c.ctypes.data_as(PD), eps.ctypes.data_as(PD)) != 0:
         raise RuntimeError(_err())
     return c, eps
+
+
+def impulse_shells(comp: dict[int, float]):
+    """(electron fraction, binding keV, p0 in m_e c) of the level-2 shells."""
+    k = _Keep()
+    m = material(comp, k)
+    f, b, p = np.zeros(8), np.zeros(8), np.zeros(8)
+    n = lib().oracle_material_ia(C.byref(m), f.ctypes.data_as(PD), b.ctypes.data_as(PD), p.ctypes.data_as(PD))
+    if n < 0:
+        raise RuntimeError(_err())
+    return f[:n], b[:n], p[:n]
 
 
 def rng_stream(seed: int, history: int, n: int) -> np.ndarray:

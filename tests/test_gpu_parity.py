@@ -84,7 +84,7 @@ def test_traverse_bit_exact(oracle):
     np.testing.assert_array_equal(gm, om)
 
 
-@pytest.mark.parametrize("level", [0, 1])
+@pytest.mark.parametrize("level", [0, 1, 2])
 def test_pencil_water_cube(oracle, level):
     ph = phantoms.water_cube(64, 0.1)
     beam = dx.PencilBeam((0.05, 0.05, -10.0), (0, 0, 1), 60.0)
@@ -119,6 +119,32 @@ def test_ct_spiral_spectrum_bowtie_aec(oracle):
     n_ex = beam.numberOfExposures()
     beam.setNumberOfParticlesPerExposure(3_000_000 // n_ex)
     g, o, hg, ho, gc, oc = run_both(ph, beam, 1, oracle=oracle)
+    v = compare(g, o, hg, ho)
+    print(v, flush=True)
+    assert v.ok, str(v)
+    assert abs(gc["deposited_energy"] + gc["escaped_energy"] - gc["emitted_energy"]) <= 1e-5 * gc["emitted_energy"]
+
+
+def test_ct_spiral_level2_iodine_insert(oracle):
+    """Correction level 2: impulse-approximation Compton and K fluorescence,
+    with an iodinated insert (5 % I by mass) and bone so K lines of I and Ca fly."""
+    ph = small_body()
+    mats = list(ph.materials) + [{53: 0.05, **{z: w * 0.95 for z, w in phantoms.SOFT_TISSUE.items()}}]
+    mat = ph.material.reshape(ph.dims[::-1]).copy()
+    dens = ph.density.reshape(ph.dims[::-1]).copy()
+    zz, yy, xx = np.meshgrid(*[(np.arange(d) + 0.5 - d / 2) * s for d, s in zip(ph.dims[::-1], ph.spacing[::-1])],
+                             indexing="ij")
+    insert = ((xx - 4.0) ** 2 + yy ** 2 < 2.5 ** 2) & (mat == 1)
+    mat[insert] = 5
+    dens[insert] = 1.05
+    ph = phantoms.Phantom(ph.dims, ph.spacing, dens.ravel(), mat.ravel(), mats, ph.names + ["iodine insert"])
+    beam = dx.CTSpiralBeam((0, 0, -12), (0, 0, 12))
+    beam.setCollimation(4.0)
+    beam.setStepAngleDeg(15)
+    beam.setTubeVoltage(120)
+    beam.addTubeFiltrationMaterial(13, 6.0)
+    beam.setNumberOfParticlesPerExposure(3_000_000 // beam.numberOfExposures())
+    g, o, hg, ho, gc, oc = run_both(ph, beam, 2, oracle=oracle)
     v = compare(g, o, hg, ho)
     print(v, flush=True)
     assert v.ok, str(v)

@@ -193,3 +193,83 @@ def test_traverse_f32_indices_contain_points(oracle):
         exp_idx = np.where(inside, flat, -1)
         np.testing.assert_array_equal(idx[r][clear], exp_idx[clear])
         np.testing.assert_array_equal(mm[r][clear & inside], mat[exp_idx[clear & inside]])
+
+
+# ----------------------------------------------------------------- level 2
+def unit_profile(x):
+    return 8.0 / (3.0 * np.pi) * (1.0 + x * x) ** -3
+
+
+def test_impulse_shells_selection(oracle):
+    f, b, p0 = oracle.impulse_shells(phantoms.WATER)
+    assert len(f) == 4 and f.sum() == pytest.approx(1.0, abs=1e-12)   # O 1s, O 2s, O 2p, H 1s
+    assert b[0] == pytest.approx(g.K_EDGE_KEV[7])                    # oxygen K binding first
+    assert f[0] == pytest.approx(2 * 0.888106 / 15.999 / (0.111894 / 1.008 + 8 * 0.888106 / 15.999), rel=1e-9)
+    assert p0[0] == pytest.approx(7.7 * 7.2973525693e-3, rel=1e-12)   # Slater 1s charge 8 - 0.3
+    f, b, p0 = oracle.impulse_shells(phantoms.BONE)
+    assert len(f) == 5 and f.sum() == pytest.approx(1.0, abs=1e-12)
+    assert b[0] == pytest.approx(g.K_EDGE_KEV[19]) and np.all(np.diff(b[:4]) <= 0)
+
+
+def recovered_pz(E, c, eps):
+    a = E / ME_C2
+    b = a * eps
+    return (a * b * (1 - c) - (a - b)) / np.sqrt(a * a + b * b - 2 * a * b * c)
+
+
+def test_impulse_approximation_profile(oracle):
+    """Hydrogen (one shell, 13.6 eV binding, so no kinematic truncation): the
+    p_z recovered from (E, E', theta) follows J(p_z) = g(p_z/p0)/p0 with
+    p0 = alpha m_e c -- the kinematics and the choice of root."""
+    E = 60.0
+    c, eps = oracle.sample_angles({1: 1.0}, E, 4, 400_000, seed=13)
+    x = recovered_pz(E, c, eps) / 7.2973525693e-3
+    # at small angles E' <= E - B truncates the profile (physics, not sampling):
+    # compare the large-angle samples, where no truncation occurs
+    big = (c < 0.5) & (np.abs(x) < 3.0)
+    assert chi2_pvalue(x[big], unit_profile, lo=-3, hi=3, bins=36) > 1e-4
+    a = E / ME_C2
+    # Doppler broadening leaves the mean near Klein-Nishina: <E'> within 0.1 %
+    assert np.mean(a * eps) == pytest.approx(np.mean(a / (1 + a * (1 - c))), rel=1e-3)
+
+
+def test_impulse_approximation_energy_bound(oracle):
+    # water at 20 keV: every scatter leaves at least its shell's binding energy
+    E = 20.0
+    c, eps = oracle.sample_angles(phantoms.WATER, E, 4, 100_000, seed=16)
+    f, b, p0 = oracle.impulse_shells(phantoms.WATER)
+    assert np.all(eps > 0) and np.all(E - eps * E >= b.min() - 1e-9)
+    # the recovered |p_z| never exceeds what the inverse-CDF range allows
+    assert np.all(np.abs(recovered_pz(E, c, eps)) <= 8.0 * p0.max() * (1 + 1e-9))
+
+
+def test_fluorescence_yield_and_lines(oracle):
+    """Pure iodine at 50 keV: P(line) = sigma_K / sigma_pe x omega_K, lines Ka1, Ka2, Kb."""
+    E = 50.0
+    _, eps = oracle.sample_angles({53: 1.0}, E, 5, 400_000, seed=14)
+    lines = eps * E
+    pe_e, pe = g.photo_table(53, g.energy_grid())
+    pk = g.photo_k_table(53, pe_e)
+    frac_k = np.exp(np.interp(np.log(E), np.log(pe_e), np.log(pk))) / np.exp(np.interp(np.log(E), np.log(pe_e), np.log(pe)))
+    omega, e1, e2, e3, p1, p2, p3 = g.k_lines(53)
+    expected = frac_k * omega
+    got = np.mean(lines > 0)
+    assert got == pytest.approx(expected, abs=4 * math.sqrt(expected * (1 - expected) / len(lines)))
+    emitted = lines[lines > 0]
+    assert set(np.round(np.unique(emitted), 6)) <= {round(e1, 6), round(e2, 6), round(e3, 6)}
+    share = np.array([np.mean(np.isclose(emitted, e)) for e in (e1, e2, e3)])
+    np.testing.assert_allclose(share, [p1, p2, p3], atol=0.01)
+    # below the K edge nothing fluoresces above 1 keV except L lines (not tracked)
+    _, eps = oracle.sample_angles({53: 1.0}, 30.0, 5, 20_000, seed=15)
+    assert np.all(eps == 0)
+
+
+def test_level2_energy_bookkeeping(oracle):
+    ph = phantoms.ct_body((32, 32, 20), (1.2, 1.2, 1.2), seed=2)
+    mats = list(ph.materials)
+    mats[4] = {53: 0.05, **{z: w * 0.95 for z, w in phantoms.SOFT_TISSUE.items()}}   # iodinated insert
+    ex, spec = pencil(50_000, energy=80.0, pos=(0.3, 0.2, -30.0))
+    se, _, nev, c = oracle.transport(ph.dims, ph.spacing, ph.density, ph.material, mats, ex, spec,
+                                     level=2, seed=4, threads=4)
+    assert abs(c["deposited_energy"] + c["escaped_energy"] - c["emitted_energy"]) < 1e-9 * c["emitted_energy"]
+    assert se.sum() == pytest.approx(c["deposited_energy"], rel=1e-12)

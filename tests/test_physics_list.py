@@ -129,10 +129,30 @@ def test_k_edge_jump():
     assert 4.0 < hi / lo < 12.0
 
 
+def test_table_edges_are_sharp():
+    # the doubled edge node carries the value below, then the value above
+    e, v = g.photo_table(53, g.energy_grid())
+    k = g.photo_k_table(53, e)
+    i = np.where(np.abs(e - g.K_EDGE_KEV[52]) < 1e-12)[0]
+    assert len(i) == 2 and v[i[1]] / v[i[0]] > 4.0
+    assert k[i[0]] == 0.0 and k[i[1]] > 0.7 * v[i[1]]
+    assert np.all(k[e < g.K_EDGE_KEV[52]] == 0.0)
+
+
+def test_k_fluorescence_data():
+    # yields of the semi-empirical fit at textbook points (Ca ~0.16, I ~0.88)
+    assert g.fluorescence_yield_k(20) == pytest.approx(0.163, abs=0.01)
+    assert g.fluorescence_yield_k(53) == pytest.approx(0.88, abs=0.01)
+    om, ka1, ka2, kb, p1, p2, p3 = g.k_lines(74)
+    assert ka1 == pytest.approx(59.318, abs=1e-3) and ka2 == pytest.approx(57.981, abs=1e-3)
+    assert p1 + p2 + p3 == pytest.approx(1.0) and p1 == pytest.approx(2 * p2)
+    assert g.k_lines(8)[0] == 0.0   # oxygen's K line is below the 1 keV cutoff
+
+
 def test_written_file_roundtrip(tmp_path):
     p = tmp_path / "pl.bin"
     g.write(p, zmax=8)
     data = p.read_bytes()
-    assert data[:8] == b"DXGPHYS1"
+    assert data[:8] == b"DXGPHYS2"
     zmax, ne, nx = np.frombuffer(data[8:20], dtype="<u4")
     assert (zmax, ne, nx) == (8, g.N_ENERGY, g.N_X)
