@@ -443,16 +443,17 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
                         const float fy = rngU(rng) * static_cast<float>(kIaInvNd - 1);
                         const int kk = min(static_cast<int>(fy), kIaInvNd - 2);
                         const float pz = lerpTab(P.ia_inv, kk, fy - static_cast<float>(kk)) * s.z;
-                        // E'/mc^2 from p_z |q| = E E'(1-cos)/mc^2 - (E - E') (in m_e c units)
+                        // E'/mc^2 from p_z |q| = E E'(1-cos)/mc^2 - (E - E') (in m_e c units):
+                        // b = k (h + p_z sqrt(Q)) / (K^2 - p_z^2), with the discriminant
+                        // written as p_z^2 Q so that FP32 does not cancel it away
                         const float K = 1.0f + k * (1.0f - cost);
                         const float p2 = pz * pz;
                         const float h = K - p2 * cost;
                         const float den = K * K - p2;
-                        const float disc = h * h - den * (1.0f - p2);
-                        if (!(den > 0.0f) || !(disc >= 0.0f))
+                        const float Q = (K - cost) * (K - cost) + (1.0f - cost * cost) * (1.0f - p2);
+                        if (!(den > 0.0f))
                             continue;
-                        const float sq = sqrtf(disc);
-                        enew = k * (h + (pz >= 0.0f ? sq : -sq)) / den * (1.0f / kInvMeC2);
+                        enew = k * (h + pz * sqrtf(Q)) / den * (1.0f / kInvMeC2);
                         if (enew > 0.0f && E - enew >= s.y)
                             break;
                     }

@@ -569,10 +569,11 @@ double sample_compton_ia(const Material& m, double E, Rng& r, double& Enew)
         const double p2 = pz * pz;
         const double h = K - p2 * c;
         const double den = K * K - p2;
-        const double disc = h * h - den * (1.0 - p2);
-        if (!(den > 0) || !(disc >= 0))
+        // discriminant h^2 - den (1 - p^2) = p^2 [(K - c)^2 + (1 - c^2)(1 - p^2)]
+        const double Q = (K - c) * (K - c) + (1.0 - c * c) * (1.0 - p2);
+        if (!(den > 0))
             continue;
-        const double b = a * (h + (pz >= 0 ? std::sqrt(disc) : -std::sqrt(disc))) / den;
+        const double b = a * (h + pz * std::sqrt(Q)) / den;
         Enew = b * ME_C2_KEV;
         if (Enew > 0 && E - Enew >= m.ia_b[s])
             return c;
