@@ -34,6 +34,19 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+PROFILE = ROOT / "profiles" / "transport_kernel_ct_spiral_1e8.md"
+
+
+def measured_traffic_per_history():
+    """Fabric traffic per history from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (copy-kernel calibrated); None when not profiled."""
+    try:
+        for line in PROFILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_history:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def ct_workload(histories: int):
@@ -134,6 +147,9 @@ def main() -> int:
     wall = time.perf_counter() - t0
     bytes_alg = 5.0 * steps + 24.0 * deposits
     achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    tph = measured_traffic_per_history()
+    hist_per_launch = hist / max(launches, 1)
+    traffic = tph * hist_per_launch if tph is not None else None
     result = {
         "metric": "histories/s",
         "value": hist / wall,
@@ -164,7 +180,9 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/transport_kernel_ct_spiral_1e8.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
+                              "per history, times histories per launch)",
             "kernel": "transport_kernel",
             "kernel_ms_per_launch": kernel_ms / max(launches, 1),
             "bytes_per_history": bytes_alg / max(hist, 1),
