@@ -32,7 +32,7 @@ enum {
     DXG_ERR_DATA = -4,         /* material index >= number of materials */
     DXG_ERR_PHYSICS = -5,      /* physics list missing or corrupt */
     DXG_ERR_DEVICE = -6,       /* HIP runtime failure / no device */
-    DXG_ERR_UNSUPPORTED = -7,  /* feature outside this build (e.g. level 2) */
+    DXG_ERR_UNSUPPORTED = -7,  /* feature outside this build (e.g. correction level 3) */
     DXG_ERR_CANCELLED = -8     /* progress stop flag was set during the run */
 };
 
@@ -110,7 +110,8 @@ const char* dxg_last_error(void);
 /* ---- World<AAVoxelGrid<5, CORRECTION, 255>> (simulationpipeline.cpp:127-131).
  * correction_level is the CORRECTION template argument: 0 = free electrons,
  * 1 = form factor / incoherent scattering function (the reference default,
- * simulationpipeline.hpp:62); 2 is reported as DXG_ERR_UNSUPPORTED. */
+ * simulationpipeline.hpp:62), 2 = impulse approximation with shell Doppler
+ * broadening and K-shell fluorescence; other values give DXG_ERR_ARGUMENT. */
 int dxg_world_create(dxg_world** out, int correction_level);
 void dxg_world_destroy(dxg_world* w);
 

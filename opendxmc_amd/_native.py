@@ -128,7 +128,10 @@ class NativeError(RuntimeError):
 
 _lib = None
 
-# (name, restype, argtypes) for every entry point of include/dxmc_hip.h
+PF = C.POINTER(C.c_float)
+
+# (name, restype, argtypes) for every entry point of include/dxmc_hip.h and
+# include/dxmc_denoise.h
 SIGNATURES = [
     ("dxg_last_error", C.c_char_p, []),
     ("dxg_world_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
@@ -161,6 +164,19 @@ SIGNATURES = [
     ("dxg_debug_device_copy", C.c_int, [u64, C.c_int, PD]),
     ("dxg_debug_traverse", C.c_int, [C.c_void_p, u64, C.POINTER(C.c_float), C.POINTER(C.c_float), u64,
                                      C.POINTER(C.c_float), C.POINTER(i64), C.POINTER(C.c_int32)]),
+    ("dxg_denoiser_num_params", u64, []),
+    ("dxg_denoiser_create", C.c_int, [C.POINTER(C.c_void_p), u64]),
+    ("dxg_denoiser_destroy", None, [C.c_void_p]),
+    ("dxg_denoiser_get_params", C.c_int, [C.c_void_p, PF, u64]),
+    ("dxg_denoiser_set_params", C.c_int, [C.c_void_p, PF, u64]),
+    ("dxg_denoiser_get_grads", C.c_int, [C.c_void_p, PF, u64]),
+    ("dxg_denoiser_train_step", C.c_int, [C.c_void_p, PF, PF, PF, u64, C.POINTER(u64), C.c_float, PF]),
+    ("dxg_denoiser_infer", C.c_int, [C.c_void_p, PF, C.POINTER(u64), PF]),
+    ("dxg_denoiser_last_ms", C.c_double, [C.c_void_p]),
+    ("dxg_denoiser_world_features", C.c_int, [C.c_void_p, C.c_void_p, PF, u64]),
+    ("dxg_denoiser_apply", C.c_int, [C.c_void_p, C.c_void_p, PD]),
+    ("dxg_conv3d_probe", C.c_int, [C.c_int, PF, PF, PF, u64, C.POINTER(u64), PF]),
+    ("dxg_mfma_probe", C.c_int, [PF, PF, PF]),
 ]
 
 

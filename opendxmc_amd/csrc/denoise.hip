@@ -1,0 +1,1209 @@
+// Dose denoiser: BF16 implicit-GEMM 3-D convolutions on CDNA4 matrix cores,
+// FP32 accumulation, hand-written backward pass and Adam (C-ABI in
+// include/dxmc_denoise.h).
+//
+// Tensors are channel-last BF16 with 32 channels: voxel v = ((n*D + z)*H + y)*W + x
+// occupies 64 bytes.  Every layer is a 3x3x3 convolution 32 -> 32 (the input
+// features and the one-channel output are zero padded to 32), so one kernel
+// serves all of them.
+//
+// Matrix-core operand layout (v_mfma_f32_16x16x32_bf16, D = A*B, A 16x32,
+// B 32x16): lane l holds A[l%16][8*(l/16)+e] and B[8*(l/16)+e][l%16] in element
+// e of its 8-element operands and receives D[4*(l/16)+i][l%16] in accumulator i.
+// dxg_mfma_probe checks exactly this on the device.
+//
+// Convolution (forward, and input gradient with flipped transposed weights):
+// A = weights (16 output channels x 32 input channels of one tap), B = 32
+// input channels x 16 voxels along x.  A wavefront owns 64 consecutive voxels
+// of one row and both output-channel halves: per tap 2 weight fragments from
+// LDS, 4 activation fragments (16 B per lane, coalesced 1 KB per instruction)
+// from memory, 8 MFMAs; 27 taps accumulate in 32 FP32 registers.
+//
+// Weight gradient: dW[co][tap][ci] = sum_v g[v][co] x[v+off(tap)][ci] reduces
+// over voxels, so K runs over 32 voxels along x.  A work-group transposes one
+// 4-row x 32-voxel tile of g and its 18-row halo of x into channel-major LDS;
+// each wavefront keeps the partial sums of 7 taps in registers and reads the
+// x-shifted operand with two aligned 16-B reads and a byte funnel shift.
+#include "dxmc_denoise.h"
+
+#include "device.hpp"
+#include "physics.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+constexpr int kC = DXG_DENOISE_CHANNELS;            // 32
+constexpr int kTaps = 27;
+constexpr int kLayerW = kC * kTaps * kC;             // 27648 weights per layer
+constexpr int kLayerP = kLayerW + kC;                // + 32 biases
+constexpr int kLayers = DXG_DENOISE_LAYERS;          // 18
+constexpr int kBlocks = (kLayers - 2) / 2;           // 8 residual blocks
+constexpr int kXtLen = 56;                           // halo row length in LDS (112 B: conflict-free 16-B reads)
+constexpr int kGtLen = 40;                           // gradient row length in LDS (80 B)
+
+int fail(int code, const std::string& msg) { return dxg::setError(code, msg); }
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        const hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess)                                                             \
+            return fail(DXG_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------- BF16 helpers
+
+__host__ __device__ inline uint16_t toBf16(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u)
+        return static_cast<uint16_t>((u >> 16) | 0x40u);   // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                       // round to nearest even
+    return static_cast<uint16_t>(u >> 16);
+}
+
+__host__ __device__ inline float fromBf16(uint32_t h)
+{
+    const uint32_t u = h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ inline uint32_t pack2(float lo, float hi)
+{
+    return static_cast<uint32_t>(toBf16(lo)) | (static_cast<uint32_t>(toBf16(hi)) << 16);
+}
+
+__device__ inline bf16x8 asOperand(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+__device__ inline u32x4 load16(const uint16_t* p) { return *reinterpret_cast<const u32x4*>(p); }
+
+// ---------------------------------------------------------------- kernels
+
+__global__ __launch_bounds__(64) void mfma_probe_kernel(const float* a, const float* b, float* d)
+{
+    const int l = threadIdx.x, lr = l & 15, lq = l >> 4;
+    u32x4 av, bv;
+    for (int e = 0; e < 4; ++e) {
+        const int k = 8 * lq + 2 * e;
+        av[e] = pack2(a[lr * 32 + k], a[lr * 32 + k + 1]);
+        bv[e] = pack2(b[k * 16 + lr], b[(k + 1) * 16 + lr]);
+    }
+    const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(asOperand(av), asOperand(bv), f32x4 { 0, 0, 0, 0 }, 0, 0, 0);
+    for (int i = 0; i < 4; ++i)
+        d[(4 * lq + i) * 16 + lr] = acc[i];
+}
+
+struct ConvArgs {
+    const uint16_t* x;       // input tensor
+    const uint16_t* w;       // [27][32 out][32 in] BF16
+    const float* bias;       // 32 or null
+    const uint16_t* res;     // residual added before the activation, or null (may alias y)
+    const uint16_t* gate;    // output multiplied by (gate > 0), or null
+    uint16_t* y;             // BF16 output or null
+    float* y32;              // FP32 output or null
+    int y32_channels;        // 1 or 32
+    int relu;
+    int nb, nz, ny, nx;
+    int wave_tiles;          // nb*nz*ny*(nx/64)
+    int chunk;               // work-group tiles (4 wave tiles) per work-group
+};
+
+// Weight fragment (tap, out channel co, 16-B chunk q) lives at chunk
+// position q ^ ((co >> 2) & 3) of its 64-B row: 16 lanes reading 16
+// consecutive rows then touch 16 distinct bank groups.
+__device__ inline int wSlot(int tap, int co, int q) { return (tap * 32 + co) * 4 + (q ^ ((co >> 2) & 3)); }
+
+__global__ __launch_bounds__(256) void conv3d_kernel(ConvArgs a)
+{
+    __shared__ u32x4 wl[kTaps * 32 * 4];   // 55,296 B
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kTaps * 32 * 4; i += 256) {
+        const int row = i >> 2;
+        wl[wSlot(row >> 5, row & 31, i & 3)] = reinterpret_cast<const u32x4*>(a.w)[i];
+    }
+    __syncthreads();
+
+    const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lq = lane >> 4;
+    float bias[2][4];
+    for (int m = 0; m < 2; ++m)
+        for (int i = 0; i < 4; ++i)
+            bias[m][i] = a.bias ? a.bias[16 * m + 4 * lq + i] : 0.0f;
+
+    // XCD-aware order: consecutive work-group ids alternate between the 8
+    // XCDs, so map them to contiguous slabs of rows per XCD.
+    const int g = gridDim.x;
+    const int logical = (blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
+    const int wg_tiles = (a.wave_tiles + 3) >> 2;
+    const int begin = logical * a.chunk;
+    const int end = min(begin + a.chunk, wg_tiles);
+    const int nxs = a.nx >> 6;
+
+    for (int wt = begin; wt < end; ++wt) {
+        const int t = wt * 4 + wave;
+        if (t >= a.wave_tiles)
+            break;
+        int r = t / nxs;
+        const int x0 = (t - r * nxs) * 64;
+        const int y = r % a.ny;
+        r /= a.ny;
+        const int z = r % a.nz;
+        const int n = r / a.nz;
+
+        f32x4 acc[2][4];
+        for (int m = 0; m < 2; ++m)
+            for (int j = 0; j < 4; ++j)
+                acc[m][j] = f32x4 { 0, 0, 0, 0 };
+
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int zz = z + dz;
+            if (static_cast<unsigned>(zz) >= static_cast<unsigned>(a.nz))
+                continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int yy = y + dy;
+                if (static_cast<unsigned>(yy) >= static_cast<unsigned>(a.ny))
+                    continue;
+                const size_t rowb = (static_cast<size_t>(n * a.nz + zz) * a.ny + yy) * a.nx;
+                const int tapb = (dz + 1) * 9 + (dy + 1) * 3;
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    bf16x8 bop[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int xx = x0 + 16 * j + lr + dx;
+                        u32x4 v = { 0, 0, 0, 0 };
+                        if (static_cast<unsigned>(xx) < static_cast<unsigned>(a.nx))
+                            v = load16(a.x + (rowb + xx) * kC + 8 * lq);
+                        bop[j] = asOperand(v);
+                    }
+                    const int tap = tapb + dx + 1;
+                    bf16x8 aop[2];
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+                        aop[m] = asOperand(wl[wSlot(tap, 16 * m + lr, lq)]);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop[m], bop[j], acc[m][j], 0, 0, 0);
+                }
+            }
+        }
+
+        const size_t orow = (static_cast<size_t>(n * a.nz + z) * a.ny + y) * a.nx;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t v = orow + x0 + 16 * j + lr;
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const int co0 = 16 * m + 4 * lq;
+                float o[4];
+                for (int i = 0; i < 4; ++i)
+                    o[i] = acc[m][j][i] + bias[m][i];
+                if (a.res) {
+                    const u32x2 rv = *reinterpret_cast<const u32x2*>(a.res + v * kC + co0);
+                    o[0] += fromBf16(rv[0] & 0xffffu);
+                    o[1] += fromBf16(rv[0] >> 16);
+                    o[2] += fromBf16(rv[1] & 0xffffu);
+                    o[3] += fromBf16(rv[1] >> 16);
+                }
+                if (a.relu)
+                    for (int i = 0; i < 4; ++i)
+                        o[i] = fmaxf(o[i], 0.0f);
+                if (a.gate) {
+                    const u32x2 gv = *reinterpret_cast<const u32x2*>(a.gate + v * kC + co0);
+                    const uint32_t h[4] = { gv[0] & 0xffffu, gv[0] >> 16, gv[1] & 0xffffu, gv[1] >> 16 };
+                    for (int i = 0; i < 4; ++i)
+                        if (h[i] == 0 || (h[i] & 0x8000u))
+                            o[i] = 0.0f;
+                }
+                if (a.y)
+                    *reinterpret_cast<u32x2*>(a.y + v * kC + co0) = u32x2 { pack2(o[0], o[1]), pack2(o[2], o[3]) };
+                if (a.y32) {
+                    if (a.y32_channels == kC)
+                        *reinterpret_cast<f32x4*>(a.y32 + v * kC + co0) = f32x4 { o[0], o[1], o[2], o[3] };
+                    else if (co0 == 0)
+                        a.y32[v] = o[0];
+                }
+            }
+        }
+    }
+}
+
+struct WgradArgs {
+    const uint16_t* x;       // layer input
+    const uint16_t* g;       // gradient of the layer's output (before its activation)
+    float* part;             // [grid][27][32 co][32 ci] partial sums
+    float* part_b;           // [grid][32]
+    int nb, nz, ny, nx;
+    int units;               // nb*nz*(ny/4)*(nx/32)
+    int chunk;
+};
+
+__global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a)
+{
+    __shared__ __align__(16) uint16_t xT[18 * 32 * kXtLen];   // halo rows (dz, y) x channel x voxel: 64,512 B
+    __shared__ __align__(16) uint16_t gT[4 * 32 * kGtLen];    // 4 rows x channel x voxel: 10,240 B
+    __shared__ float bsh[32];
+    const int tid = threadIdx.x;
+    if (tid < 32)
+        bsh[tid] = 0.0f;
+    const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lq = lane >> 4;
+    const int tap0 = wave * 7;
+    const int ntap = min(7, kTaps - tap0);
+
+    f32x4 acc[7][2][2];
+    for (int k = 0; k < 7; ++k)
+        for (int t = 0; t < 2; ++t)
+            for (int s = 0; s < 2; ++s)
+                acc[k][t][s] = f32x4 { 0, 0, 0, 0 };
+    float bsum = 0.0f;
+
+    const int g = gridDim.x;
+    const int logical = (blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
+    const int begin = logical * a.chunk;
+    const int end = min(begin + a.chunk, a.units);
+    const int nxc = a.nx >> 5, nyq = a.ny >> 2;
+
+    for (int u = begin; u < end; ++u) {
+        int r = u / nxc;
+        const int x0 = (u - r * nxc) * 32;
+        const int y0 = (r % nyq) * 4;
+        r /= nyq;
+        const int z = r % a.nz;
+        const int n = r / a.nz;
+        __syncthreads();   // the previous tile is consumed
+        for (int i = tid; i < 4 * 32 * 4; i += 256) {
+            const int q = i & 3, vox = (i >> 2) & 31, rr = i >> 7;
+            const size_t v = (static_cast<size_t>(n * a.nz + z) * a.ny + y0 + rr) * a.nx + x0 + vox;
+            const u32x4 d = load16(a.g + v * kC + 8 * q);
+            uint16_t* dst = gT + (rr * 32 + 8 * q) * kGtLen + vox;
+            for (int e = 0; e < 4; ++e) {
+                dst[(2 * e) * kGtLen] = static_cast<uint16_t>(d[e] & 0xffffu);
+                dst[(2 * e + 1) * kGtLen] = static_cast<uint16_t>(d[e] >> 16);
+            }
+        }
+        for (int i = tid; i < 18 * 34 * 4; i += 256) {
+            const int q = i & 3, t = i >> 2;
+            const int row = t / 34, vox = t - row * 34;
+            const int zz = z + row / 6 - 1, yy = y0 + row % 6 - 1, xx = x0 + vox - 1;
+            u32x4 d = { 0, 0, 0, 0 };
+            if (static_cast<unsigned>(zz) < static_cast<unsigned>(a.nz) && static_cast<unsigned>(yy) < static_cast<unsigned>(a.ny)
+                && static_cast<unsigned>(xx) < static_cast<unsigned>(a.nx))
+                d = load16(a.x + ((static_cast<size_t>(n * a.nz + zz) * a.ny + yy) * a.nx + xx) * kC + 8 * q);
+            uint16_t* dst = xT + (row * 32 + 8 * q) * kXtLen + vox;
+            for (int e = 0; e < 4; ++e) {
+                dst[(2 * e) * kXtLen] = static_cast<uint16_t>(d[e] & 0xffffu);
+                dst[(2 * e + 1) * kXtLen] = static_cast<uint16_t>(d[e] >> 16);
+            }
+        }
+        __syncthreads();
+        if (tid < 128) {
+            const uint16_t* s = gT + ((tid >> 5) * 32 + (tid & 31)) * kGtLen;
+            for (int v = 0; v < 32; ++v)
+                bsum += fromBf16(s[v]);
+        }
+        for (int rr = 0; rr < 4; ++rr) {
+            bf16x8 ga[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+                ga[t] = asOperand(load16(gT + (rr * 32 + 16 * t + lr) * kGtLen + 8 * lq));
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                if (k >= ntap)
+                    break;
+                const int tap = tap0 + k;
+                const int dzi = tap / 9, dyi = (tap / 3) % 3, dxi = tap % 3;
+                const int row = dzi * 6 + rr + dyi;
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const uint16_t* p = xT + (row * 32 + 16 * s + lr) * kXtLen + 8 * lq;
+                    const u32x4 lo = load16(p), hi = load16(p + 8);
+                    u32x4 b;
+                    if (dxi == 0) {
+                        b = lo;
+                    } else if (dxi == 1) {
+                        b = u32x4 { __builtin_amdgcn_alignbyte(lo[1], lo[0], 2), __builtin_amdgcn_alignbyte(lo[2], lo[1], 2),
+                                    __builtin_amdgcn_alignbyte(lo[3], lo[2], 2), __builtin_amdgcn_alignbyte(hi[0], lo[3], 2) };
+                    } else {
+                        b = u32x4 { lo[1], lo[2], lo[3], hi[0] };
+                    }
+                    const bf16x8 bop = asOperand(b);
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        acc[k][t][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ga[t], bop, acc[k][t][s], 0, 0, 0);
+                }
+            }
+        }
+    }
+
+    float* part = a.part + static_cast<size_t>(blockIdx.x) * kLayerW;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        if (k >= ntap)
+            break;
+        const int tap = tap0 + k;
+        for (int t = 0; t < 2; ++t)
+            for (int s = 0; s < 2; ++s)
+                for (int i = 0; i < 4; ++i)
+                    part[(tap * 32 + 16 * t + 4 * lq + i) * 32 + 16 * s + lr] = acc[k][t][s][i];
+    }
+    __syncthreads();
+    if (tid < 128)
+        atomicAdd(&bsh[tid & 31], bsum);
+    __syncthreads();
+    if (tid < 32)
+        a.part_b[static_cast<size_t>(blockIdx.x) * 32 + tid] = bsh[tid];
+}
+
+// Sum the per-work-group partials into one layer's gradient in parameter
+// layout ([co][tap][ci] weights, then 32 biases).
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float* part, const float* part_b, int parts, float* grad)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < kLayerW) {
+        float s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        int p = 0;
+        for (; p + 4 <= parts; p += 4) {
+            s0 += part[static_cast<size_t>(p) * kLayerW + i];
+            s1 += part[static_cast<size_t>(p + 1) * kLayerW + i];
+            s2 += part[static_cast<size_t>(p + 2) * kLayerW + i];
+            s3 += part[static_cast<size_t>(p + 3) * kLayerW + i];
+        }
+        for (; p < parts; ++p)
+            s0 += part[static_cast<size_t>(p) * kLayerW + i];
+        const int tap = i >> 10, co = (i >> 5) & 31, ci = i & 31;
+        grad[(co * kTaps + tap) * kC + ci] = (s0 + s1) + (s2 + s3);
+    } else if (i < kLayerP) {
+        const int co = i - kLayerW;
+        float s = 0;
+        for (int p = 0; p < parts; ++p)
+            s += part_b[p * 32 + co];
+        grad[kLayerW + co] = s;
+    }
+}
+
+// FP32 master parameters -> BF16 forward weights [tap][co][ci] and input-
+// gradient weights wd[tap][co][ci] = W[ci][26 - tap][co] (flipped, transposed).
+__global__ __launch_bounds__(256) void prep_weights_kernel(const float* params, uint16_t* wf, uint16_t* wd, int layers)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= layers * kLayerW)
+        return;
+    const int l = i / kLayerW, r = i - l * kLayerW;
+    const int tap = r >> 10, co = (r >> 5) & 31, ci = r & 31;
+    const float* p = params + static_cast<size_t>(l) * kLayerP;
+    wf[i] = toBf16(p[(co * kTaps + tap) * kC + ci]);
+    wd[i] = toBf16(p[(ci * kTaps + (kTaps - 1 - tap)) * kC + co]);
+}
+
+__global__ __launch_bounds__(256) void adam_kernel(float* p, float* m, float* v, const float* g, int n, float lr, float b1,
+                                                   float b2, float eps, float bc1, float bc2)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * bc1) / (sqrtf(vi * bc2) + eps);
+}
+
+// Features (4 FP32 per voxel) -> 32-channel BF16 input tensor and an FP32 copy
+// of feature 0 (the relative dose the network corrects).
+__global__ __launch_bounds__(256) void pack_features_kernel(const float* f, uint16_t* xin, float* feat0, size_t nv)
+{
+    const size_t v = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (v >= nv)
+        return;
+    const f32x4 q = *reinterpret_cast<const f32x4*>(f + 4 * v);
+    u32x4* dst = reinterpret_cast<u32x4*>(xin + v * kC);
+    dst[0] = u32x4 { pack2(q[0], q[1]), pack2(q[2], q[3]), 0u, 0u };
+    dst[1] = u32x4 { 0u, 0u, 0u, 0u };
+    dst[2] = u32x4 { 0u, 0u, 0u, 0u };
+    dst[3] = u32x4 { 0u, 0u, 0u, 0u };
+    feat0[v] = q[0];
+}
+
+// Masked squared error of pred = feat0 + correction against the target, and
+// its gradient into channel 0 of a 32-channel BF16 tensor.
+__global__ __launch_bounds__(256) void loss_kernel(const float* y32, const float* feat0, const float* target,
+                                                   const float* mask, float inv_count, uint16_t* gout, float* lpart,
+                                                   size_t nv)
+{
+    __shared__ float red[4];
+    const size_t v = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    float l = 0.0f;
+    if (v < nv) {
+        const float diff = feat0[v] + y32[v] - target[v];
+        const float mk = mask[v];
+        l = mk * diff * diff;
+        u32x4* dst = reinterpret_cast<u32x4*>(gout + v * kC);
+        dst[0] = u32x4 { static_cast<uint32_t>(toBf16(2.0f * mk * diff * inv_count)), 0u, 0u, 0u };
+        dst[1] = u32x4 { 0u, 0u, 0u, 0u };
+        dst[2] = u32x4 { 0u, 0u, 0u, 0u };
+        dst[3] = u32x4 { 0u, 0u, 0u, 0u };
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        l += __shfl_down(l, off, 64);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        lpart[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(256) void finalize_kernel(const float* y32, const float* feat0, float* pred, size_t nv)
+{
+    const size_t v = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (v < nv)
+        pred[v] = feat0[v] + y32[v];
+}
+
+// ---- features and dose straight from a world's device tally
+
+__device__ inline float wordDensity(uint32_t word) { return __uint_as_float(word & 0xFFFFFF00u); }
+
+__global__ __launch_bounds__(64) void zero2_kernel(double* p)
+{
+    if (threadIdx.x < 2)
+        p[threadIdx.x] = 0.0;
+}
+
+// acc[0] += sum over body voxels (material != 0) of sum_wdE / density, acc[1] += their count.
+__global__ __launch_bounds__(256) void dose_mean_kernel(const uint32_t* voxels, const double* tally, size_t nv, double* acc)
+{
+    __shared__ double red[2][4];
+    double s = 0.0, c = 0.0;
+    for (size_t v = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; v < nv; v += static_cast<size_t>(gridDim.x) * 256) {
+        const uint32_t word = voxels[v];
+        if ((word & 0xFFu) != 0) {
+            const float rho = wordDensity(word);
+            if (rho > 0.0f)
+                s += tally[4 * v] / static_cast<double>(rho);
+            c += 1.0;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_down(s, off, 64);
+        c += __shfl_down(c, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = s;
+        red[1][threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&acc[0], (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
+        atomicAdd(&acc[1], (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+    }
+}
+
+__device__ inline double doseRef(const double* acc)
+{
+    const double r = acc[1] > 0.0 ? acc[0] / acc[1] : 0.0;
+    return r > 0.0 ? r : 1.0;
+}
+
+struct WorldFeatArgs {
+    const uint32_t* voxels;
+    const double* tally;
+    const float* emb;
+    int n_emb;
+    int nx, nxp;            // grid x extent and its padded extent
+    size_t nvp;             // padded voxels
+    double histories;
+    const double* acc;      // dose_mean_kernel sums
+    uint16_t* xin;          // 32-channel BF16 network input
+    float* feat0;
+    float* features;        // optional FP32 copy (nvp x 4)
+};
+
+// The four input features of every (padded) voxel, as the host computes them
+// from DoseScore: D / D_ref, density, material embedding, relative standard
+// error.  Padding voxels get zero features.
+__global__ __launch_bounds__(256) void world_features_kernel(WorldFeatArgs a)
+{
+    const size_t v = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (v >= a.nvp)
+        return;
+    const size_t row = v / a.nxp;
+    const int x = static_cast<int>(v - row * a.nxp);
+    float f[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+    if (x < a.nx) {
+        const size_t i = row * a.nx + x;
+        const uint32_t word = a.voxels[i];
+        const uint32_t m = word & 0xFFu;
+        const float rho = wordDensity(word);
+        const double e = a.tally[4 * i], e2 = a.tally[4 * i + 1];
+        const double d = rho > 0.0f ? e / static_cast<double>(rho) : 0.0;
+        f[0] = static_cast<float>(d / doseRef(a.acc));
+        f[1] = rho;
+        f[2] = static_cast<int>(m) < a.n_emb ? a.emb[m] : 0.0f;
+        float rel = 1.0f;
+        if (d > 0.0 && e > 0.0) {
+            const double h = a.histories;
+            const double var = h > 1.0 ? fmax(0.0, h / (h - 1.0) * (e2 - e * e / h)) : 0.0;
+            rel = fminf(static_cast<float>(sqrt(var) / e), 1.0f);
+        }
+        f[3] = rel;
+    }
+    u32x4* dst = reinterpret_cast<u32x4*>(a.xin + v * kC);
+    dst[0] = u32x4 { pack2(f[0], f[1]), pack2(f[2], f[3]), 0u, 0u };
+    dst[1] = u32x4 { 0u, 0u, 0u, 0u };
+    dst[2] = u32x4 { 0u, 0u, 0u, 0u };
+    dst[3] = u32x4 { 0u, 0u, 0u, 0u };
+    a.feat0[v] = f[0];
+    if (a.features)
+        *reinterpret_cast<f32x4*>(a.features + 4 * v) = f32x4 { f[0], f[1], f[2], f[3] };
+}
+
+// Denoised dose (mGy) from the predicted relative dose.
+__global__ __launch_bounds__(256) void world_dose_kernel(const float* pred, int nx, int nxp, size_t nv, const double* acc,
+                                                         double scale, double* out)
+{
+    const size_t v = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (v >= nv)
+        return;
+    const size_t row = v / nx;
+    const int x = static_cast<int>(v - row * nx);
+    out[v] = fmax(static_cast<double>(pred[row * nxp + x]), 0.0) * doseRef(acc) * scale;
+}
+
+// ---------------------------------------------------------------- host side
+
+int computeUnits()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess
+            && n > 0)
+            cus = n;
+        else
+            cus = 256;
+    }
+    return cus;
+}
+
+int roundUp8(int v) { return (v + 7) & ~7; }
+
+struct Dims {
+    int nb, nz, ny, nx;
+    size_t voxels() const { return static_cast<size_t>(nb) * nz * ny * nx; }
+};
+
+int checkDims(const char* fn, uint64_t batch, const uint64_t* dims, Dims& d)
+{
+    if (!dims)
+        return fail(DXG_ERR_ARGUMENT, std::string(fn) + ": null dims");
+    if (batch == 0 || dims[0] == 0 || dims[1] == 0 || dims[2] == 0)
+        return fail(DXG_ERR_ARGUMENT, std::string(fn) + ": empty volume");
+    if (dims[2] % 64 != 0)
+        return fail(DXG_ERR_ARGUMENT, std::string(fn) + ": the x extent must be a multiple of 64 (pad the volume)");
+    const double nv = static_cast<double>(batch) * dims[0] * dims[1] * dims[2];
+    if (nv > 4.0e9 || dims[0] > (1u << 20) || dims[1] > (1u << 20) || dims[2] > (1u << 20))
+        return fail(DXG_ERR_ARGUMENT, std::string(fn) + ": volume too large");
+    d = { static_cast<int>(batch), static_cast<int>(dims[0]), static_cast<int>(dims[1]), static_cast<int>(dims[2]) };
+    return DXG_OK;
+}
+
+hipError_t launchConv(ConvArgs a, const Dims& d, hipStream_t s)
+{
+    a.nb = d.nb;
+    a.nz = d.nz;
+    a.ny = d.ny;
+    a.nx = d.nx;
+    a.wave_tiles = d.nb * d.nz * d.ny * (d.nx / 64);
+    const int wg_tiles = (a.wave_tiles + 3) / 4;
+    const int grid = roundUp8(std::min(wg_tiles, 2 * computeUnits()));
+    a.chunk = (wg_tiles + grid - 1) / grid;
+    hipLaunchKernelGGL(conv3d_kernel, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+int wgradGrid(const Dims& d)
+{
+    const int units = d.nb * d.nz * (d.ny / 4) * (d.nx / 32);
+    return roundUp8(std::min(units, 2 * computeUnits()));
+}
+
+hipError_t launchWgrad(const uint16_t* x, const uint16_t* g, float* part, float* part_b, float* grad, const Dims& d,
+                       hipStream_t s)
+{
+    WgradArgs a {};
+    a.x = x;
+    a.g = g;
+    a.part = part;
+    a.part_b = part_b;
+    a.nb = d.nb;
+    a.nz = d.nz;
+    a.ny = d.ny;
+    a.nx = d.nx;
+    a.units = d.nb * d.nz * (d.ny / 4) * (d.nx / 32);
+    const int grid = wgradGrid(d);
+    a.chunk = (a.units + grid - 1) / grid;
+    hipLaunchKernelGGL(wgrad_kernel, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((kLayerP + 255) / 256), dim3(256), 0, s, part, part_b, grid, grad);
+    return hipGetLastError();
+}
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t reserve(size_t count)
+    {
+        if (count <= n && p)
+            return hipSuccess;
+        release();
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess)
+            n = count;
+        return e;
+    }
+};
+
+}  // namespace
+
+struct dxg_denoiser {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DBuf<float> params, adam_m, adam_v, grads;
+    DBuf<uint16_t> wf, wd;
+    DBuf<uint16_t> xin, acts, gbuf;
+    DBuf<float> feat0, y32, target, mask, features, lpart, part, part_b, wemb;
+    DBuf<double> wacc, wdose;
+    uint64_t step = 0;
+    bool weights_ready = false;
+    double last_ms = 0.0;
+
+    ~dxg_denoiser()
+    {
+        if (ev0)
+            (void)hipEventDestroy(ev0);
+        if (ev1)
+            (void)hipEventDestroy(ev1);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+
+    int prepWeights()
+    {
+        const int n = kLayers * kLayerW;
+        hipLaunchKernelGGL(prep_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, params.p, wf.p, wd.p, kLayers);
+        HIP_TRY(hipGetLastError());
+        weights_ready = true;
+        return DXG_OK;
+    }
+
+    const uint16_t* layerW(int l) const { return wf.p + static_cast<size_t>(l) * kLayerW; }
+    const uint16_t* layerWd(int l) const { return wd.p + static_cast<size_t>(l) * kLayerW; }
+    const float* layerBias(int l) const { return params.p + static_cast<size_t>(l) * kLayerP + kLayerW; }
+    float* layerGrad(int l) const { return grads.p + static_cast<size_t>(l) * kLayerP; }
+
+    // Stages the features on the device and packs them.
+    int upload(const float* features_host, const Dims& d)
+    {
+        const size_t nv = d.voxels();
+        HIP_TRY(features.reserve(4 * nv));
+        HIP_TRY(xin.reserve(nv * kC));
+        HIP_TRY(feat0.reserve(nv));
+        HIP_TRY(y32.reserve(nv));
+        HIP_TRY(hipMemcpyAsync(features.p, features_host, 4 * nv * sizeof(float), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(pack_features_kernel, dim3((nv + 255) / 256), dim3(256), 0, stream, features.p, xin.p, feat0.p, nv);
+        HIP_TRY(hipGetLastError());
+        return DXG_OK;
+    }
+
+    // Forward pass.  With keep, activations a_0..a_8 and h_0..h_7 are kept
+    // for the backward pass (17 tensors); otherwise two tensors are reused.
+    int forward(const Dims& d, bool keep)
+    {
+        const size_t nv = d.voxels(), ts = nv * kC;
+        HIP_TRY(acts.reserve((keep ? 17 : 2) * ts));
+        auto aT = [&](int b) { return acts.p + (keep ? static_cast<size_t>(b) : 0) * ts; };
+        auto hT = [&](int b) { return acts.p + (keep ? static_cast<size_t>(9 + b) : 1) * ts; };
+        ConvArgs c {};
+        c.x = xin.p;
+        c.w = layerW(0);
+        c.bias = layerBias(0);
+        c.y = aT(0);
+        c.relu = 1;
+        HIP_TRY(launchConv(c, d, stream));
+        for (int b = 0; b < kBlocks; ++b) {
+            c = ConvArgs {};
+            c.x = aT(b);
+            c.w = layerW(2 * b + 1);
+            c.bias = layerBias(2 * b + 1);
+            c.y = hT(b);
+            c.relu = 1;
+            HIP_TRY(launchConv(c, d, stream));
+            c = ConvArgs {};
+            c.x = hT(b);
+            c.w = layerW(2 * b + 2);
+            c.bias = layerBias(2 * b + 2);
+            c.res = aT(b);
+            c.y = aT(b + 1);
+            c.relu = 1;
+            HIP_TRY(launchConv(c, d, stream));
+        }
+        c = ConvArgs {};
+        c.x = aT(kBlocks);
+        c.w = layerW(kLayers - 1);
+        c.bias = layerBias(kLayers - 1);
+        c.y32 = y32.p;
+        c.y32_channels = 1;
+        HIP_TRY(launchConv(c, d, stream));
+        return DXG_OK;
+    }
+
+    // Features of a built world's current scores into xin / feat0 (and the
+    // FP32 features buffer when asked); dm receives the padded volume.
+    int worldFeatures(const dxg_world* w, dxg::WorldView& v, Dims& dm, bool keep_fp32)
+    {
+        if (int rc = dxg::worldView(w, v); rc != DXG_OK)
+            return rc;
+        int dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        if (dev != v.device)
+            return fail(DXG_ERR_ARGUMENT, "denoiser and world live on different devices");
+        if (v.embedding.empty())
+            return fail(DXG_ERR_STATE, "world has no materials");
+        const int nxp = (v.nx + 63) / 64 * 64;
+        dm = { 1, v.nz, v.ny, nxp };
+        const size_t nvp = dm.voxels();
+        const size_t nv = static_cast<size_t>(v.nx) * v.ny * v.nz;
+        HIP_TRY(xin.reserve(nvp * kC));
+        HIP_TRY(feat0.reserve(nvp));
+        HIP_TRY(y32.reserve(nvp));
+        HIP_TRY(wacc.reserve(2));
+        HIP_TRY(wemb.reserve(v.embedding.size()));
+        if (keep_fp32)
+            HIP_TRY(features.reserve(4 * nvp));
+        HIP_TRY(hipMemcpyAsync(wemb.p, v.embedding.data(), v.embedding.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(zero2_kernel, dim3(1), dim3(64), 0, stream, wacc.p);
+        const int mblocks = static_cast<int>(std::min<size_t>((nv + 255) / 256, 4096));
+        hipLaunchKernelGGL(dose_mean_kernel, dim3(mblocks), dim3(256), 0, stream, v.voxels, v.tally, nv, wacc.p);
+        WorldFeatArgs a {};
+        a.voxels = v.voxels;
+        a.tally = v.tally;
+        a.emb = wemb.p;
+        a.n_emb = static_cast<int>(v.embedding.size());
+        a.nx = v.nx;
+        a.nxp = nxp;
+        a.nvp = nvp;
+        a.histories = static_cast<double>(v.histories);
+        a.acc = wacc.p;
+        a.xin = xin.p;
+        a.feat0 = feat0.p;
+        a.features = keep_fp32 ? features.p : nullptr;
+        hipLaunchKernelGGL(world_features_kernel, dim3((nvp + 255) / 256), dim3(256), 0, stream, a);
+        HIP_TRY(hipGetLastError());
+        return DXG_OK;
+    }
+
+    // Backward pass from the loss gradient in gbuf tensor 0; writes every
+    // layer's gradient into grads.
+    int backward(const Dims& d)
+    {
+        const size_t ts = d.voxels() * kC;
+        auto aT = [&](int b) { return acts.p + static_cast<size_t>(b) * ts; };
+        auto hT = [&](int b) { return acts.p + static_cast<size_t>(9 + b) * ts; };
+        uint16_t* gout = gbuf.p;
+        uint16_t* gs = gbuf.p + ts;
+        uint16_t* gh = gbuf.p + 2 * ts;
+        const int grid = wgradGrid(d);
+        HIP_TRY(part.reserve(static_cast<size_t>(grid) * kLayerW));
+        HIP_TRY(part_b.reserve(static_cast<size_t>(grid) * 32));
+
+        HIP_TRY(launchWgrad(aT(kBlocks), gout, part.p, part_b.p, layerGrad(kLayers - 1), d, stream));
+        ConvArgs c {};
+        c.x = gout;
+        c.w = layerWd(kLayers - 1);
+        c.gate = aT(kBlocks);
+        c.y = gs;
+        HIP_TRY(launchConv(c, d, stream));
+        for (int b = kBlocks - 1; b >= 0; --b) {
+            HIP_TRY(launchWgrad(hT(b), gs, part.p, part_b.p, layerGrad(2 * b + 2), d, stream));
+            c = ConvArgs {};
+            c.x = gs;
+            c.w = layerWd(2 * b + 2);
+            c.gate = hT(b);
+            c.y = gh;
+            HIP_TRY(launchConv(c, d, stream));
+            HIP_TRY(launchWgrad(aT(b), gh, part.p, part_b.p, layerGrad(2 * b + 1), d, stream));
+            c = ConvArgs {};
+            c.x = gh;
+            c.w = layerWd(2 * b + 1);
+            c.res = gs;
+            c.gate = aT(b);
+            c.y = gs;
+            HIP_TRY(launchConv(c, d, stream));
+        }
+        HIP_TRY(launchWgrad(xin.p, gs, part.p, part_b.p, layerGrad(0), d, stream));
+        return DXG_OK;
+    }
+};
+
+namespace {
+
+void initParams(std::vector<float>& p, uint64_t seed)
+{
+    p.assign(static_cast<size_t>(kLayers) * kLayerP, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (int l = 0; l < kLayers - 1; ++l) {
+        const int ci_active = l == 0 ? DXG_DENOISE_FEATURES : kC;
+        // He initialisation; the second convolution of each residual block
+        // starts small so every block begins close to the identity.
+        const bool second = l > 0 && (l % 2) == 0;
+        std::normal_distribution<float> nd(0.0f, (second ? 0.1f : 1.0f) * std::sqrt(2.0f / (kTaps * ci_active)));
+        float* w = p.data() + static_cast<size_t>(l) * kLayerP;
+        for (int co = 0; co < kC; ++co)
+            for (int tap = 0; tap < kTaps; ++tap)
+                for (int ci = 0; ci < ci_active; ++ci)
+                    w[(co * kTaps + tap) * kC + ci] = nd(rng);
+    }
+    // the correction layer starts at zero: the untrained network is the identity on the log dose
+}
+
+bool hasDevice()
+{
+    int n = 0;
+    return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t dxg_denoiser_num_params(void) { return static_cast<uint64_t>(kLayers) * kLayerP; }
+
+int dxg_denoiser_create(dxg_denoiser** out, uint64_t seed)
+{
+    if (!out)
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_create: null output pointer");
+    *out = nullptr;
+    if (!hasDevice())
+        return fail(DXG_ERR_DEVICE, "dxg_denoiser_create: no HIP device");
+    auto* d = new (std::nothrow) dxg_denoiser();
+    if (!d)
+        return fail(DXG_ERR_DEVICE, "dxg_denoiser_create: out of host memory");
+    auto cleanup = [&](int code) {
+        delete d;
+        return code;
+    };
+    const size_t np = dxg_denoiser_num_params();
+    std::vector<float> host;
+    initParams(host, seed);
+#define TRY_CLEAN(expr)                                                                                   \
+    do {                                                                                                  \
+        const hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                             \
+            return cleanup(fail(DXG_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)));        \
+    } while (0)
+    TRY_CLEAN(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    TRY_CLEAN(hipEventCreate(&d->ev0));
+    TRY_CLEAN(hipEventCreate(&d->ev1));
+    TRY_CLEAN(d->params.reserve(np));
+    TRY_CLEAN(d->adam_m.reserve(np));
+    TRY_CLEAN(d->adam_v.reserve(np));
+    TRY_CLEAN(d->grads.reserve(np));
+    TRY_CLEAN(d->wf.reserve(static_cast<size_t>(kLayers) * kLayerW));
+    TRY_CLEAN(d->wd.reserve(static_cast<size_t>(kLayers) * kLayerW));
+    TRY_CLEAN(hipMemcpyAsync(d->params.p, host.data(), np * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    TRY_CLEAN(hipMemsetAsync(d->adam_m.p, 0, np * sizeof(float), d->stream));
+    TRY_CLEAN(hipMemsetAsync(d->adam_v.p, 0, np * sizeof(float), d->stream));
+    TRY_CLEAN(hipMemsetAsync(d->grads.p, 0, np * sizeof(float), d->stream));
+    TRY_CLEAN(hipStreamSynchronize(d->stream));
+#undef TRY_CLEAN
+    *out = d;
+    return DXG_OK;
+}
+
+void dxg_denoiser_destroy(dxg_denoiser* d)
+{
+    if (d && d->stream)
+        (void)hipStreamSynchronize(d->stream);
+    delete d;
+}
+
+int dxg_denoiser_get_params(const dxg_denoiser* d, float* params, uint64_t n)
+{
+    if (!d || !params)
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_get_params: null argument");
+    if (n != dxg_denoiser_num_params())
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_get_params: wrong parameter count");
+    HIP_TRY(hipMemcpyAsync(params, d->params.p, n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return DXG_OK;
+}
+
+int dxg_denoiser_get_grads(const dxg_denoiser* d, float* grads, uint64_t n)
+{
+    if (!d || !grads)
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_get_grads: null argument");
+    if (n != dxg_denoiser_num_params())
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_get_grads: wrong parameter count");
+    HIP_TRY(hipMemcpyAsync(grads, d->grads.p, n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return DXG_OK;
+}
+
+int dxg_denoiser_set_params(dxg_denoiser* d, const float* params, uint64_t n)
+{
+    if (!d || !params)
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_set_params: null argument");
+    if (n != dxg_denoiser_num_params())
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_set_params: wrong parameter count");
+    HIP_TRY(hipMemcpyAsync(d->params.p, params, n * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipMemsetAsync(d->adam_m.p, 0, n * sizeof(float), d->stream));
+    HIP_TRY(hipMemsetAsync(d->adam_v.p, 0, n * sizeof(float), d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    d->step = 0;
+    d->weights_ready = false;
+    return DXG_OK;
+}
+
+int dxg_denoiser_train_step(dxg_denoiser* d, const float* features, const float* target, const float* mask,
+                            uint64_t batch, const uint64_t dims[3], float lr, float* loss)
+{
+    if (!d || !features || !target || !mask || !loss)
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_train_step: null argument");
+    Dims dm {};
+    if (int rc = checkDims("dxg_denoiser_train_step", batch, dims, dm); rc != DXG_OK)
+        return rc;
+    if (dm.ny % 4 != 0)
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_train_step: the y extent must be a multiple of 4");
+    if (!(lr > 0.0f) || !std::isfinite(lr))
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_train_step: learning rate must be positive");
+    const size_t nv = dm.voxels();
+    double count = 0.0;
+    for (size_t i = 0; i < nv; ++i)
+        count += mask[i];
+    if (!(count > 0.0))
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_train_step: empty mask");
+
+    HIP_TRY(hipEventRecord(d->ev0, d->stream));
+    if (int rc = d->upload(features, dm); rc != DXG_OK)
+        return rc;
+    HIP_TRY(d->target.reserve(nv));
+    HIP_TRY(d->mask.reserve(nv));
+    HIP_TRY(hipMemcpyAsync(d->target.p, target, nv * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipMemcpyAsync(d->mask.p, mask, nv * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    if (int rc = d->prepWeights(); rc != DXG_OK)
+        return rc;
+    if (int rc = d->forward(dm, true); rc != DXG_OK)
+        return rc;
+    HIP_TRY(d->gbuf.reserve(3 * nv * kC));
+    const int lblocks = static_cast<int>((nv + 255) / 256);
+    HIP_TRY(d->lpart.reserve(lblocks));
+    hipLaunchKernelGGL(loss_kernel, dim3(lblocks), dim3(256), 0, d->stream, d->y32.p, d->feat0.p, d->target.p, d->mask.p,
+                       static_cast<float>(1.0 / count), d->gbuf.p, d->lpart.p, nv);
+    HIP_TRY(hipGetLastError());
+    if (int rc = d->backward(dm); rc != DXG_OK)
+        return rc;
+    d->step += 1;
+    const double b1 = 0.9, b2 = 0.999;
+    const float bc1 = static_cast<float>(1.0 / (1.0 - std::pow(b1, static_cast<double>(d->step))));
+    const float bc2 = static_cast<float>(1.0 / (1.0 - std::pow(b2, static_cast<double>(d->step))));
+    const int np = static_cast<int>(dxg_denoiser_num_params());
+    hipLaunchKernelGGL(adam_kernel, dim3((np + 255) / 256), dim3(256), 0, d->stream, d->params.p, d->adam_m.p, d->adam_v.p,
+                       d->grads.p, np, lr, static_cast<float>(b1), static_cast<float>(b2), 1e-8f, bc1, bc2);
+    HIP_TRY(hipGetLastError());
+    d->weights_ready = false;
+    HIP_TRY(hipEventRecord(d->ev1, d->stream));
+    std::vector<float> lp(lblocks);
+    HIP_TRY(hipMemcpyAsync(lp.data(), d->lpart.p, lblocks * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+    d->last_ms = ms;
+    double s = 0.0;
+    for (float v : lp)
+        s += v;
+    *loss = static_cast<float>(s / count);
+    return DXG_OK;
+}
+
+int dxg_denoiser_infer(dxg_denoiser* d, const float* features, const uint64_t dims[3], float* prediction)
+{
+    if (!d || !features || !prediction)
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_infer: null argument");
+    Dims dm {};
+    if (int rc = checkDims("dxg_denoiser_infer", 1, dims, dm); rc != DXG_OK)
+        return rc;
+    const size_t nv = dm.voxels();
+    if (int rc = d->upload(features, dm); rc != DXG_OK)
+        return rc;
+    HIP_TRY(hipEventRecord(d->ev0, d->stream));
+    if (!d->weights_ready)
+        if (int rc = d->prepWeights(); rc != DXG_OK)
+            return rc;
+    if (int rc = d->forward(dm, false); rc != DXG_OK)
+        return rc;
+    hipLaunchKernelGGL(finalize_kernel, dim3((nv + 255) / 256), dim3(256), 0, d->stream, d->y32.p, d->feat0.p, d->y32.p, nv);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(d->ev1, d->stream));
+    HIP_TRY(hipMemcpyAsync(prediction, d->y32.p, nv * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+    d->last_ms = ms;
+    return DXG_OK;
+}
+
+double dxg_denoiser_last_ms(const dxg_denoiser* d) { return d ? d->last_ms : 0.0; }
+
+int dxg_denoiser_world_features(dxg_denoiser* d, const dxg_world* w, float* features, uint64_t n)
+{
+    if (!d || !w || !features)
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_world_features: null argument");
+    dxg::WorldView v;
+    Dims dm {};
+    if (int rc = d->worldFeatures(w, v, dm, true); rc != DXG_OK)
+        return rc;
+    if (n != 4 * dm.voxels())
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_world_features: n must be 4 x nz x ny x (nx padded to 64)");
+    HIP_TRY(hipMemcpyAsync(features, d->features.p, n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return DXG_OK;
+}
+
+int dxg_denoiser_apply(dxg_denoiser* d, const dxg_world* w, double* dose_mgy)
+{
+    if (!d || !w)
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_apply: null argument");
+    HIP_TRY(hipEventRecord(d->ev0, d->stream));
+    dxg::WorldView v;
+    Dims dm {};
+    if (int rc = d->worldFeatures(w, v, dm, false); rc != DXG_OK)
+        return rc;
+    if (!d->weights_ready)
+        if (int rc = d->prepWeights(); rc != DXG_OK)
+            return rc;
+    if (int rc = d->forward(dm, false); rc != DXG_OK)
+        return rc;
+    const size_t nvp = dm.voxels();
+    const size_t nv = static_cast<size_t>(v.nx) * v.ny * v.nz;
+    hipLaunchKernelGGL(finalize_kernel, dim3((nvp + 255) / 256), dim3(256), 0, d->stream, d->y32.p, d->feat0.p, d->y32.p, nvp);
+    HIP_TRY(d->wdose.reserve(nv));
+    const double voxel_cm3 = v.spacing[0] * v.spacing[1] * v.spacing[2];
+    const double scale = dxg::kKeVToJ / (voxel_cm3 * 1e-3) * 1e3;   // keV per g/cm3 -> mGy
+    hipLaunchKernelGGL(world_dose_kernel, dim3((nv + 255) / 256), dim3(256), 0, d->stream, d->y32.p, v.nx, dm.nx, nv,
+                       d->wacc.p, scale, d->wdose.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(d->ev1, d->stream));
+    if (dose_mgy)
+        HIP_TRY(hipMemcpyAsync(dose_mgy, d->wdose.p, nv * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+    d->last_ms = ms;
+    return DXG_OK;
+}
+
+int dxg_conv3d_probe(int mode, const float* x, const float* w, const float* bias_or_dy, uint64_t n, const uint64_t dims[3],
+                     float* out)
+{
+    if (!x || !w || !out || (mode != 1 && !bias_or_dy))
+        return fail(DXG_ERR_ARGUMENT, "dxg_conv3d_probe: null argument");
+    if (mode < 0 || mode > 2)
+        return fail(DXG_ERR_ARGUMENT, "dxg_conv3d_probe: mode must be 0, 1 or 2");
+    Dims dm {};
+    if (int rc = checkDims("dxg_conv3d_probe", n, dims, dm); rc != DXG_OK)
+        return rc;
+    if (mode == 2 && dm.ny % 4 != 0)
+        return fail(DXG_ERR_ARGUMENT, "dxg_conv3d_probe: the y extent must be a multiple of 4");
+    if (!hasDevice())
+        return fail(DXG_ERR_DEVICE, "dxg_conv3d_probe: no HIP device");
+    const size_t nv = dm.voxels(), ts = nv * kC;
+    std::vector<uint16_t> xh(ts);
+    for (size_t i = 0; i < ts; ++i)
+        xh[i] = toBf16(x[i]);
+    std::vector<float> layer(kLayerP, 0.0f);
+    std::copy(w, w + kLayerW, layer.begin());
+    if (mode == 0)
+        std::copy(bias_or_dy, bias_or_dy + kC, layer.begin() + kLayerW);
+    DBuf<uint16_t> dx, dy, wf, wd;
+    DBuf<float> dp, dout, part, part_b;
+    HIP_TRY(dx.reserve(ts));
+    HIP_TRY(dp.reserve(kLayerP));
+    HIP_TRY(wf.reserve(kLayerW));
+    HIP_TRY(wd.reserve(kLayerW));
+    HIP_TRY(hipMemcpy(dx.p, xh.data(), ts * 2, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dp.p, layer.data(), kLayerP * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(prep_weights_kernel, dim3((kLayerW + 255) / 256), dim3(256), 0, nullptr, dp.p, wf.p, wd.p, 1);
+    HIP_TRY(hipGetLastError());
+    if (mode == 2) {
+        std::vector<uint16_t> gh(ts);
+        for (size_t i = 0; i < ts; ++i)
+            gh[i] = toBf16(bias_or_dy[i]);
+        HIP_TRY(dy.reserve(ts));
+        HIP_TRY(hipMemcpy(dy.p, gh.data(), ts * 2, hipMemcpyHostToDevice));
+        const int grid = wgradGrid(dm);
+        HIP_TRY(part.reserve(static_cast<size_t>(grid) * kLayerW));
+        HIP_TRY(part_b.reserve(static_cast<size_t>(grid) * 32));
+        HIP_TRY(dout.reserve(kLayerP));
+        HIP_TRY(launchWgrad(dx.p, dy.p, part.p, part_b.p, dout.p, dm, nullptr));
+        HIP_TRY(hipMemcpy(out, dout.p, kLayerP * sizeof(float), hipMemcpyDeviceToHost));
+        return DXG_OK;
+    }
+    HIP_TRY(dout.reserve(ts));
+    ConvArgs c {};
+    c.x = dx.p;
+    c.w = mode == 0 ? wf.p : wd.p;
+    c.bias = mode == 0 ? dp.p + kLayerW : nullptr;
+    c.y32 = dout.p;
+    c.y32_channels = kC;
+    HIP_TRY(launchConv(c, dm, nullptr));
+    HIP_TRY(hipMemcpy(out, dout.p, ts * sizeof(float), hipMemcpyDeviceToHost));
+    return DXG_OK;
+}
+
+int dxg_mfma_probe(const float* a, const float* b, float* d)
+{
+    if (!a || !b || !d)
+        return fail(DXG_ERR_ARGUMENT, "dxg_mfma_probe: null argument");
+    if (!hasDevice())
+        return fail(DXG_ERR_DEVICE, "dxg_mfma_probe: no HIP device");
+    DBuf<float> da, db, dd;
+    HIP_TRY(da.reserve(16 * 32));
+    HIP_TRY(db.reserve(32 * 16));
+    HIP_TRY(dd.reserve(16 * 16));
+    HIP_TRY(hipMemcpy(da.p, a, 16 * 32 * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(db.p, b, 32 * 16 * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, nullptr, da.p, db.p, dd.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(d, dd.p, 16 * 16 * sizeof(float), hipMemcpyDeviceToHost));
+    return DXG_OK;
+}
+
+}  // extern "C"

@@ -1,9 +1,12 @@
 // Types shared by the host world code and the HIP transport kernels.
 #pragma once
 
+#include "dxmc_hip.h"
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace dxg {
 
@@ -112,5 +115,18 @@ int transportBlockSize();
 void launchCopy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 // Resident blocks per CU of the transport kernel at the given LDS size.
 int transportBlocksPerCU(bool lds_lut, int level, size_t lds_bytes);
+
+// Device-resident state of a built world, for kernels outside the transport
+// (the dose denoiser reads the tally and the voxel words in place).
+struct WorldView {
+    const uint32_t* voxels;    // packed words (packVoxel)
+    const double* tally;       // 4 doubles per voxel: sum w dE, sum (w dE)^2, events, unused
+    int nx, ny, nz;
+    double spacing[3];         // cm
+    uint64_t histories;
+    int device;
+    std::vector<float> embedding;   // per material: total mass attenuation at 60 keV over water's
+};
+int worldView(const dxg_world* w, WorldView& out);
 
 }  // namespace dxg

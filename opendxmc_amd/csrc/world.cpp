@@ -124,6 +124,39 @@ int setError(int code, const std::string& msg)
     g_error = msg;
     return code;
 }
+
+int worldView(const dxg_world* w, WorldView& out)
+{
+    if (!w || !w->built)
+        return setError(DXG_ERR_STATE, "world not built");
+    std::string err;
+    const PhysicsList* pl = sharedPhysics(err);
+    if (!pl)
+        return setError(DXG_ERR_PHYSICS, err);
+    const int32_t wz[2] = { 1, 8 };
+    const double ww[2] = { 0.111894, 0.888106 };
+    MixMaterial water;
+    if (!makeMaterial(*pl, wz, ww, 2, water, err))
+        return setError(DXG_ERR_PHYSICS, err);
+    double pe, inc, coh;
+    water.attenuation(*pl, 60.0, pe, inc, coh);
+    const double mu_w = pe + inc + coh;
+    out.embedding.clear();
+    for (const auto& m : w->mats) {
+        m.attenuation(*pl, 60.0, pe, inc, coh);
+        out.embedding.push_back(static_cast<float>((pe + inc + coh) / mu_w));
+    }
+    out.voxels = w->d_voxels.ptr;
+    out.tally = w->d_tally.ptr;
+    out.nx = static_cast<int>(w->dims[0]);
+    out.ny = static_cast<int>(w->dims[1]);
+    out.nz = static_cast<int>(w->dims[2]);
+    for (int a = 0; a < 3; ++a)
+        out.spacing[a] = w->spacing[a];
+    out.histories = w->histories;
+    out.device = w->device;
+    return DXG_OK;
+}
 }  // namespace dxg
 
 namespace {

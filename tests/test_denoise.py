@@ -1,0 +1,313 @@
+"""Dose denoiser (include/dxmc_denoise.h).
+
+CPU: the C-ABI's argument checks, the feature maps, and the PyTorch
+restatement of the network's backward pass against autograd (which pins the
+reference the GPU tests use).  GPU: the matrix-core register layout, each
+convolution kernel (forward, input gradient, weight gradient) against
+torch.nn.functional.conv3d and torch.nn.grad on BF16-rounded inputs, the
+whole network's prediction, loss and every parameter gradient against the
+restatement, and that training lowers the loss.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from opendxmc_amd import _native as N
+from opendxmc_amd import denoise as dn
+
+import denoise_reference as ref
+
+
+def _bf16(a: np.ndarray) -> np.ndarray:
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(torch.bfloat16).to(torch.float32).numpy()
+
+
+def _pf(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _probe(mode, x, w, extra, n, dims):
+    dims_a = np.array(dims, dtype=np.uint64)
+    out = np.zeros(ref.LAYER_P if mode == 2 else x.size, dtype=np.float32)
+    extra_p = _pf(extra) if extra is not None else None
+    rc = N.lib().dxg_conv3d_probe(mode, _pf(x), _pf(w), extra_p, n, dims_a.ctypes.data_as(C.POINTER(N.u64)), _pf(out))
+    N.check(rc)
+    return out
+
+
+# ------------------------------------------------------------------ CPU
+
+def test_parameter_count():
+    # 18 layers x (32 x 27 x 32 weights + 32 biases), padded channels included
+    assert dn.Denoiser.num_params() == 18 * (32 * 27 * 32 + 32)
+    # the unpadded network: 4->32, 16 x 32->32, 32->1
+    live = (4 * 27 * 32 + 32) + 16 * (32 * 27 * 32 + 32) + (32 * 27 + 1)
+    assert live == 447_233
+
+
+def test_probe_rejects_bad_shapes():
+    x = np.zeros(2 * 4 * 4 * 48 * 32, dtype=np.float32)
+    w = np.zeros(ref.LAYER_W, dtype=np.float32)
+    b = np.zeros(32, dtype=np.float32)
+    with pytest.raises(N.NativeError) as e:
+        _probe(0, x, w, b, 1, (4, 4, 48))
+    assert e.value.code == N.DXG_ERR_ARGUMENT and "multiple of 64" in str(e.value)
+    x = np.zeros(1 * 2 * 6 * 64 * 32, dtype=np.float32)
+    with pytest.raises(N.NativeError) as e:
+        _probe(2, x, w, x, 1, (2, 6, 64))
+    assert e.value.code == N.DXG_ERR_ARGUMENT and "multiple of 4" in str(e.value)
+    with pytest.raises(N.NativeError) as e:
+        N.check(N.lib().dxg_conv3d_probe(3, _pf(x), _pf(w), _pf(b), 1,
+                                         np.array([2, 6, 64], dtype=np.uint64).ctypes.data_as(C.POINTER(N.u64)),
+                                         _pf(x)))
+    assert e.value.code == N.DXG_ERR_ARGUMENT
+
+
+def test_create_without_device_fails_loudly():
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(N.NativeError) as e:
+        dn.Denoiser(seed=1)
+    assert e.value.code == N.DXG_ERR_DEVICE
+
+
+def test_feature_maps():
+    shape = (2, 3, 4)
+    dose = np.full(shape, 2.0)
+    dose[0, 0, 0] = 0.0
+    var = (0.1 * dose) ** 2
+    material = np.ones(shape, dtype=np.uint8)
+    material[1, 2, 3] = 0
+    density = np.full(shape, 1.05)
+    emb = np.array([0.9, 1.1], dtype=np.float32)
+    f, dref = dn.features(dn.DoseVolume(dose, var, density, material, emb))
+    body = material != 0
+    assert dref == pytest.approx(dose[body].mean())
+    assert f.shape == shape + (4,) and f.dtype == np.float32
+    assert f[1, 1, 1, 0] == pytest.approx(2.0 / dref, rel=1e-6)
+    assert f[0, 0, 0, 0] == 0.0
+    assert f[1, 1, 1, 3] == pytest.approx(0.1, rel=1e-6)
+    assert f[0, 0, 0, 3] == 1.0                       # no events: full uncertainty
+    assert f[1, 2, 3, 2] == pytest.approx(0.9) and f[0, 1, 1, 2] == pytest.approx(1.1)
+    back = dn.dose_from_relative(dn.relative_dose(dose, dref), dref)
+    np.testing.assert_allclose(back, dose, rtol=1e-6, atol=1e-9)
+
+
+def test_material_embedding_water_is_one():
+    emb = dn.material_embedding([{1: 0.111894, 8: 0.888106}, {7: 0.8, 8: 0.2}, {20: 1.0}])
+    assert emb[0] == pytest.approx(1.0, rel=1e-12)
+    assert 0.8 < emb[1] < 1.0          # air: slightly lower mass attenuation than water at 60 keV
+    assert emb[2] > 3.0                # calcium: photoelectric dominated
+
+
+def test_pad_and_error_metric():
+    a = np.ones((2, 3, 70, 4))
+    p = dn.pad_x(a)
+    assert p.shape == (2, 3, 128, 4) and p[:, :, 70:].sum() == 0
+    r = np.array([1.0, 2.5, 10.0, 0.1])
+    d = np.array([1.1, 2.5, 9.0, 5.0])
+    # voxels above 20 % of max: 2.5 and 10.0 (1.0 and 0.1 are below 2.0)
+    assert dn.mean_relative_error(d, r) == pytest.approx((0.0 + 0.1) / 2)
+
+
+def test_reference_backward_matches_autograd():
+    """The hand-written backward of the restatement (unrounded, float64) equals
+    autograd of its own forward, so a wrong gate, a transposed or unflipped
+    weight or a missing residual term in it would fail here."""
+    rng = np.random.default_rng(5)
+    params = ref.random_params(3, scale=0.7)
+    f = rng.normal(0.0, 1.0, (1, 2, 4, 4, 4))
+    target = rng.normal(0.0, 1.0, (1, 2, 4, 4))
+    mask = (rng.random((1, 2, 4, 4)) < 0.8).astype(np.float64)
+    loss, _, grads = ref.loss_and_grads(params, f, target, mask, rounding=False, dtype=torch.float64)
+
+    layers = [(w.to(torch.float64).requires_grad_(), b.to(torch.float64).requires_grad_()) for w, b in ref.split(params)]
+    x0 = ref.pad_features(f, torch.float64)
+    a = torch.relu(ref.conv(x0, *layers[0]))
+    for k in range(ref.BLOCKS):
+        h = torch.relu(ref.conv(a, *layers[2 * k + 1]))
+        a = torch.relu(ref.conv(h, *layers[2 * k + 2]) + a)
+    pred = torch.from_numpy(f[..., 0]) + ref.conv(a, *layers[-1])[..., 0]
+    m = torch.from_numpy(mask)
+    l2 = (m * (pred - torch.from_numpy(target)) ** 2).sum() / m.sum()
+    l2.backward()
+    auto = np.concatenate([ref.flat_grad(w.grad, b.grad) for w, b in layers])
+    assert loss == pytest.approx(float(l2), rel=1e-12)
+    np.testing.assert_allclose(grads, auto, rtol=1e-9, atol=1e-12 * np.abs(auto).max())
+
+
+# ------------------------------------------------------------------ GPU
+
+@pytest.mark.gpu
+def test_mfma_register_layout():
+    rng = np.random.default_rng(1)
+    a = rng.integers(-4, 5, (16, 32)).astype(np.float32)
+    b = rng.integers(-4, 5, (32, 16)).astype(np.float32)
+    d = np.zeros((16, 16), dtype=np.float32)
+    N.check(N.lib().dxg_mfma_probe(_pf(a), _pf(b), _pf(d)))
+    np.testing.assert_array_equal(d, a @ b)
+
+
+CONV_CASES = [(2, (5, 8, 64)), (1, (3, 4, 128)), (1, (1, 4, 64))]
+
+
+def _conv_inputs(seed, n, dims):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0.0, 1.0, (n,) + dims + (32,)).astype(np.float32)
+    w = (rng.normal(0.0, 1.0, (32, 27, 32)) / np.sqrt(27 * 32)).astype(np.float32)
+    b = rng.normal(0.0, 0.1, 32).astype(np.float32)
+    return x, w, b
+
+
+def _torch_w(w):
+    return torch.from_numpy(_bf16(w).reshape(32, 3, 3, 3, 32).transpose(0, 4, 1, 2, 3).copy())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,dims", CONV_CASES)
+def test_conv_forward_matches_torch(n, dims):
+    x, w, b = _conv_inputs(11, n, dims)
+    out = _probe(0, x, w.reshape(-1), b, n, dims).reshape(x.shape)
+    want = ref.conv(torch.from_numpy(_bf16(x)), _torch_w(w), torch.from_numpy(b)).numpy()
+    # FP32 accumulation of 864 BF16 products in another order: ~1e-6 relative
+    np.testing.assert_allclose(out, want, rtol=0, atol=2e-5 * np.abs(want).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,dims", CONV_CASES)
+def test_conv_input_grad_matches_torch(n, dims):
+    g, w, _ = _conv_inputs(12, n, dims)
+    out = _probe(1, g, w.reshape(-1), None, n, dims).reshape(g.shape)
+    want = ref.conv_input_grad(torch.from_numpy(_bf16(g)), _torch_w(w)).numpy()
+    np.testing.assert_allclose(out, want, rtol=0, atol=2e-5 * np.abs(want).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,dims", CONV_CASES)
+def test_conv_weight_grad_matches_torch(n, dims):
+    x, w, _ = _conv_inputs(13, n, dims)
+    g = np.random.default_rng(14).normal(0.0, 1.0, x.shape).astype(np.float32)
+    out = _probe(2, x, w.reshape(-1), g, n, dims)
+    xt, gt = torch.from_numpy(_bf16(x)), torch.from_numpy(_bf16(g))
+    want = ref.flat_grad(ref.conv_weight_grad(xt, gt), gt.sum(dim=(0, 1, 2, 3)))
+    # sums over up to 6144 voxels in FP32
+    np.testing.assert_allclose(out, want, rtol=0, atol=5e-5 * np.abs(want).max())
+
+
+def _net_case(seed, batch, dims):
+    rng = np.random.default_rng(seed)
+    f = np.empty((batch,) + dims + (4,), dtype=np.float32)
+    f[..., 0] = rng.normal(0.0, 1.0, (batch,) + dims)
+    f[..., 1] = rng.uniform(0.0, 2.0, (batch,) + dims)
+    f[..., 2] = rng.uniform(0.8, 1.6, (batch,) + dims)
+    f[..., 3] = rng.uniform(0.0, 1.0, (batch,) + dims)
+    target = (f[..., 0] * 0.5 + rng.normal(0.0, 0.1, (batch,) + dims)).astype(np.float32)
+    mask = (rng.random((batch,) + dims) < 0.7).astype(np.float32)
+    return f, target, mask
+
+
+@pytest.mark.gpu
+def test_network_prediction_matches_reference():
+    params = ref.random_params(21, scale=0.8)
+    net = dn.Denoiser(seed=1)
+    net.set_params(params)
+    f, _, _ = _net_case(22, 1, (4, 8, 64))
+    pred = net.infer(f[0])
+    want, _ = ref.forward(params, f)
+    want = want.numpy()[0]
+    err = np.abs(pred - want)
+    # identical BF16 rounding points; FP32 summation order differs
+    assert err.max() <= 2e-3 * max(1.0, np.abs(want).max()), err.max()
+
+
+@pytest.mark.gpu
+def test_network_loss_and_gradients_match_reference():
+    params = ref.random_params(31, scale=0.8)
+    net = dn.Denoiser(seed=1)
+    net.set_params(params)
+    f, target, mask = _net_case(32, 2, (4, 8, 64))
+    loss = net.train_step(f, target, mask, lr=1e-6)
+    grads = net.grads().astype(np.float64)
+    rloss, _, rgrads = ref.loss_and_grads(params, f, target, mask)
+    assert loss == pytest.approx(rloss, rel=2e-4)
+    for l in range(ref.LAYERS):
+        g = grads[l * ref.LAYER_P:(l + 1) * ref.LAYER_P]
+        r = rgrads[l * ref.LAYER_P:(l + 1) * ref.LAYER_P]
+        scale = np.abs(r).max()
+        assert scale > 0, l
+        err = np.abs(g - r).max() / scale
+        assert err < 2e-2, (l, err)
+        # the padded channels stay exactly zero
+        if l == 0:
+            assert np.all(g[:ref.LAYER_W].reshape(32, 27, 32)[:, :, 4:] == 0)
+        if l == ref.LAYERS - 1:
+            assert np.all(g[:ref.LAYER_W].reshape(32, 27, 32)[1:] == 0) and np.all(g[ref.LAYER_W + 1:] == 0)
+
+
+@pytest.mark.gpu
+def test_training_lowers_the_loss():
+    """Noisy copies of a smooth dose field: the untrained network is the
+    identity (loss = noise variance); training must remove part of the noise."""
+    rng = np.random.default_rng(41)
+    dims = (8, 16, 64)
+    z, y, x = np.meshgrid(*[np.linspace(-1, 1, n) for n in dims], indexing="ij")
+    net = dn.Denoiser(seed=3)
+    losses = []
+    for step in range(60):
+        f = np.zeros((2,) + dims + (4,), dtype=np.float32)
+        t = np.zeros((2,) + dims, dtype=np.float32)
+        for b in range(2):
+            c = rng.uniform(-0.5, 0.5, 3)
+            field = np.exp(-2.0 * ((z - c[0]) ** 2 + (y - c[1]) ** 2 + (x - c[2]) ** 2))
+            t[b] = field
+            f[b, ..., 0] = field + rng.normal(0.0, 0.5, dims)
+            f[b, ..., 1] = 1.0
+            f[b, ..., 2] = 1.0
+            f[b, ..., 3] = 0.5
+        losses.append(net.train_step(f, t, np.ones_like(t), lr=1e-3))
+    assert losses[0] == pytest.approx(0.25, rel=0.1)
+    assert np.mean(losses[-10:]) < 0.5 * losses[0], losses
+    assert net.last_ms() > 0
+
+
+@pytest.mark.gpu
+def test_world_features_and_identity_apply():
+    """Device features from the world's tally equal the host features from
+    DoseScore; an untrained network (zero correction layer) returns the dose."""
+    import opendxmc_amd as dx
+    from opendxmc_amd import phantoms
+
+    ph = phantoms.ct_body((96, 64, 24), (0.4, 0.4, 0.4), seed=3)     # x padded 96 -> 128
+    world = dx.World()
+    grid = world.addItem(dx.AAVoxelGrid(correction=1))
+    grid.setData(ph.dims, ph.density, ph.material, [dx.Material.byWeight(m) for m in ph.materials])
+    grid.setSpacing(ph.spacing)
+    world.build()
+    beam = dx.CTSpiralBeam((0, 0, -4), (0, 0, 4))
+    beam.setCollimation(4.0)
+    beam.setStepAngleDeg(10)
+    beam.setTubeVoltage(120)
+    beam.setNumberOfParticlesPerExposure(4000)
+    dx.Transport(seed=5)(world, beam, None, True)
+
+    net = dn.Denoiser(seed=1)
+    fdev = net.world_features(grid)
+    assert fdev.shape == (24, 64, 128, 4)
+    assert np.all(fdev[:, :, 96:] == 0)
+    vol = dn.DoseVolume.from_grid(grid, ph.dims, ph.density, ph.material, dn.material_embedding(ph.materials))
+    fhost, dref = dn.features(vol)
+    fd = fdev[:, :, :96]
+    # device density is the packed FP32 word (2^-15 relative truncation)
+    np.testing.assert_allclose(fd[..., 1], fhost[..., 1], rtol=4e-5)
+    np.testing.assert_allclose(fd[..., 2], fhost[..., 2], rtol=1e-6)
+    np.testing.assert_allclose(fd[..., 0], fhost[..., 0], rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(fd[..., 3], fhost[..., 3], rtol=0, atol=1e-5)
+
+    dose = net.apply(grid)
+    d, _, _ = grid.doseScored()
+    np.testing.assert_allclose(dose, d, rtol=1e-4, atol=1e-7 * dref)
+    assert net.last_ms() > 0
