@@ -175,6 +175,7 @@ def evaluate(net: dn.Denoiser, ref_histories: float, plain_hist, dn_hist, seed: 
         ref_ms += run(world, grid, beam(h), seed=9000 + k)
         done += h
         k += 1
+        log(f"reference run {k}: {done:.2e} histories, {ref_ms / 1e3:.1f} s device")
     d_ref, v_ref, _ = grid.doseScored()
     # the reference's worker deletes air dose by default: score body voxels only
     d_ref = np.where(ph.material != 0, d_ref, 0.0)

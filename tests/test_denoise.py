@@ -210,18 +210,29 @@ def _net_case(seed, batch, dims):
     return f, target, mask
 
 
+def _rounding_noise(a_gpu, a_bf16, a_exact):
+    """GPU deviation from the BF16-rounded restatement, and the rounded
+    restatement's own deviation from the exact (FP64, unrounded) network.
+    Rounding flips compound over 18 layers, so two BF16 evaluations that sum
+    in different orders differ by about as much as either differs from the
+    exact result; a wrong tap, gate or bias would differ by far more."""
+    return float(np.mean(np.abs(a_gpu - a_bf16))), float(np.mean(np.abs(a_bf16 - a_exact)))
+
+
 @pytest.mark.gpu
 def test_network_prediction_matches_reference():
     params = ref.random_params(21, scale=0.8)
     net = dn.Denoiser(seed=1)
     net.set_params(params)
     f, _, _ = _net_case(22, 1, (4, 8, 64))
-    pred = net.infer(f[0])
-    want, _ = ref.forward(params, f)
-    want = want.numpy()[0]
-    err = np.abs(pred - want)
-    # identical BF16 rounding points; FP32 summation order differs
-    assert err.max() <= 2e-3 * max(1.0, np.abs(want).max()), err.max()
+    pred = net.infer(f[0]).astype(np.float64)
+    want = ref.forward(params, f)[0].numpy()[0].astype(np.float64)
+    exact = ref.forward(params, f.astype(np.float64), rounding=False, dtype=torch.float64)[0].numpy()[0]
+    dev, noise = _rounding_noise(pred, want, exact)
+    scale = np.abs(exact).max()
+    print(f"prediction: GPU vs BF16 restatement {dev:.2e}, restatement vs exact {noise:.2e}, max {scale:.2f}")
+    assert dev <= 2.0 * noise + 1e-6 * scale
+    assert np.abs(pred - exact).max() <= 0.03 * scale
 
 
 @pytest.mark.gpu
@@ -233,14 +244,19 @@ def test_network_loss_and_gradients_match_reference():
     loss = net.train_step(f, target, mask, lr=1e-6)
     grads = net.grads().astype(np.float64)
     rloss, _, rgrads = ref.loss_and_grads(params, f, target, mask)
-    assert loss == pytest.approx(rloss, rel=2e-4)
+    eloss, _, egrads = ref.loss_and_grads(params, f.astype(np.float64), target.astype(np.float64),
+                                          mask.astype(np.float64), rounding=False, dtype=torch.float64)
+    print(f"loss: GPU {loss:.6f}, BF16 restatement {rloss:.6f}, exact {eloss:.6f}")
+    assert abs(loss - rloss) <= 2.0 * abs(rloss - eloss) + 1e-4 * eloss
     for l in range(ref.LAYERS):
-        g = grads[l * ref.LAYER_P:(l + 1) * ref.LAYER_P]
-        r = rgrads[l * ref.LAYER_P:(l + 1) * ref.LAYER_P]
-        scale = np.abs(r).max()
+        sl = slice(l * ref.LAYER_P, (l + 1) * ref.LAYER_P)
+        g, r, e = grads[sl], rgrads[sl], egrads[sl]
+        scale = np.abs(e).max()
         assert scale > 0, l
-        err = np.abs(g - r).max() / scale
-        assert err < 2e-2, (l, err)
+        dev, noise = _rounding_noise(g, r, e)
+        print(f"layer {l:2d}: GPU vs BF16 restatement {dev / scale:.2e}, restatement vs exact {noise / scale:.2e}")
+        assert dev <= 2.0 * noise + 1e-5 * scale, l
+        assert np.abs(g - e).max() <= 0.05 * scale, l
         # the padded channels stay exactly zero
         if l == 0:
             assert np.all(g[:ref.LAYER_W].reshape(32, 27, 32)[:, :, 4:] == 0)
