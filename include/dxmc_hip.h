@@ -192,6 +192,26 @@ int dxg_debug_traverse(const dxg_world* w, uint64_t n_rays, const float* pos, co
  * `bytes` read and `bytes` written.  *ms receives the mean launch time. */
 int dxg_debug_device_copy(uint64_t bytes, int iters, double* ms);
 
+/* ---- Batched transport (no counterpart in the reference, whose Transport
+ * runs one world and beam per call, simulationpipeline.cpp:159-170): many
+ * (world, beam) instances of uneven size in ONE persistent launch.  Blocks
+ * start on instances in proportion to their histories and move to instances
+ * with histories left when theirs is exhausted.  The launch is captured into
+ * a HIP graph on the first run and replayed by later runs of the same batch.
+ * Each instance behaves exactly like dxg_transport_run(world, beam, NULL, 1,
+ * seed): same RNG streams (keyed by seed and the world's history count), same
+ * scores and counters, read through the world afterwards.  Worlds must be
+ * built, live on one device, share one correction level, hold one instance of
+ * the batch and outlive it.  No progress or cancellation inside a run. */
+typedef struct dxg_batch dxg_batch;
+int dxg_batch_create(dxg_batch** out);
+void dxg_batch_destroy(dxg_batch* b);
+/* The beam is copied. */
+int dxg_batch_add(dxg_batch* b, dxg_world* w, const dxg_beam* beam, uint64_t seed);
+uint64_t dxg_batch_size(const dxg_batch* b);
+/* One launch over every instance; *kernel_ms (may be null) receives its device time. */
+int dxg_batch_run(dxg_batch* b, double* kernel_ms);
+
 #ifdef __cplusplus
 }
 #endif

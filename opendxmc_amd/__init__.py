@@ -4,7 +4,9 @@ Host API mirrors the reference (World / AAVoxelGrid / Transport /
 TransportProgress / beams); compute runs in libdxmc_hip.so (HIP, gfx950).
 """
 from .engine import (AAVoxelGrid, BowtieFilter, CBCTBeam, CTAECFilter, CTSequentialBeam, CTSpiralBeam,
-                     CTSpiralDualEnergyBeam, DXBeam, Material, PencilBeam, Transport, TransportProgress, Tube, World)
+                     CTSpiralDualEnergyBeam, DXBeam, Material, PencilBeam, Transport, TransportBatch, TransportProgress,
+                     Tube, World)
 
 __all__ = ["AAVoxelGrid", "BowtieFilter", "CBCTBeam", "CTAECFilter", "CTSequentialBeam", "CTSpiralBeam",
-           "CTSpiralDualEnergyBeam", "DXBeam", "Material", "PencilBeam", "Transport", "TransportProgress", "Tube", "World"]
+           "CTSpiralDualEnergyBeam", "DXBeam", "Material", "PencilBeam", "Transport", "TransportBatch", "TransportProgress",
+           "Tube", "World"]

@@ -164,6 +164,11 @@ SIGNATURES = [
     ("dxg_debug_device_copy", C.c_int, [u64, C.c_int, PD]),
     ("dxg_debug_traverse", C.c_int, [C.c_void_p, u64, C.POINTER(C.c_float), C.POINTER(C.c_float), u64,
                                      C.POINTER(C.c_float), C.POINTER(i64), C.POINTER(C.c_int32)]),
+    ("dxg_batch_create", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("dxg_batch_destroy", None, [C.c_void_p]),
+    ("dxg_batch_add", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Beam), u64]),
+    ("dxg_batch_size", u64, [C.c_void_p]),
+    ("dxg_batch_run", C.c_int, [C.c_void_p, PD]),
     ("dxg_denoiser_num_params", u64, []),
     ("dxg_denoiser_create", C.c_int, [C.POINTER(C.c_void_p), u64]),
     ("dxg_denoiser_destroy", None, [C.c_void_p]),

@@ -33,6 +33,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
@@ -128,6 +129,21 @@ struct ConvArgs {
 // consecutive rows then touch 16 distinct bank groups.
 __device__ inline int wSlot(int tap, int co, int q) { return (tap * 32 + co) * 4 + (q ^ ((co >> 2) & 3)); }
 
+// DPP row moves (16-lane rows) on the four dwords of an operand fragment;
+// lanes whose source falls outside their row keep `old`.
+constexpr int kRowShl1 = 0x101, kRowShl15 = 0x10F, kRowShr1 = 0x111, kRowShr15 = 0x11F;
+
+template <int CTRL>
+__device__ inline u32x4 dpp4(u32x4 old, u32x4 src)
+{
+    u32x4 r;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+        r[e] = static_cast<unsigned>(__builtin_amdgcn_update_dpp(static_cast<int>(old[e]), static_cast<int>(src[e]), CTRL, 0xF, 0xF, false));
+    return r;
+}
+
+template <bool SHIFT>
 __global__ __launch_bounds__(256) void conv3d_kernel(ConvArgs a)
 {
     __shared__ u32x4 wl[kTaps * 32 * 4];   // 55,296 B
@@ -179,6 +195,49 @@ __global__ __launch_bounds__(256) void conv3d_kernel(ConvArgs a)
                     continue;
                 const size_t rowb = (static_cast<size_t>(n * a.nz + zz) * a.ny + yy) * a.nx;
                 const int tapb = (dz + 1) * 9 + (dy + 1) * 3;
+                if (SHIFT) {
+                    // One load per voxel tile; the dx = -1 / +1 operands are the
+                    // same row moved by one lane inside each 16-lane DPP row,
+                    // with the voxel that crosses a tile edge taken from the
+                    // neighbouring tile (or the halo voxel at x0 - 1 / x0 + 64).
+                    u32x4 c[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        c[j] = load16(a.x + (rowb + x0 + 16 * j + lr) * kC + 8 * lq);
+                    const u32x4 zero = { 0, 0, 0, 0 };
+                    u32x4 left = zero, right = zero;
+                    if (lr == 15 && x0 > 0)
+                        left = load16(a.x + (rowb + x0 - 1) * kC + 8 * lq);
+                    if (lr == 0 && x0 + 64 < a.nx)
+                        right = load16(a.x + (rowb + x0 + 64) * kC + 8 * lq);
+#pragma unroll
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        bf16x8 bop[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            if (dx == 0) {
+                                bop[j] = asOperand(c[j]);
+                            } else if (dx < 0) {
+                                const u32x4 edge = dpp4<kRowShl15>(zero, j == 0 ? left : c[j - 1]);   // lane 0 <- lane 15
+                                bop[j] = asOperand(dpp4<kRowShr1>(edge, c[j]));                      // lane i <- lane i-1
+                            } else {
+                                const u32x4 edge = dpp4<kRowShr15>(zero, j == 3 ? right : c[j + 1]);  // lane 15 <- lane 0
+                                bop[j] = asOperand(dpp4<kRowShl1>(edge, c[j]));                      // lane i <- lane i+1
+                            }
+                        }
+                        const int tap = tapb + dx + 1;
+                        bf16x8 aop[2];
+#pragma unroll
+                        for (int m = 0; m < 2; ++m)
+                            aop[m] = asOperand(wl[wSlot(tap, 16 * m + lr, lq)]);
+#pragma unroll
+                        for (int m = 0; m < 2; ++m)
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop[m], bop[j], acc[m][j], 0, 0, 0);
+                    }
+                    continue;
+                }
 #pragma unroll
                 for (int dx = -1; dx <= 1; ++dx) {
                     bf16x8 bop[4];
@@ -604,6 +663,17 @@ int computeUnits()
 
 int roundUp8(int v) { return (v + 7) & ~7; }
 
+// Activation operands for the x-neighbour taps: moved across lanes (default)
+// or loaded again per tap (DXG_CONV_SHIFT=0).
+bool convShift()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("DXG_CONV_SHIFT");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 struct Dims {
     int nb, nz, ny, nx;
     size_t voxels() const { return static_cast<size_t>(nb) * nz * ny * nx; }
@@ -634,7 +704,10 @@ hipError_t launchConv(ConvArgs a, const Dims& d, hipStream_t s)
     const int wg_tiles = (a.wave_tiles + 3) / 4;
     const int grid = roundUp8(std::min(wg_tiles, 2 * computeUnits()));
     a.chunk = (wg_tiles + grid - 1) / grid;
-    hipLaunchKernelGGL(conv3d_kernel, dim3(grid), dim3(256), 0, s, a);
+    if (convShift())
+        hipLaunchKernelGGL(conv3d_kernel<true>, dim3(grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(conv3d_kernel<false>, dim3(grid), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

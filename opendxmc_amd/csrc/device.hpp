@@ -115,6 +115,11 @@ int transportBlockSize();
 void launchCopy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 // Resident blocks per CU of the transport kernel at the given LDS size.
 int transportBlocksPerCU(bool lds_lut, int level, size_t lds_bytes);
+// Many instances in one persistent launch (pool and counters of every
+// instance reset first); params and start_inst are device arrays.
+void launchTransportBatch(const DevParams* params, int n_inst, const int* start_inst, bool lds_lut, int level, size_t lds,
+                          int blocks, hipStream_t s);
+int transportBatchBlocksPerCU(bool lds_lut, int level, size_t lds_bytes);
 
 // Device-resident state of a built world, for kernels outside the transport
 // (the dose denoiser reads the tally and the voxel words in place).
@@ -128,5 +133,13 @@ struct WorldView {
     std::vector<float> embedding;   // per material: total mass attenuation at 60 keV over water's
 };
 int worldView(const dxg_world* w, WorldView& out);
+
+// Run preparation shared by dxg_transport_run and batched runs: validates the
+// beam, uploads its tables to the world and fills p (all but the history
+// range); total receives the beam's history count.
+int prepareRun(dxg_world* w, const dxg_beam* beam, uint64_t seed, DevParams& p, uint64_t& total);
+// After a run outside dxg_transport_run: account n histories and set the counters.
+void addHistories(dxg_world* w, uint64_t n, const dxg_counters& c);
+int worldDevice(const dxg_world* w);
 
 }  // namespace dxg

@@ -190,10 +190,11 @@ __device__ inline void energyBin(float E, const float* s_maj, int& ej, float& et
 
 }  // namespace
 
-template <bool LDS_LUT, int LEVEL>
-__global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
+// Stages one instance's tables into LDS: the majorant, the attenuation table
+// when LDS_LUT, and the beam blob.  Returns the attenuation table to read.
+template <bool LDS_LUT>
+__device__ inline const float* stageTables(const DevParams& P, uint32_t* lds)
 {
-    extern __shared__ uint32_t lds[];
     float* s_maj = reinterpret_cast<float*>(lds);
     float* s_mu = s_maj + kLutNd;
     uint32_t* s_blob = lds + kLutNd + (LDS_LUT ? P.n_mat * kLutNd : 0);
@@ -204,18 +205,17 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
             s_mu[i] = P.mu_tot[i];
     for (int i = threadIdx.x; i < P.bb.words; i += blockDim.x)
         s_blob[i] = P.blob[i];
-    __syncthreads();
-    const float* mu_tab = LDS_LUT ? s_mu : P.mu_tot;
+    return LDS_LUT ? s_mu : P.mu_tot;
+}
 
-    // per-wavefront FP64 energy sums (emitted, escaped, deposited) in LDS
-    __shared__ double s_energy[kBlock / 64][3];
+// The history loop of one wavefront for one instance: claims batches of
+// [P.h_begin, P.h_end) from P.pool until none is left and every lane is done,
+// then adds the wavefront's counters and energy sums to P.counters.
+template <bool LDS_LUT, int LEVEL>
+__device__ __forceinline__ void runHistories(const DevParams& P, const float* s_maj, const float* mu_tab, const uint32_t* s_blob,
+                                             double* e_wave)
+{
     const int lane = static_cast<int>(__lane_id());
-    const int wave = static_cast<int>(threadIdx.x) >> 6;
-    if (lane < 3)
-        s_energy[wave][lane] = 0.0;
-    __syncthreads();
-    double* const e_wave = s_energy[wave];
-
     float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 1, E = 0, w = 0;
     float maj = 1, et = 0;
     int ej = 0, vi = 0;
@@ -550,6 +550,84 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
     }
 }
 
+template <bool LDS_LUT, int LEVEL>
+__global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
+{
+    extern __shared__ uint32_t lds[];
+    const float* mu_tab = stageTables<LDS_LUT>(P, lds);
+    // per-wavefront FP64 energy sums (emitted, escaped, deposited) in LDS
+    __shared__ double s_energy[kBlock / 64][3];
+    const int lane = static_cast<int>(__lane_id());
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    if (lane < 3)
+        s_energy[wave][lane] = 0.0;
+    __syncthreads();
+    const float* s_maj = reinterpret_cast<const float*>(lds);
+    const uint32_t* s_blob = lds + kLutNd + (LDS_LUT ? P.n_mat * kLutNd : 0);
+    runHistories<LDS_LUT, LEVEL>(P, s_maj, mu_tab, s_blob, s_energy[wave]);
+}
+
+// Many instances (world + beam) in one persistent launch.  A block starts on
+// the instance start_inst[blockIdx.x]; when that instance's pool is empty and
+// all of the block's wavefronts are done, the block moves to an instance with
+// histories left (scanning from a block-dependent offset), re-stages its LDS
+// tables and continues, until no instance has work.
+template <bool LDS_LUT, int LEVEL>
+__global__ void __launch_bounds__(kBlock) transport_batch_kernel(const DevParams* __restrict__ params, int n_inst,
+                                                                 const int* __restrict__ start_inst)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ double s_energy[kBlock / 64][3];
+    __shared__ int s_next;
+    const int lane = static_cast<int>(__lane_id());
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    int inst = start_inst[blockIdx.x];
+    for (;;) {
+        inst = __builtin_amdgcn_readfirstlane(inst);
+        const DevParams& P = params[inst];
+        const float* mu_tab = stageTables<LDS_LUT>(P, lds);
+        if (lane < 3)
+            s_energy[wave][lane] = 0.0;
+        __syncthreads();
+        const float* s_maj = reinterpret_cast<const float*>(lds);
+        const uint32_t* s_blob = lds + kLutNd + (LDS_LUT ? P.n_mat * kLutNd : 0);
+        runHistories<LDS_LUT, LEVEL>(P, s_maj, mu_tab, s_blob, s_energy[wave]);
+        __syncthreads();   // every wavefront is done with this instance's tables
+        if (threadIdx.x == 0) {
+            int next = -1;
+            for (int k = 0; k < n_inst && next < 0; ++k) {
+                const int c = (static_cast<int>(blockIdx.x) + k) % n_inst;
+                const DevParams& Q = params[c];
+                const unsigned long long claimed = __hip_atomic_load(Q.pool, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (claimed < Q.h_end - Q.h_begin)
+                    next = c;
+            }
+            s_next = next;
+        }
+        __syncthreads();
+        inst = s_next;
+        if (inst < 0)
+            break;
+    }
+}
+
+__global__ void batch_reset_kernel(const DevParams* __restrict__ params, int n_inst)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n_inst)
+        return;
+    *params[i].pool = 0;
+    DevCounters* c = params[i].counters;
+    c->steps = 0;
+    c->real_events = 0;
+    c->deposits = 0;
+    c->histories = 0;
+    c->emitted = 0;
+    c->escaped = 0;
+    c->deposited = 0;
+    c->uncollided = 0;
+}
+
 __global__ void traverse_kernel(const DevParams P, uint64_t n_rays, const float* __restrict__ pos, const float* __restrict__ dir,
                                 uint64_t n_steps, const float* __restrict__ steps, int64_t* __restrict__ out_index,
                                 int32_t* __restrict__ out_material)
@@ -627,6 +705,36 @@ void launchTransport(const DevParams& p, int blocks, int threads, hipStream_t s)
     void* args[] = { const_cast<DevParams*>(&p) };
     (void)hipLaunchKernel(transportInstance(p.lds_lut != 0, p.level), dim3(static_cast<unsigned>(blocks)),
                           dim3(static_cast<unsigned>(threads)), args, lds, s);
+}
+
+namespace {
+
+const void* batchInstance(bool lds_lut, int level)
+{
+    using K = void (*)(const DevParams*, int, const int*);
+    static const K table[2][3] = {
+        { &transport_batch_kernel<false, 0>, &transport_batch_kernel<false, 1>, &transport_batch_kernel<false, 2> },
+        { &transport_batch_kernel<true, 0>, &transport_batch_kernel<true, 1>, &transport_batch_kernel<true, 2> },
+    };
+    return reinterpret_cast<const void*>(table[lds_lut ? 1 : 0][level < 0 ? 0 : (level > 2 ? 2 : level)]);
+}
+
+}  // namespace
+
+void launchTransportBatch(const DevParams* params, int n_inst, const int* start_inst, bool lds_lut, int level, size_t lds,
+                          int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(static_cast<unsigned>((n_inst + 63) / 64)), dim3(64), 0, s, params, n_inst);
+    void* args[] = { const_cast<DevParams**>(&params), &n_inst, const_cast<int**>(&start_inst) };
+    (void)hipLaunchKernel(batchInstance(lds_lut, level), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), args, lds, s);
+}
+
+int transportBatchBlocksPerCU(bool lds_lut, int level, size_t lds_bytes)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batchInstance(lds_lut, level), kBlock, lds_bytes) != hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
 }
 
 int transportBlocksPerCU(bool lds_lut, int level, size_t lds_bytes)

@@ -367,9 +367,15 @@ int dxg_world_clear_scores(dxg_world* w)
     return DXG_OK;
 }
 
-int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress, int apply_calibration, uint64_t seed)
+}  // extern "C"
+
+namespace dxg {
+
+// Validates a beam, uploads its exposure and spectrum/bowtie tables to the
+// world's device buffers and fills the launch parameters (everything but the
+// history range).  total receives the beam's history count.
+int prepareRun(dxg_world* w, const dxg_beam* beam, uint64_t seed, DevParams& p, uint64_t& total)
 {
-    (void)apply_calibration;
     if (!w || !beam)
         return fail(DXG_ERR_ARGUMENT, "dxg_transport_run: null argument");
     if (!w->built)
@@ -443,7 +449,7 @@ int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress
     // ---- exposures
     std::vector<dxg::DevExposure> ex(beam->n_exposures);
     std::vector<unsigned long long> ex_start(beam->n_exposures);
-    uint64_t total = 0;
+    total = 0;
     for (uint64_t i = 0; i < beam->n_exposures; ++i) {
         const dxg_exposure& e = beam->exposures[i];
         if (e.spectrum_index >= beam->n_spectra)
@@ -519,7 +525,6 @@ int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress
     }
     HIP_TRY(w->d_blob.upload(blob, w->stream));
 
-    dxg::DevParams p;
     fillParams(w, p);
     p.exposures = w->d_ex.ptr;
     p.ex_start = w->d_ex_start.ptr;
@@ -548,9 +553,31 @@ int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress
     p.lds_lut = 1;
     if (dxg::transportLdsBytes(p) > kLdsLutLimit)
         p.lds_lut = 0;
-    const size_t lds_bytes = dxg::transportLdsBytes(p);
-    if (lds_bytes > 64 * 1024)
+    if (dxg::transportLdsBytes(p) > 64 * 1024)
         return fail(DXG_ERR_ARGUMENT, "beam tables (spectra + bowties) exceed the LDS budget");
+    return DXG_OK;
+}
+
+void addHistories(dxg_world* w, uint64_t n, const dxg_counters& c)
+{
+    w->histories += n;
+    w->counters = c;
+}
+
+int worldDevice(const dxg_world* w) { return w->device; }
+
+}  // namespace dxg
+
+extern "C" {
+
+int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress, int apply_calibration, uint64_t seed)
+{
+    (void)apply_calibration;
+    dxg::DevParams p;
+    uint64_t total = 0;
+    if (int rc = dxg::prepareRun(w, beam, seed, p, total); rc != DXG_OK)
+        return rc;
+    const size_t lds_bytes = dxg::transportLdsBytes(p);
 
     // ---- launch geometry: fill every CU to the occupancy limit
     int cus = 0;

@@ -94,14 +94,15 @@ def generate(net: dn.Denoiser, n_phantoms: int, dims, hpv_target: float, hpv_inp
         dual = bool(rng.random() < 0.5)
         nv = ph.size
         run(world, grid, spiral_beam(z_half, pitch, kv, dual, hpv_target * nv), seed=10 * i + 1)
-        d_hi = grid.doseScored()[0].reshape(nz, ny, nx)
+        # doses are totals over a run: compare them per history
+        d_hi = grid.doseScored()[0].reshape(nz, ny, nx) / grid.histories()
         mat = ph.material.reshape(nz, ny, nx)
         mask = (mat != 0).astype(np.float32)
         for j, hpv in enumerate(hpv_inputs):
             grid.clearScores()
             run(world, grid, spiral_beam(z_half, pitch, kv, dual, hpv * nv), seed=10 * i + 2 + j)
             f = net.world_features(grid)[:, :, :nx]
-            d_lo = grid.doseScored()[0].reshape(nz, ny, nx)
+            d_lo = grid.doseScored()[0].reshape(nz, ny, nx) / grid.histories()
             ref = dn.dose_reference(d_lo, mat)
             t = (d_hi / ref).astype(np.float32)
             for _ in range(patches):
@@ -176,34 +177,53 @@ def evaluate(net: dn.Denoiser, ref_histories: float, plain_hist, dn_hist, seed: 
         done += h
         k += 1
         log(f"reference run {k}: {done:.2e} histories, {ref_ms / 1e3:.1f} s device")
+    h_ref = grid.histories()
     d_ref, v_ref, _ = grid.doseScored()
-    # the reference's worker deletes air dose by default: score body voxels only
-    d_ref = np.where(ph.material != 0, d_ref, 0.0)
+    # the reference's worker deletes air dose by default: score body voxels only;
+    # doses are totals over a run, so every comparison is per history
+    d_ref = np.where(ph.material != 0, d_ref, 0.0) / h_ref
+    v_ref = v_ref / float(h_ref) ** 2
     sel = d_ref > 0.2 * d_ref.max()
-    ref_rel_se = float(np.sqrt(np.mean(v_ref[sel] / d_ref[sel] ** 2)))
-    log(f"reference: {grid.histories():.3e} histories, {ref_ms / 1e3:.1f} s, rms relative SE {ref_rel_se:.4f}, "
+    r = d_ref[sel]
+    ref_rel_var = v_ref[sel] / r ** 2
+    ref_rel_se = float(np.sqrt(np.mean(ref_rel_var)))
+    log(f"reference: {h_ref:.3e} histories, {ref_ms / 1e3:.1f} s, rms relative SE {ref_rel_se:.4f}, "
         f"{int(sel.sum())} voxels above 20 %")
+    del v_ref
 
-    def err(d):
-        return float(np.mean(np.abs(d[sel] - d_ref[sel]) / d_ref[sel]))
+    def err(d, h):
+        """(mean relative error, RMS relative error with the reference's own
+        variance removed) of a run of h histories."""
+        x = d[sel] / h
+        rel = (x - r) / r
+        mre = float(np.mean(np.abs(rel)))
+        rms = float(np.sqrt(max(float(np.mean(rel * rel)) - float(np.mean(ref_rel_var)), 0.0)))
+        return mre, rms
 
     plain = []
     for i, h in enumerate(plain_hist):
         grid.clearScores()
         ms = run(world, grid, beam(h), seed=100 + i)
-        e = err(grid.doseScored()[0])
-        plain.append({"histories": grid.histories(), "mc_ms": ms, "error": e})
-        log(f"plain {grid.histories():.2e}: {ms:.1f} ms, error {e:.4f}")
+        hh = grid.histories()
+        d, v, _ = grid.doseScored()
+        e, e_rms = err(d, hh)
+        # the run's own variance estimate: Gaussian noise gives MRE = sqrt(2/pi) x relative SE
+        own = float(np.sqrt(2.0 / np.pi) * np.mean(np.sqrt(np.maximum(v[sel], 0.0)) / np.maximum(d[sel], 1e-300)))
+        plain.append({"histories": hh, "mc_ms": ms, "error": e, "error_rms_corrected": e_rms, "error_from_own_variance": own})
+        log(f"plain {hh:.2e}: {ms:.1f} ms, error {e:.4f} (rms corrected {e_rms:.4f}, own-variance {own:.4f})")
     denoised = []
     for i, h in enumerate(dn_hist):
         grid.clearScores()
         ms = run(world, grid, beam(h), seed=200 + i)
-        e_in = err(grid.doseScored()[0])
+        hh = grid.histories()
+        e_in, _ = err(grid.doseScored()[0], hh)
         d = net.apply(grid)
         ams = net.last_ms()
-        e = err(d)
-        denoised.append({"histories": grid.histories(), "mc_ms": ms, "denoise_ms": ams, "error": e, "input_error": e_in})
-        log(f"denoised {grid.histories():.2e}: MC {ms:.1f} ms + denoise {ams:.1f} ms, error {e:.4f} (input {e_in:.4f})")
+        e, e_rms = err(d, hh)
+        denoised.append({"histories": hh, "mc_ms": ms, "denoise_ms": ams, "error": e, "error_rms_corrected": e_rms,
+                         "input_error": e_in})
+        log(f"denoised {hh:.2e}: MC {ms:.1f} ms + denoise {ams:.1f} ms, error {e:.4f} (rms corrected {e_rms:.4f}, "
+            f"input {e_in:.4f})")
 
     rate = sum(p["histories"] for p in plain) / (sum(p["mc_ms"] for p in plain) * 1e-3)
     # plain MC: error ~ a / sqrt(H), fitted on the measured points
@@ -217,7 +237,7 @@ def evaluate(net: dn.Denoiser, ref_histories: float, plain_hist, dn_hist, seed: 
         t_dn = h_dn / rate + dn_ms * 1e-3
     return {
         "workload": "CT spiral 512x512x400 at 0.08 cm, seeded body, 120 kVp dual source 2x40 mm, pitch 3.2",
-        "reference_histories": done,
+        "reference_histories": float(h_ref),
         "reference_rms_relative_se": ref_rel_se,
         "voxels_above_20pct": int(sel.sum()),
         "histories_per_s": rate,

@@ -522,3 +522,36 @@ class Transport:
             return False
         N.check(rc)
         return True
+
+
+class TransportBatch:
+    """Many (world, beam) instances in one persistent device launch, replayed
+    from a captured HIP graph (no counterpart in the reference).  Each
+    instance's scores are read from its grid afterwards."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        N.check(N.lib().dxg_batch_create(C.byref(h)))
+        self._h = h
+        self._worlds = []
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            N.lib().dxg_batch_destroy(self._h)
+            self._h = None
+
+    def add(self, world: World, beam, seed: int = 1) -> None:
+        k = _Keep()
+        b = beam_struct(beam, k)
+        N.check(N.lib().dxg_batch_add(self._h, world.items[0]._h, C.byref(b), int(seed)))
+        self._worlds.append(world)      # the worlds must outlive the batch
+
+    def __len__(self) -> int:
+        return int(N.lib().dxg_batch_size(self._h))
+
+    def run(self) -> float:
+        """One launch over every instance; returns its device time in ms."""
+        ms = C.c_double()
+        N.check(N.lib().dxg_batch_run(self._h, C.byref(ms)))
+        return ms.value
+
