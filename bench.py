@@ -11,10 +11,12 @@ The grid, tables and tallies are resident in device memory before the timed
 region (World::build() runs once, outside it).
 
 Roofline: algorithmic bytes per history = 5 S + 24 D, with S the Woodcock
-steps (1 B material + 4 B density gathered) and D the scored deposits (24 B
-of FP64/u64 tally state), both counted on the device; achieved = those bytes
-over the transport kernels' own duration (HIP events on the stream they run
-on), against 8 TB/s HBM.
+steps that read a voxel (1 B material + 4 B density gathered) and D the scored
+deposits (24 B of FP64/u64 tally state), both counted on the device; achieved
+= those bytes over the transport kernels' own duration (HIP events on the
+stream they run on), against 8 TB/s HBM.  Macro-cell boundary crossings (one
+read of a small cache-resident majorant array, no voxel read) are reported
+beside them.
 
 CPU baseline: the oracle (FP64 C++ restatement, std::thread) on a bounded
 sample of the same workload, on this host's cores.
@@ -132,7 +134,7 @@ def main() -> int:
         transport.seed = 1000 + i
         transport(world, beam, None, True)
         print(f"warmup {i} done", file=sys.stderr, flush=True)
-    steps = bytes_alg = kernel_ms = launches = hist = deposits = 0
+    steps = bytes_alg = kernel_ms = launches = hist = deposits = crossings = 0
     t0 = time.perf_counter()
     for i in range(args.steps):
         transport.seed = i + 1
@@ -140,6 +142,7 @@ def main() -> int:
         c = grid.counters()
         steps += c["steps"]
         deposits += c["deposits"]
+        crossings += c["cell_crossings"]
         kernel_ms += c["kernel_ms"]
         launches += c["kernel_launches"]
         hist += c["histories"]
@@ -172,6 +175,7 @@ def main() -> int:
             "world_build_s": build_s,
             "steps_per_history": steps / max(hist, 1),
             "deposits_per_history": deposits / max(hist, 1),
+            "cell_crossings_per_history": crossings / max(hist, 1),
             "kernel_launches_per_step": launches / max(args.steps, 1),
         },
         "roofline": {

@@ -103,6 +103,7 @@ typedef struct dxg_counters {
     double kernel_ms;          /* summed duration of the transport kernels */
     uint64_t kernel_launches;
     uint64_t uncollided;       /* histories leaving the grid with no real interaction */
+    uint64_t cell_crossings;   /* macro-cell boundary crossings (majorant changes, no voxel read) */
 } dxg_counters;
 
 const char* dxg_last_error(void);

@@ -65,7 +65,7 @@ class Counters(C.Structure):
         ("histories", u64), ("steps", u64), ("real_events", u64), ("deposits", u64),
         ("emitted_energy", C.c_double), ("escaped_energy", C.c_double),
         ("deposited_energy", C.c_double), ("kernel_ms", C.c_double), ("kernel_launches", u64),
-        ("uncollided", u64),
+        ("uncollided", u64), ("cell_crossings", u64),
     ]
 
     def as_dict(self) -> dict:

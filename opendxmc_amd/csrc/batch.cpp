@@ -290,6 +290,7 @@ int dxg_batch_run(dxg_batch* b, double* kernel_ms)
         acc.escaped_energy = c.escaped;
         acc.deposited_energy = c.deposited;
         acc.uncollided = c.uncollided;
+        acc.cell_crossings = c.crossings;
         acc.kernel_ms = ms;          // the shared launch
         acc.kernel_launches = 1;
         dxg::addHistories(b->inst[static_cast<size_t>(i)].world, b->inst[static_cast<size_t>(i)].histories, acc);

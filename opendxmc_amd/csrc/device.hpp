@@ -33,6 +33,7 @@ struct DevCounters {
     double escaped;
     double deposited;
     unsigned long long uncollided;   // histories that left the grid with no real interaction
+    unsigned long long crossings;    // macro-cell boundary crossings (no voxel read)
 };
 
 // Packed voxel word: the FP32 density with its 8 low mantissa bits replaced
@@ -81,6 +82,13 @@ struct DevParams {
     const float* ia_inv;
     const float* fl_cum;
     const float* fl_line;
+
+    // macro-cell majorants (null: the global majorant alone); cell c of
+    // cell_len cm per axis bounds mu * rho by cell_ratio[c] * majorant_j
+    const float* cell_ratio;
+    int cnx, cny, cnz;
+    float cell_len[3];
+    float cell_inv[3];
 
     // tallies: per voxel {sum w dE, sum (w dE)^2, events (u64 bits), unused}
     double* tally;

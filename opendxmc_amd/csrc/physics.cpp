@@ -256,6 +256,14 @@ void buildTables(const PhysicsList& pl, const std::vector<MixMaterial>& mats, co
         // a vacuum grid still needs a positive majorant: photons then cross it in one step
         t.majorant[static_cast<size_t>(j)] = roundUp(std::max(mx * (1.0 + 1e-6), 1e-20));
     }
+    t.region_gain.assign(static_cast<size_t>(nm), 0.0);
+    for (int m = 0; m < nm; ++m)
+        for (int j = 0; j < kLutN; ++j) {
+            const size_t o = static_cast<size_t>(m) * kLutN + j;
+            const double hi = j + 1 < kLutN ? std::max(tot[o], tot[o + 1]) : tot[o];
+            t.region_gain[static_cast<size_t>(m)] =
+                std::max(t.region_gain[static_cast<size_t>(m)], hi / static_cast<double>(t.majorant[static_cast<size_t>(j)]));
+        }
     // Sampling tables for correction level 1.
     constexpr int kFine = 16384;
     t.s_table.assign(static_cast<size_t>(nm) * kSfN, 0.f);
@@ -437,6 +445,37 @@ std::string defaultPhysicsListPath()
         return dir + "/physics/physics_list.bin";
     }
     return "physics/physics_list.bin";
+}
+
+std::vector<float> cellRatios(const DeviceTables& t, const uint64_t dims[3], const std::vector<double>& density,
+                              const std::vector<uint8_t>& material, int cell, int cn[3])
+{
+    for (int a = 0; a < 3; ++a)
+        cn[a] = static_cast<int>((dims[a] + static_cast<uint64_t>(cell) - 1) / static_cast<uint64_t>(cell));
+    const size_t ncell = static_cast<size_t>(cn[0]) * cn[1] * cn[2];
+    const int nm = t.n_mat;
+    // densest voxel of every material in every cell
+    std::vector<double> rho(ncell * static_cast<size_t>(nm), 0.0);
+    for (uint64_t z = 0; z < dims[2]; ++z)
+        for (uint64_t y = 0; y < dims[1]; ++y) {
+            const size_t row = (static_cast<size_t>(z / cell) * cn[1] + y / cell) * cn[0];
+            const size_t v0 = (z * dims[1] + y) * dims[0];
+            for (uint64_t x = 0; x < dims[0]; ++x) {
+                const size_t c = row + x / cell;
+                double& r = rho[c * nm + material[v0 + x]];
+                r = std::max(r, density[v0 + x]);
+            }
+        }
+    // max over j and m commute: the ratio is max_m g_m rho_m(c); padded so
+    // that the FP32 product ratio * majorant still bounds the FP32 mu * rho
+    std::vector<float> ratio(ncell, 0.0f);
+    for (size_t c = 0; c < ncell; ++c) {
+        double r = 0.0;
+        for (int m = 0; m < nm; ++m)
+            r = std::max(r, t.region_gain[static_cast<size_t>(m)] * rho[c * nm + m]);
+        ratio[c] = r > 0.0 ? roundUp(r * (1.0 + 1e-5)) : 0.0f;
+    }
+    return ratio;
 }
 
 }  // namespace dxg

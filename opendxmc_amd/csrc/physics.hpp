@@ -91,7 +91,17 @@ struct DeviceTables {
     std::vector<float> ia_inv;     // [k] x(u_k) of the unit Compton profile (8/3pi)(1+x^2)^-3, u_k = k/(kIaInvN-1)
     std::vector<float> fl_cum;     // [m][j][kFlMax]: cumulative P(K-shell absorption in fluorescing element) at E_j
     std::vector<float> fl_line;    // [m][kFlMax][8]: omega_K, E_Ka1, E_Ka2, E_Kb, P(Ka1), P(Ka1)+P(Ka2), 0, 0
+    // host only: g_m = max over bins j of binmax_m(j) / majorant_j, so that for
+    // a region whose material m reaches density rho_m, max_m g_m rho_m times
+    // majorant_j bounds mu * rho of every voxel of the region over bin j
+    std::vector<double> region_gain;
 };
+
+// Macro-cell majorants: the grid is tiled by cells of `cell` voxels per axis;
+// ratio[c] * majorant_j bounds mu * rho over bin j for every voxel of cell c
+// (x fastest: c = cx + cnx (cy + cny cz)).  density/material in grid order.
+std::vector<float> cellRatios(const DeviceTables& t, const uint64_t dims[3], const std::vector<double>& density,
+                              const std::vector<uint8_t>& material, int cell, int cn[3]);
 
 // Impulse approximation: the shells of a material (Material<5>: five).
 constexpr int kIaShells = 5;

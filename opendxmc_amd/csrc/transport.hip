@@ -179,6 +179,28 @@ __device__ inline float lerpTab(const float* tab, int j, float t)
 
 __device__ inline float asFloat(uint32_t u) { return __uint_as_float(u); }
 
+// Macro-cell coordinate of position p on one axis; a photon on a cell face
+// moving backwards belongs to the cell behind the face.
+__device__ inline int cellCoord(float p, float bmin, float inv, int n, float d)
+{
+    const float f = (p - bmin) * inv;
+    const float fl = floorf(f);
+    int c = static_cast<int>(fl);
+    if (d < 0.0f && f == fl)
+        c -= 1;
+    return min(max(c, 0), n - 1);
+}
+
+// Path length to the exit face of cell c along one axis (+inf when the
+// direction is parallel to the faces; never negative).
+__device__ inline float exitDistance(float p, float d, float bmin, float len, int c)
+{
+    if (d == 0.0f)
+        return __builtin_inff();
+    const float face = bmin + static_cast<float>(c + (d > 0.0f ? 1 : 0)) * len;
+    return fmaxf(__fdividef(face - p, d), 0.0f);
+}
+
 // Energy bin of the attenuation tables and the bin's majorant.
 __device__ inline void energyBin(float E, const float* s_maj, int& ej, float& et, float& maj)
 {
@@ -220,11 +242,13 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
     float maj = 1, et = 0;
     int ej = 0, vi = 0;
     int state = kDead;
+    float tau = -1.0f;   // optical depth left in the current flight (macro cells; < 0: draw a new one)
+    int cellc = 0;       // macro cell of the photon: cx | cy << 10 | cz << 20
     Rng rng { 1, 2, 3, 4 };
     bool collided = false;
 
     // wavefront-uniform event counters (popcounts of ballots, kept in SGPRs)
-    unsigned long long n_steps = 0, n_real = 0, n_dep = 0, n_hist = 0, n_unc = 0;
+    unsigned long long n_steps = 0, n_real = 0, n_dep = 0, n_hist = 0, n_unc = 0, n_cross = 0;
     // wavefront-private batch of pool histories [cur, end), claimed P.batch at a time
     unsigned long long cur = 0, end = 0;
 
@@ -334,6 +358,11 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         et = f - static_cast<float>(ej);
                         maj = s_maj[ej];
                         state = kTrack;
+                        tau = -1.0f;
+                        if (P.cell_ratio)
+                            cellc = cellCoord(px, P.bmin[0], P.cell_inv[0], P.cnx, dx)
+                                    | (cellCoord(py, P.bmin[1], P.cell_inv[1], P.cny, dy) << 10)
+                                    | (cellCoord(pz, P.bmin[2], P.cell_inv[2], P.cnz, dz) << 20);
                     } else {
                         atomicAdd(&e_wave[1], we);
                     }
@@ -350,6 +379,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
             if (state == kInteract) {
                 collided = true;
                 state = kTrack;
+                tau = -1.0f;   // the photon stays in its macro cell; its next flight is drawn anew
                 const int m = static_cast<int>(P.voxels[vi] & 0xFFu);
                 const float4 a = P.mu_parts[m * kLutNd + ej];
                 const float4 b = P.mu_parts[m * kLutNd + ej + 1];
@@ -513,6 +543,69 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
 
         // ------------------------------------------------------------ step
         bool uncollided_exit = false;
+        if (P.cell_ratio) {
+            // Woodcock tracking with macro-cell majorants: a flight carries an
+            // optical depth tau through cells of majorant ratio[c] * maj; each
+            // iteration either crosses into the next cell (tau -= mu_c * d)
+            // or ends the flight inside the cell at a tentative collision,
+            // accepted as real with probability mu(voxel) / mu_c.
+            bool tentative = false, crossed = false;
+            if (state == kTrack) {
+                if (tau < 0.0f)
+                    tau = -__logf(rngU(rng));
+                int cx = cellc & 1023, cy = (cellc >> 10) & 1023, cz = cellc >> 20;
+                const float mu_c = P.cell_ratio[cx + P.cnx * (cy + P.cny * cz)] * maj;
+                const float tx = exitDistance(px, dx, P.bmin[0], P.cell_len[0], cx);
+                const float ty = exitDistance(py, dy, P.bmin[1], P.cell_len[1], cy);
+                const float tz = exitDistance(pz, dz, P.bmin[2], P.cell_len[2], cz);
+                const float t_exit = fminf(tx, fminf(ty, tz));
+                if (tau < mu_c * t_exit) {
+                    tentative = true;
+                    const float s = __fdividef(tau, mu_c);
+                    tau = -1.0f;
+                    px = __fmaf_rn(s, dx, px);
+                    py = __fmaf_rn(s, dy, py);
+                    pz = __fmaf_rn(s, dz, pz);
+                    vi = voxelOf(P, px, py, pz);
+                    if (vi < 0) {
+                        atomicAdd(&e_wave[1], static_cast<double>(w) * static_cast<double>(E));
+                        uncollided_exit = !collided;
+                        state = kDead;
+                    } else {
+                        const uint32_t word = P.voxels[vi];
+                        const int m = static_cast<int>(word & 0xFFu);
+                        const float rho = asFloat(word & 0xFFFFFF00u);
+                        const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
+                        if (!(rngU(rng) * mu_c > mu))
+                            state = kInteract;
+                    }
+                } else {
+                    crossed = true;
+                    tau -= mu_c * t_exit;
+                    px = __fmaf_rn(t_exit, dx, px);
+                    py = __fmaf_rn(t_exit, dy, py);
+                    pz = __fmaf_rn(t_exit, dz, pz);
+                    if (tx == t_exit)
+                        cx += dx > 0.0f ? 1 : -1;
+                    else if (ty == t_exit)
+                        cy += dy > 0.0f ? 1 : -1;
+                    else
+                        cz += dz > 0.0f ? 1 : -1;
+                    if (static_cast<unsigned>(cx) >= static_cast<unsigned>(P.cnx) || static_cast<unsigned>(cy) >= static_cast<unsigned>(P.cny)
+                        || static_cast<unsigned>(cz) >= static_cast<unsigned>(P.cnz)) {
+                        atomicAdd(&e_wave[1], static_cast<double>(w) * static_cast<double>(E));
+                        uncollided_exit = !collided;
+                        state = kDead;
+                    } else {
+                        cellc = cx | (cy << 10) | (cz << 20);
+                    }
+                }
+            }
+            n_steps += __popcll(__ballot(tentative));
+            n_cross += __popcll(__ballot(crossed));
+            n_unc += __popcll(__ballot(uncollided_exit));
+            continue;
+        }
         if (state == kTrack) {
             const float s = __fdividef(-__logf(rngU(rng)), maj);
             px = __fmaf_rn(s, dx, px);
@@ -544,6 +637,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
         atomicAdd(&P.counters->deposits, n_dep);
         atomicAdd(&P.counters->histories, n_hist);
         atomicAdd(&P.counters->uncollided, n_unc);
+        atomicAdd(&P.counters->crossings, n_cross);
         atomicAdd(&P.counters->emitted, e_emit);
         atomicAdd(&P.counters->escaped, e_esc);
         atomicAdd(&P.counters->deposited, e_dep);
@@ -626,6 +720,7 @@ __global__ void batch_reset_kernel(const DevParams* __restrict__ params, int n_i
     c->escaped = 0;
     c->deposited = 0;
     c->uncollided = 0;
+    c->crossings = 0;
 }
 
 __global__ void traverse_kernel(const DevParams P, uint64_t n_rays, const float* __restrict__ pos, const float* __restrict__ dir,
@@ -669,6 +764,7 @@ __global__ void reset_kernel(unsigned long long* pool, DevCounters* c)
         c->escaped = 0;
         c->deposited = 0;
         c->uncollided = 0;
+        c->crossings = 0;
     }
 }
 

@@ -98,6 +98,8 @@ struct dxg_world {
     DevBuf<float4> d_ia_shell;
     DevBuf<float> d_ia_inv, d_fl_cum, d_fl_line;
     DevBuf<double> d_tally;
+    DevBuf<float> d_cells;   // macro-cell majorant ratios
+    int cell = 0, cn[3] = { 0, 0, 0 };
     DevBuf<unsigned long long> d_pool;
     DevBuf<dxg::DevCounters> d_counters;
     // per-run beam tables
@@ -189,6 +191,16 @@ void fillParams(const dxg_world* w, dxg::DevParams& p)
     p.tally = w->d_tally.ptr;
     p.pool = w->d_pool.ptr;
     p.counters = w->d_counters.ptr;
+    if (w->cell > 0) {
+        p.cell_ratio = w->d_cells.ptr;
+        p.cnx = w->cn[0];
+        p.cny = w->cn[1];
+        p.cnz = w->cn[2];
+        for (int a = 0; a < 3; ++a) {
+            p.cell_len[a] = static_cast<float>(w->cell) * static_cast<float>(w->spacing[a]);
+            p.cell_inv[a] = 1.0f / p.cell_len[a];
+        }
+    }
 }
 
 // Walker alias table (Vose's construction).
@@ -341,6 +353,18 @@ int dxg_world_build(dxg_world* w)
         HIP_TRY(w->d_ia_inv.upload(w->tables.ia_inv, w->stream));
         HIP_TRY(w->d_fl_cum.upload(w->tables.fl_cum, w->stream));
         HIP_TRY(w->d_fl_line.upload(w->tables.fl_line, w->stream));
+    }
+    // macro-cell majorants are opt-in (DXG_CELL = cell edge in voxels); the
+    // global majorant is the default
+    w->cell = 0;
+    if (const char* env = std::getenv("DXG_CELL"))
+        w->cell = std::clamp(std::atoi(env), 0, 64);
+    if (w->cell > 0) {
+        const std::vector<float> ratio = dxg::cellRatios(w->tables, w->dims, w->density, w->material, w->cell, w->cn);
+        if (w->cn[0] > 1023 || w->cn[1] > 1023 || w->cn[2] > 1023)
+            w->cell = 0;   // cell coordinates are packed in 10 bits each
+        else
+            HIP_TRY(w->d_cells.upload(ratio, w->stream));
     }
     HIP_TRY(w->d_tally.alloc(4 * n));
     HIP_TRY(hipMemsetAsync(w->d_tally.ptr, 0, 4 * n * sizeof(double), w->stream));
@@ -632,6 +656,7 @@ int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress
         acc.escaped_energy += c.escaped;
         acc.deposited_energy += c.deposited;
         acc.uncollided += c.uncollided;
+        acc.cell_crossings += c.crossings;
         w->histories += p.h_end - p.h_begin;
         if (progress)
             progress->n.store(p.h_end);
