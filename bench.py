@@ -69,6 +69,37 @@ def ct_workload(histories: int):
     return ph, beam
 
 
+def ct_dual_workload(histories: int):
+    """The full CT spiral: 512 x 512 x 400 at 0.08 cm, seeded body, 120 kVp
+    dual source, 2 x 4 cm collimation, pitch 3.2, 32 cm."""
+    import opendxmc_amd as dx
+    from opendxmc_amd import phantoms
+    from opendxmc_amd.bowtie import DEFAULT_CT_BOWTIE, read_filters
+
+    ph = phantoms.ct_body((512, 512, 400), (0.08, 0.08, 0.08), seed=1)
+    beam = dx.CTSpiralDualEnergyBeam((0.0, 0.0, -16.0), (0.0, 0.0, 16.0), {13: 6.0})
+    beam.setTubeAVoltage(120.0)
+    beam.setTubeBVoltage(120.0)
+    beam.setCollimation(4.0)
+    beam.setPitch(3.2)
+    beam.setStepAngleDeg(5.0)
+    beam.setSourceDetectorDistance(119.0)
+    bow = read_filters()[DEFAULT_CT_BOWTIE]
+    beam.setBowtieFilterA(bow)
+    beam.setBowtieFilterB(bow)
+    beam.setNumberOfParticlesPerExposure(max(1, round(histories / beam.numberOfExposures())))
+    return ph, beam
+
+
+WORKLOADS = {
+    "ct-spiral-256": (ct_workload, 1e8, "CT spiral plain MC, 256x256x200 at 0.15 cm seeded body phantom, 120 kVp 6 mm Al, "
+                                        "Siemens Definition Flash W1 120kV bowtie, 4 cm collimation, pitch 1, 30 cm"),
+    "ct-spiral-512-dual": (ct_dual_workload, 1e9, "CT spiral dual source plain MC, 512x512x400 at 0.08 cm seeded body "
+                                                  "phantom, 2 x 120 kVp 6 mm Al, W1 bowtie, 2 x 4 cm collimation, "
+                                                  "pitch 3.2, 32 cm"),
+}
+
+
 def cpu_baseline(ph, beam, seconds_target: float = 15.0):
     """Time the oracle on a bounded sample: every exposure, fewer particles."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -109,7 +140,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--histories", type=float, default=1e8)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="ct-spiral-256")
+    ap.add_argument("--histories", type=float, default=None, help="histories per step (default per workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -119,8 +151,9 @@ def main() -> int:
 
     import opendxmc_amd as dx
 
-    histories = int(args.histories)
-    ph, beam = ct_workload(histories)
+    make, default_hist, description = WORKLOADS[args.workload]
+    histories = int(args.histories if args.histories else default_hist)
+    ph, beam = make(histories)
     world = dx.World()
     grid = world.addItem(dx.AAVoxelGrid(correction=1))
     grid.setData(ph.dims, ph.density, ph.material, [dx.Material.byWeight(m) for m in ph.materials])
@@ -150,7 +183,8 @@ def main() -> int:
     wall = time.perf_counter() - t0
     bytes_alg = 5.0 * steps + 24.0 * deposits
     achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
-    tph = measured_traffic_per_history()
+    # the committed counter profile covers the default workload only
+    tph = measured_traffic_per_history() if args.workload == "ct-spiral-256" else None
     hist_per_launch = hist / max(launches, 1)
     traffic = tph * hist_per_launch if tph is not None else None
     result = {
@@ -167,9 +201,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "CT spiral plain MC, 256x256x200 at 0.15 cm seeded body phantom, 120 kVp 6 mm Al, "
-                        "Siemens Definition Flash W1 120kV bowtie, 4 cm collimation, pitch 1, 30 cm, "
-                        f"{histories:.0e} histories per step, correction level 1, FP64 tally",
+            "workload": f"{description}, {histories:.0e} histories per step, correction level 1, FP64 tally",
             "histories_per_step": hist // max(args.steps, 1),
             "exposures": beam.numberOfExposures(),
             "world_build_s": build_s,

@@ -142,6 +142,35 @@ int dxg_ct_spiral_exposures(const dxg_ct_spiral_beam* b, dxg_exposure* out, uint
 int dxg_ct_sequential_exposures(const dxg_ct_sequential_beam* b, dxg_exposure* out, uint64_t cap, uint64_t* n);
 int dxg_cbct_exposures(const dxg_cbct_beam* b, dxg_exposure* out, uint64_t cap, uint64_t* n);
 
+/* ---- Beam calibration (the beams' DAPvalue / airKerma / CTDIw / CTDIvol
+ * settings, beamsettingsmodel.cpp:574-583, 673-681, 1054-1071, 1276-1295,
+ * applied through Transport's useBeamCalibration, simulationpipeline.cpp:165).
+ * Each returns the factor f that dxg_beam.calibration takes: f x (dose summed
+ * over the run's histories) is the dose in mGy for the stated output.  The
+ * beam's exposures carry the particle counts of the run to be calibrated.
+ *
+ * DX / CBCT: the dose-area product (mGy cm2) of the whole beam.  The simulated
+ * photons' air-KERMA-area product is sum over histories of w E mu_tr/rho(air)(E)
+ * (per exposure: n x weight x spectrum mean), mu_tr from the physics list with
+ * the Klein-Nishina energy-transfer fraction, dry air (C, N, O, Ar). */
+int dxg_calibration_dap(const dxg_beam* beam, double dap_mgycm2, double* factor);
+/* Pencil beam: the air KERMA (mGy) over a 1 cm2 cross-section of the beam. */
+int dxg_calibration_air_kerma(const dxg_beam* beam, double air_kerma_mgy, double* factor);
+/* CT: CTDIw (mGy) in a PMMA CTDI phantom (1.19 g/cm3, the given diameter,
+ * 15 cm long) scored as dose to PMMA in five 100 mm x 1 cm chamber cylinders,
+ * one on the axis and four 1 cm below the surface at 0, 90, 180, 270 degrees:
+ * CTDI100 = (mean chamber dose) x 10 cm / collimation, CTDIw = centre / 3 +
+ * 2/3 x mean of the four.  `rotation` holds the exposures of ONE axial
+ * rotation about the z axis through the origin with the beam's spectra,
+ * bowties and weights (n_particles per exposure each); it is simulated on the
+ * device at correction level 1 with `seed`.  The factor is ctdi_w /
+ * CTDIw(simulated) x (rotation particles per exposure / run particles per
+ * exposure).  For a spiral pass ctdi_w = CTDIvol x pitch, for a sequential
+ * scan CTDIvol x slice spacing / collimation.  ctdiw_sim (may be null)
+ * receives the simulated CTDIw per rotation, in the rotation's dose units. */
+int dxg_calibration_ctdi(const dxg_beam* rotation, double ctdi_w_mgy, double diameter_cm, double collimation_cm,
+                         uint64_t run_particles_per_exposure, uint64_t seed, double* factor, double* ctdiw_sim);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,6 +89,9 @@ typedef struct dxg_beam {
     uint64_t n_spectra;
     const dxg_bowtie* bowties;
     uint64_t n_bowties;
+    /* Beam calibration: mGy per (dose summed over the run's histories), from
+     * dxg_calibration_* (include/dxmc_beams.h); 0 means uncalibrated (1). */
+    double calibration;
 } dxg_beam;
 
 /* Device-side event counters of the last dxg_transport_run. */
@@ -141,8 +144,11 @@ uint64_t dxg_grid_size(const dxg_world* w);
  * `progress` may be null.  When another thread calls
  * dxg_progress_request_stop() the run stops taking new histories and returns
  * DXG_ERR_CANCELLED (the reference returns without collecting dose,
- * simulationpipeline.cpp:169-170).  apply_calibration is accepted for
- * signature parity; the calibration factor applied is 1. */
+ * simulationpipeline.cpp:169-170).  With apply_calibration set (the
+ * reference's useBeamCalibration, simulationpipeline.cpp:165) every deposit of
+ * the run is multiplied by beam->calibration (when > 0), so each beam's
+ * contribution to the world's scores is in mGy for its stated output; the
+ * energy counters stay uncalibrated. */
 int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress,
                       int apply_calibration, uint64_t seed);
 
@@ -200,7 +206,7 @@ int dxg_debug_device_copy(uint64_t bytes, int iters, double* ms);
  * with histories left when theirs is exhausted.  The launch is captured into
  * a HIP graph on the first run and replayed by later runs of the same batch.
  * Each instance behaves exactly like dxg_transport_run(world, beam, NULL, 1,
- * seed): same RNG streams (keyed by seed and the world's history count), same
+ * seed) (calibration included): same RNG streams (keyed by seed and the world's history count), same
  * scores and counters, read through the world afterwards.  Worlds must be
  * built, live on one device, share one correction level, hold one instance of
  * the batch and outlive it.  No progress or cancellation inside a run. */

@@ -57,6 +57,7 @@ class Beam(C.Structure):
         ("exposures", C.POINTER(Exposure)), ("n_exposures", u64),
         ("spectra", C.POINTER(Spectrum)), ("n_spectra", u64),
         ("bowties", C.POINTER(Bowtie)), ("n_bowties", u64),
+        ("calibration", C.c_double),
     ]
 
 
@@ -164,6 +165,9 @@ SIGNATURES = [
     ("dxg_debug_device_copy", C.c_int, [u64, C.c_int, PD]),
     ("dxg_debug_traverse", C.c_int, [C.c_void_p, u64, C.POINTER(C.c_float), C.POINTER(C.c_float), u64,
                                      C.POINTER(C.c_float), C.POINTER(i64), C.POINTER(C.c_int32)]),
+    ("dxg_calibration_dap", C.c_int, [C.POINTER(Beam), C.c_double, PD]),
+    ("dxg_calibration_air_kerma", C.c_int, [C.POINTER(Beam), C.c_double, PD]),
+    ("dxg_calibration_ctdi", C.c_int, [C.POINTER(Beam), C.c_double, C.c_double, C.c_double, u64, u64, PD, PD]),
     ("dxg_batch_create", C.c_int, [C.POINTER(C.c_void_p)]),
     ("dxg_batch_destroy", None, [C.c_void_p]),
     ("dxg_batch_add", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Beam), u64]),

@@ -57,7 +57,7 @@ struct BeamCopy {
         for (size_t i = 0; i < bow_a.size(); ++i)
             bowties.push_back({ bow_a[i].data(), bow_w[i].data(), bow_a[i].size() });
         view = { exposures.data(), exposures.size(), spectra.data(), spectra.size(),
-                 bowties.empty() ? nullptr : bowties.data(), bowties.size() };
+                 bowties.empty() ? nullptr : bowties.data(), bowties.size(), b->calibration };
         return DXG_OK;
     }
 };
@@ -194,6 +194,8 @@ int dxg_batch_run(dxg_batch* b, double* kernel_ms)
         if (int rc = dxg::prepareRun(in.world, &in.beam.view, in.seed, ps[static_cast<size_t>(i)], total); rc != DXG_OK)
             return rc;
         in.histories = total;
+        if (in.beam.view.calibration > 0.0)
+            ps[static_cast<size_t>(i)].dose_scale = static_cast<float>(in.beam.view.calibration);
         ps[static_cast<size_t>(i)].h_begin = 0;
         ps[static_cast<size_t>(i)].h_end = total;
         lut = lut && ps[static_cast<size_t>(i)].lds_lut != 0;

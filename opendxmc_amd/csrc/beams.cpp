@@ -475,3 +475,83 @@ int dxg_cbct_exposures(const dxg_cbct_beam* b, dxg_exposure* out, uint64_t cap, 
 }
 
 }  // extern "C"
+
+namespace {
+
+// Air-KERMA-area product (mGy cm2) of the simulated photons of a beam: sum over
+// exposures of n x weight x <E mu_tr/rho(air)(E)> of the exposure's spectrum.
+int simulatedKermaArea(const dxg_beam* beam, double& kap, const char* fn)
+{
+    if (!beam || !beam->exposures || beam->n_exposures == 0 || !beam->spectra || beam->n_spectra == 0)
+        return fail(DXG_ERR_ARGUMENT, std::string(fn) + ": beam needs exposures and spectra");
+    std::string err;
+    const dxg::PhysicsList* pl = dxg::sharedPhysics(err);
+    if (!pl)
+        return fail(DXG_ERR_PHYSICS, err);
+    const int32_t airZ[4] = { 6, 7, 8, 18 };
+    const double airW[4] = { 0.000124, 0.755268, 0.231781, 0.012827 };
+    dxg::MixMaterial air;
+    if (!dxg::makeMaterial(*pl, airZ, airW, 4, air, err))
+        return fail(DXG_ERR_MATERIAL, err);
+    std::vector<double> mean(beam->n_spectra, 0.0);
+    for (uint64_t s = 0; s < beam->n_spectra; ++s) {
+        const dxg_spectrum& sp = beam->spectra[s];
+        double wsum = 0.0, acc = 0.0;
+        for (uint64_t k = 0; k < sp.n; ++k) {
+            const double e = sp.energy_kev[k];
+            if (!(e > 0.0) || !(sp.weight[k] >= 0.0))
+                return fail(DXG_ERR_ARGUMENT, std::string(fn) + ": invalid spectrum");
+            double pe, inc, coh;
+            air.attenuation(*pl, e, pe, inc, coh);
+            acc += sp.weight[k] * e * (pe + inc * knTransferFraction(e));
+            wsum += sp.weight[k];
+        }
+        if (!(wsum > 0.0))
+            return fail(DXG_ERR_ARGUMENT, std::string(fn) + ": spectrum weights sum to zero");
+        mean[s] = acc / wsum;
+    }
+    kap = 0.0;
+    for (uint64_t i = 0; i < beam->n_exposures; ++i) {
+        const dxg_exposure& e = beam->exposures[i];
+        if (e.spectrum_index >= beam->n_spectra)
+            return fail(DXG_ERR_ARGUMENT, std::string(fn) + ": exposure references a missing spectrum");
+        kap += static_cast<double>(e.n_particles) * e.weight * mean[e.spectrum_index];
+    }
+    kap *= dxg::kKeVToJ * 1e3 * 1e3;   // keV cm2/g -> mGy cm2
+    if (!(kap > 0.0))
+        return fail(DXG_ERR_ARGUMENT, std::string(fn) + ": beam has no particles");
+    return DXG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dxg_calibration_dap(const dxg_beam* beam, double dap_mgycm2, double* factor)
+{
+    if (!factor)
+        return fail(DXG_ERR_ARGUMENT, "dxg_calibration_dap: null output");
+    if (!(dap_mgycm2 > 0.0))
+        return fail(DXG_ERR_ARGUMENT, "dxg_calibration_dap: the dose-area product must be positive");
+    double kap = 0.0;
+    if (int rc = simulatedKermaArea(beam, kap, "dxg_calibration_dap"); rc != DXG_OK)
+        return rc;
+    *factor = dap_mgycm2 / kap;
+    return DXG_OK;
+}
+
+int dxg_calibration_air_kerma(const dxg_beam* beam, double air_kerma_mgy, double* factor)
+{
+    if (!factor)
+        return fail(DXG_ERR_ARGUMENT, "dxg_calibration_air_kerma: null output");
+    if (!(air_kerma_mgy > 0.0))
+        return fail(DXG_ERR_ARGUMENT, "dxg_calibration_air_kerma: the air KERMA must be positive");
+    double kap = 0.0;
+    if (int rc = simulatedKermaArea(beam, kap, "dxg_calibration_air_kerma"); rc != DXG_OK)
+        return rc;
+    *factor = air_kerma_mgy * 1.0 / kap;   // KERMA over a 1 cm2 cross-section
+    return DXG_OK;
+}
+
+}  // extern "C"
+

@@ -92,6 +92,7 @@ struct DevParams {
 
     // tallies: per voxel {sum w dE, sum (w dE)^2, events (u64 bits), unused}
     double* tally;
+    float dose_scale;   // beam calibration applied to every deposit (1: none)
 
     // beam
     const DevExposure* exposures;

@@ -51,6 +51,7 @@ constexpr float kInvMeC2 = 1.0f / 510.99895f;
 constexpr float kCutoff = 1.0f;
 constexpr float kTwoPi = 6.283185307179586f;
 constexpr int kBlock = 256;
+constexpr int kMaxCross = 8;    // macro-cell crossings a lane walks per iteration
 constexpr int kIaShellsd = 5;    // impulse-approximation shells per material
 constexpr int kIaInvNd = 2048;   // unit Compton profile inverse-CDF nodes
 constexpr int kFlMaxd = 12;      // fluorescing elements per material
@@ -163,7 +164,7 @@ __device__ inline void rotate(float& dx, float& dy, float& dz, float cost, float
 
 __device__ inline void tallyAdd(const DevParams& P, int idx, float e)
 {
-    const double de = static_cast<double>(e);
+    const double de = static_cast<double>(e) * static_cast<double>(P.dose_scale);
     double* t = P.tally + 4 * static_cast<size_t>(idx);
     atomicAdd(t, de);
     atomicAdd(t + 1, de * de);
@@ -545,64 +546,85 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
         bool uncollided_exit = false;
         if (P.cell_ratio) {
             // Woodcock tracking with macro-cell majorants: a flight carries an
-            // optical depth tau through cells of majorant ratio[c] * maj; each
-            // iteration either crosses into the next cell (tau -= mu_c * d)
-            // or ends the flight inside the cell at a tentative collision,
-            // accepted as real with probability mu(voxel) / mu_c.
-            bool tentative = false, crossed = false;
+            // optical depth tau through cells of majorant ratio[c] * maj.  In
+            // one iteration a tracking lane walks up to kMaxCross cells
+            // (Amanatides-Woo: the next face per axis advances by one cell
+            // length over |d|), subtracting mu_c * segment from tau, and ends
+            // the flight at a tentative collision inside the cell where tau
+            // runs out; that collision is real with probability mu(voxel) / mu_c.
+            bool tentative = false;
+            int crossed = 0;
             if (state == kTrack) {
                 if (tau < 0.0f)
                     tau = -__logf(rngU(rng));
                 int cx = cellc & 1023, cy = (cellc >> 10) & 1023, cz = cellc >> 20;
-                const float mu_c = P.cell_ratio[cx + P.cnx * (cy + P.cny * cz)] * maj;
-                const float tx = exitDistance(px, dx, P.bmin[0], P.cell_len[0], cx);
-                const float ty = exitDistance(py, dy, P.bmin[1], P.cell_len[1], cy);
-                const float tz = exitDistance(pz, dz, P.bmin[2], P.cell_len[2], cz);
-                const float t_exit = fminf(tx, fminf(ty, tz));
-                if (tau < mu_c * t_exit) {
-                    tentative = true;
-                    const float s = __fdividef(tau, mu_c);
-                    tau = -1.0f;
-                    px = __fmaf_rn(s, dx, px);
-                    py = __fmaf_rn(s, dy, py);
-                    pz = __fmaf_rn(s, dz, pz);
-                    vi = voxelOf(P, px, py, pz);
-                    if (vi < 0) {
-                        atomicAdd(&e_wave[1], static_cast<double>(w) * static_cast<double>(E));
-                        uncollided_exit = !collided;
-                        state = kDead;
-                    } else {
-                        const uint32_t word = P.voxels[vi];
-                        const int m = static_cast<int>(word & 0xFFu);
-                        const float rho = asFloat(word & 0xFFFFFF00u);
-                        const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
-                        if (!(rngU(rng) * mu_c > mu))
-                            state = kInteract;
+                float mu_c = P.cell_ratio[cx + P.cnx * (cy + P.cny * cz)] * maj;
+                float tmx = exitDistance(px, dx, P.bmin[0], P.cell_len[0], cx);
+                float tmy = exitDistance(py, dy, P.bmin[1], P.cell_len[1], cy);
+                float tmz = exitDistance(pz, dz, P.bmin[2], P.cell_len[2], cz);
+                const float ddx = dx != 0.0f ? __fdividef(P.cell_len[0], fabsf(dx)) : __builtin_inff();
+                const float ddy = dy != 0.0f ? __fdividef(P.cell_len[1], fabsf(dy)) : __builtin_inff();
+                const float ddz = dz != 0.0f ? __fdividef(P.cell_len[2], fabsf(dz)) : __builtin_inff();
+                float t0 = 0.0f;   // path walked in this iteration up to the current cell's entry face
+                float s = 0.0f;    // path to move the photon by at the end of the iteration
+                bool out = false;
+                for (;;) {
+                    const float t1 = fminf(tmx, fminf(tmy, tmz));
+                    const float seg = t1 - t0;
+                    if (tau < mu_c * seg) {
+                        tentative = true;
+                        s = t0 + __fdividef(tau, mu_c);
+                        tau = -1.0f;
+                        break;
                     }
-                } else {
-                    crossed = true;
-                    tau -= mu_c * t_exit;
-                    px = __fmaf_rn(t_exit, dx, px);
-                    py = __fmaf_rn(t_exit, dy, py);
-                    pz = __fmaf_rn(t_exit, dz, pz);
-                    if (tx == t_exit)
+                    if (crossed == kMaxCross) {   // the rest of the flight goes on next iteration
+                        s = t0;
+                        break;
+                    }
+                    tau -= mu_c * seg;
+                    t0 = t1;
+                    ++crossed;
+                    if (tmx == t1) {
                         cx += dx > 0.0f ? 1 : -1;
-                    else if (ty == t_exit)
+                        tmx += ddx;
+                    } else if (tmy == t1) {
                         cy += dy > 0.0f ? 1 : -1;
-                    else
+                        tmy += ddy;
+                    } else {
                         cz += dz > 0.0f ? 1 : -1;
+                        tmz += ddz;
+                    }
                     if (static_cast<unsigned>(cx) >= static_cast<unsigned>(P.cnx) || static_cast<unsigned>(cy) >= static_cast<unsigned>(P.cny)
                         || static_cast<unsigned>(cz) >= static_cast<unsigned>(P.cnz)) {
-                        atomicAdd(&e_wave[1], static_cast<double>(w) * static_cast<double>(E));
-                        uncollided_exit = !collided;
-                        state = kDead;
-                    } else {
-                        cellc = cx | (cy << 10) | (cz << 20);
+                        out = true;
+                        s = t0;
+                        break;
                     }
+                    mu_c = P.cell_ratio[cx + P.cnx * (cy + P.cny * cz)] * maj;
+                }
+                px = __fmaf_rn(s, dx, px);
+                py = __fmaf_rn(s, dy, py);
+                pz = __fmaf_rn(s, dz, pz);
+                cellc = cx | (cy << 10) | (cz << 20);
+                if (tentative)
+                    vi = voxelOf(P, px, py, pz);
+                if (out || (tentative && vi < 0)) {
+                    atomicAdd(&e_wave[1], static_cast<double>(w) * static_cast<double>(E));
+                    uncollided_exit = !collided;
+                    state = kDead;
+                    tentative = false;
+                } else if (tentative) {
+                    const uint32_t word = P.voxels[vi];
+                    const int m = static_cast<int>(word & 0xFFu);
+                    const float rho = asFloat(word & 0xFFFFFF00u);
+                    const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
+                    if (!(rngU(rng) * mu_c > mu))
+                        state = kInteract;
                 }
             }
             n_steps += __popcll(__ballot(tentative));
-            n_cross += __popcll(__ballot(crossed));
+            for (int b = 0; b < 4; ++b)   // crossed <= kMaxCross < 16: sum it bit by bit
+                n_cross += static_cast<unsigned long long>(__popcll(__ballot((crossed >> b) & 1))) << b;
             n_unc += __popcll(__ballot(uncollided_exit));
             continue;
         }

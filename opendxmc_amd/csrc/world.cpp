@@ -561,6 +561,7 @@ int prepareRun(dxg_world* w, const dxg_beam* beam, uint64_t seed, DevParams& p, 
     p.bb = bb;
     p.seed = seed;
     p.rng_offset = w->histories;
+    p.dose_scale = 1.0f;
     p.refill_threshold = 16;
     if (const char* env = std::getenv("DXG_REFILL"))
         p.refill_threshold = std::clamp(std::atoi(env), 1, 64);
@@ -596,11 +597,12 @@ extern "C" {
 
 int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress, int apply_calibration, uint64_t seed)
 {
-    (void)apply_calibration;
     dxg::DevParams p;
     uint64_t total = 0;
     if (int rc = dxg::prepareRun(w, beam, seed, p, total); rc != DXG_OK)
         return rc;
+    if (apply_calibration && beam->calibration > 0.0)
+        p.dose_scale = static_cast<float>(beam->calibration);
     const size_t lds_bytes = dxg::transportLdsBytes(p);
 
     // ---- launch geometry: fill every CU to the occupancy limit

@@ -273,6 +273,19 @@ class _BeamBase:
     def bowties(self) -> list[BowtieFilter]:
         return []
 
+    def calibrationFactor(self) -> float:
+        """mGy per (dose summed over a run's histories) for the beam's stated
+        output (DAP, air KERMA or CTDI); 0 when none is set."""
+        return 0.0
+
+
+def _calibrate(fn, beam, value: float) -> float:
+    k = _Keep()
+    b = beam_struct(beam, k, calibrated=False)
+    f = C.c_double()
+    N.check(fn(C.byref(b), float(value), C.byref(f)))
+    return f.value
+
 
 class PencilBeam(_BeamBase):
     def __init__(self, position=(0.0, 0.0, 0.0), direction=(0.0, 0.0, 1.0), energy: float = 60.0):
@@ -293,6 +306,18 @@ class PencilBeam(_BeamBase):
 
     def spectra(self):
         return [(np.array([float(self.energy)]), np.array([1.0]))]
+
+    air_kerma = 0.0
+
+    def setAirKerma(self, mgy: float) -> None:
+        """Air KERMA (mGy) over a 1 cm2 cross-section of the beam (beamsettingsmodel.cpp:673-681)."""
+        self.air_kerma = float(mgy)
+
+    def airKerma(self) -> float:
+        return self.air_kerma
+
+    def calibrationFactor(self) -> float:
+        return _calibrate(N.lib().dxg_calibration_air_kerma, self, self.air_kerma) if self.air_kerma > 0 else 0.0
 
 
 class DXBeam(_BeamBase):
@@ -328,6 +353,18 @@ class DXBeam(_BeamBase):
         d = np.cross(c0, c1)
         pos = np.asarray(self.rotation_center) - self.spd * d
         return pos, c0, c1
+
+    dap = 0.0
+
+    def setDAPvalue(self, mgycm2: float) -> None:
+        """Dose-area product (mGy cm2) of the beam (beamsettingsmodel.cpp:574-583)."""
+        self.dap = float(mgycm2)
+
+    def DAPvalue(self) -> float:
+        return self.dap
+
+    def calibrationFactor(self) -> float:
+        return _calibrate(N.lib().dxg_calibration_dap, self, self.dap) if self.dap > 0 else 0.0
 
     def exposures(self) -> np.ndarray:
         pos, c0, c1 = self._geometry()
@@ -387,6 +424,32 @@ class CTSpiralBeam(_BeamBase):
         k = _Keep()
         return self._collect(N.lib().dxg_ct_spiral_exposures, self._params(k))
 
+    ctdi_vol = 0.0
+    ctdi_diameter = 32.0
+
+    def setCTDIvol(self, mgy: float) -> None:
+        """CTDIvol (mGy) the scan is calibrated to (beamsettingsmodel.cpp:1276-1284)."""
+        self.ctdi_vol = float(mgy)
+
+    def CTDIvol(self) -> float:
+        return self.ctdi_vol
+
+    def setCTDIdiameter(self, cm: float) -> None:
+        """Diameter (cm) of the PMMA CTDI phantom (beamsettingsmodel.cpp:1286-1295)."""
+        self.ctdi_diameter = float(cm)
+
+    def CTDIdiameter(self) -> float:
+        return self.ctdi_diameter
+
+    def _rotation_tubes(self):
+        """(tube, spectrum, bowtie, start angle, fov, relative mAs) per tube."""
+        return [(self.tube, 0, self.bowtie, self.start_angle, self.fov, getattr(self, "mas_a", 1.0))]
+
+    def calibrationFactor(self) -> float:
+        if self.ctdi_vol <= 0:
+            return 0.0
+        return _ctdi_factor(self, self.ctdi_vol * self.pitch)
+
 
 class CTSpiralDualEnergyBeam(CTSpiralBeam):
     """CTSpiralDualEnergyBeam<false>: tubes A and B, exposures interleaved A, B."""
@@ -420,6 +483,10 @@ class CTSpiralDualEnergyBeam(CTSpiralBeam):
 
     def spectra(self):
         return [self.tube.getSpecter(), self.tubeB.getSpecter()]
+
+    def _rotation_tubes(self):
+        return [(self.tube, 0, self.bowtie, self.start_angle, self.fov, self.mas_a),
+                (self.tubeB, 1, self.bowtieB or self.bowtie, self.start_angle + self.tube_b_offset, self.fov_b, self.mas_b)]
 
 
 class CTSequentialBeam(_BeamBase):
@@ -457,6 +524,76 @@ class CTSequentialBeam(_BeamBase):
                                  self.numberOfParticlesPerExposure, int(self.bowtie is not None), 0)
         return self._collect(N.lib().dxg_ct_sequential_exposures, p)
 
+    ctdi_w = 0.0
+    ctdi_diameter = 32.0
+
+    def setCTDIw(self, mgy: float) -> None:
+        """CTDIw (mGy) of one rotation (beamsettingsmodel.cpp:1050-1060)."""
+        self.ctdi_w = float(mgy)
+
+    def CTDIw(self) -> float:
+        return self.ctdi_w
+
+    def setCTDIdiameter(self, cm: float) -> None:
+        self.ctdi_diameter = float(cm)
+
+    def CTDIdiameter(self) -> float:
+        return self.ctdi_diameter
+
+    def _rotation_tubes(self):
+        return [(self.tube, 0, self.bowtie, self.start_angle, self.fov, 1.0)]
+
+    def calibrationFactor(self) -> float:
+        return _ctdi_factor(self, self.ctdi_w) if self.ctdi_w > 0 else 0.0
+
+
+CTDI_PARTICLES_PER_EXPOSURE = 100_000
+
+
+class _Rotation(_BeamBase):
+    """One axial rotation at the origin of a CT beam's tubes (CTDI phantom run)."""
+
+    def __init__(self, ct):
+        self.ct = ct
+        parts, self._spectra, self._bowties = [], [], []
+        for tube, idx, bow, start, fov, mas in ct._rotation_tubes():
+            p = N.CTSequentialParams((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1, 1.0, start, ct.step_angle, ct.collimation, ct.sdd,
+                                     fov, CTDI_PARTICLES_PER_EXPOSURE, int(bow is not None), 0)
+            ex = self._collect(N.lib().dxg_ct_sequential_exposures, p)
+            ex["spectrum_index"] = len(self._spectra)
+            self._spectra.append(tube.getSpecter())
+            if bow is not None:
+                ex["bowtie_index"] = len(self._bowties)
+                self._bowties.append(bow)
+            ex["weight"] *= mas
+            parts.append(ex)
+        self._ex = np.concatenate(parts)
+
+    def exposures(self):
+        return self._ex
+
+    def spectra(self):
+        return self._spectra
+
+    def bowties(self):
+        return self._bowties
+
+
+def _ctdi_factor(ct, ctdi_w: float) -> float:
+    key = (ctdi_w, ct.ctdi_diameter, ct.numberOfParticlesPerExposure, ct.collimation, ct.sdd, ct.step_angle,
+           repr([(t.voltage, t.anode_angle_deg, sorted(t.filtration.items()), i, id(b), a, f, m)
+                 for t, i, b, a, f, m in ct._rotation_tubes()]))
+    cache = getattr(ct, "_ctdi_cache", None)
+    if cache and cache[0] == key:
+        return cache[1]
+    k = _Keep()
+    b = beam_struct(_Rotation(ct), k, calibrated=False)
+    f, sim = C.c_double(), C.c_double()
+    N.check(N.lib().dxg_calibration_ctdi(C.byref(b), float(ctdi_w), float(ct.ctdi_diameter), float(ct.collimation),
+                                          int(ct.numberOfParticlesPerExposure), 12345, C.byref(f), C.byref(sim)))
+    ct._ctdi_cache = (key, f.value)
+    return f.value
+
 
 class CBCTBeam(_BeamBase):
     """CBCTBeam<false> (beamsettingsmodel.cpp:720-900)."""
@@ -477,6 +614,18 @@ class CBCTBeam(_BeamBase):
     def setSourceDetectorDistance(self, d): self.sdd = float(d)
     def setCollimationHalfAnglesDeg(self, a, b): self.half_angles = (math.radians(a), math.radians(b))
 
+    dap = 0.0
+
+    def setDAPvalue(self, mgycm2: float) -> None:
+        """Dose-area product (mGy cm2) of the whole scan (beamsettingsmodel.cpp:859-868)."""
+        self.dap = float(mgycm2)
+
+    def DAPvalue(self) -> float:
+        return self.dap
+
+    def calibrationFactor(self) -> float:
+        return _calibrate(N.lib().dxg_calibration_dap, self, self.dap) if self.dap > 0 else 0.0
+
     def exposures(self) -> np.ndarray:
         p = N.CBCTParams(tuple(self.isocenter), tuple(self.rotation_axis), self.start_angle, self.stop_angle,
                          self.step_angle, self.sdd, tuple(self.half_angles), self.numberOfParticlesPerExposure)
@@ -484,8 +633,9 @@ class CBCTBeam(_BeamBase):
 
 
 # ------------------------------------------------------------------ transport
-def beam_struct(beam, keep: _Keep) -> N.Beam:
-    """Flatten a beam object into the dxg_beam record (arrays kept alive by `keep`)."""
+def beam_struct(beam, keep: _Keep, calibrated: bool = True) -> N.Beam:
+    """Flatten a beam object into the dxg_beam record (arrays kept alive by
+    `keep`); with `calibrated` the beam's calibration factor is included."""
     ex = keep.keep(np.ascontiguousarray(beam.exposures()))
     specs = []
     for e, w in beam.spectra():
@@ -499,7 +649,8 @@ def beam_struct(beam, keep: _Keep) -> N.Beam:
         w = keep.keep(np.ascontiguousarray(b.weight, dtype=np.float64))
         bows.append(N.Bowtie(N.ptr(a, C.c_double), N.ptr(w, C.c_double), len(a)))
     bow_arr = keep.keep((N.Bowtie * max(1, len(bows)))(*bows))
-    return N.Beam(ex.ctypes.data_as(C.POINTER(N.Exposure)), len(ex), spec_arr, len(specs), bow_arr, len(bows))
+    cal = beam.calibrationFactor() if calibrated else 0.0
+    return N.Beam(ex.ctypes.data_as(C.POINTER(N.Exposure)), len(ex), spec_arr, len(specs), bow_arr, len(bows), cal)
 
 
 class Transport:
@@ -514,7 +665,7 @@ class Transport:
 
     def __call__(self, world: World, beam, progress: TransportProgress | None = None, useBeamCalibration: bool = True):
         k = _Keep()
-        b = beam_struct(beam, k)
+        b = beam_struct(beam, k, calibrated=useBeamCalibration)
         grid = world.items[0]
         rc = N.lib().dxg_transport_run(grid._h, C.byref(b), progress._h if progress else None,
                                        int(bool(useBeamCalibration)), self.seed)

@@ -1,0 +1,180 @@
+"""Beam calibration (include/dxmc_beams.h dxg_calibration_*; Transport's
+useBeamCalibration): DAP and air KERMA in closed form, CTDI through a
+simulated PMMA phantom.  DXMClib's own calibration is absent, so these pin the
+definitions: NIST air mu_en/rho, exact linearity in the stated output and in
+the particle count, and self-consistency of a calibrated CT scan measured in
+the CTDI phantom."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import opendxmc_amd as dx
+from opendxmc_amd import _native as N
+from opendxmc_amd.bowtie import DEFAULT_CT_BOWTIE, read_filters
+
+KEV_CM2_PER_G_TO_MGY_CM2 = 1.602176634e-16 * 1e3 * 1e3
+
+
+def pencil(n=1000, energy=60.0):
+    b = dx.PencilBeam((0, 0, -10), (0, 0, 1), energy)
+    b.setNumberOfParticlesPerExposure(n)
+    return b
+
+
+@pytest.mark.parametrize("energy,nist,tol", [(100.0, 0.02325, 0.03), (60.0, 0.03041, 0.07)])
+def test_dap_matches_nist_air_mu_en(energy, nist, tol):
+    """The factor implies mu_tr/rho(air); NIST's mu_en/rho (radiative losses in
+    air are ~0.02 %, so mu_tr = mu_en to that level).  The physics list's
+    photoelectric cross section runs low toward lower energies (about -6 % of
+    this total at 60 keV, where photoabsorption is 40 % of it)."""
+    b = pencil(1000, energy)
+    b.setAirKerma(1.0)
+    f = b.calibrationFactor()
+    mu_tr = 1.0 / (f * 1000 * energy * KEV_CM2_PER_G_TO_MGY_CM2)
+    assert mu_tr == pytest.approx(nist, rel=tol)
+
+
+def test_calibration_is_linear_in_output_and_particles():
+    b = dx.DXBeam({13: 2.0, 29: 0.1})
+    b.setNumberOfParticlesPerExposure(100_000)
+    b.setDAPvalue(10.0)
+    f1 = b.calibrationFactor()
+    b.setDAPvalue(30.0)
+    assert b.calibrationFactor() == pytest.approx(3 * f1, rel=1e-12)
+    b.setNumberOfParticlesPerExposure(400_000)
+    assert b.calibrationFactor() == pytest.approx(3 * f1 / 4, rel=1e-12)
+    b.setDAPvalue(0.0)
+    assert b.calibrationFactor() == 0.0
+
+
+def test_air_kerma_equals_dap_over_one_cm2():
+    b = pencil(5000, 80.0)
+    b.setAirKerma(2.5)
+    k = dx.engine._Keep()
+    s = dx.engine.beam_struct(b, k, calibrated=False)
+    import ctypes as C
+    f = C.c_double()
+    N.check(N.lib().dxg_calibration_dap(C.byref(s), 2.5, C.byref(f)))
+    assert b.calibrationFactor() == pytest.approx(f.value, rel=1e-15)
+
+
+def test_dap_weights_and_spectra():
+    """Two exposures of different weight and energy: the simulated KERMA-area
+    product is the sum of n x weight x E mu_tr/rho over exposures."""
+    b = pencil(1000, 50.0)
+    k = dx.engine._Keep()
+    import ctypes as C
+    ex = np.zeros(2, dtype=N.EXPOSURE_DTYPE)
+    ex[:] = b.exposures()[0]
+    ex["weight"] = (1.0, 3.0)
+    ex["spectrum_index"] = (0, 1)
+    ex["n_particles"] = (1000, 2000)
+    specs = []
+    for e in (50.0, 100.0):
+        ea = k.keep(np.array([e]))
+        wa = k.keep(np.array([1.0]))
+        specs.append(N.Spectrum(N.ptr(ea, C.c_double), N.ptr(wa, C.c_double), 1))
+    arr = (N.Spectrum * 2)(*specs)
+    beam = N.Beam(ex.ctypes.data_as(C.POINTER(N.Exposure)), 2, arr, 2, None, 0, 0.0)
+    f = C.c_double()
+    N.check(N.lib().dxg_calibration_dap(C.byref(beam), 1.0, C.byref(f)))
+    # per-energy KERMA-area products from single-exposure beams
+    def kap(e):
+        p = pencil(1, e)
+        p.setAirKerma(1.0)
+        return 1.0 / p.calibrationFactor()
+    expected = 1000 * 1.0 * kap(50.0) + 2000 * 3.0 * kap(100.0)
+    assert 1.0 / f.value == pytest.approx(expected, rel=1e-12)
+
+
+def test_calibration_rejects_bad_values():
+    import ctypes as C
+    k = dx.engine._Keep()
+    s = dx.engine.beam_struct(pencil(), k, calibrated=False)
+    f = C.c_double()
+    with pytest.raises(N.NativeError) as e:
+        N.check(N.lib().dxg_calibration_dap(C.byref(s), 0.0, C.byref(f)))
+    assert e.value.code == N.DXG_ERR_ARGUMENT
+    with pytest.raises(N.NativeError) as e:
+        N.check(N.lib().dxg_calibration_ctdi(C.byref(s), 10.0, 2.0, 4.0, 1000, 1, C.byref(f), None))
+    assert e.value.code == N.DXG_ERR_ARGUMENT
+
+
+# ------------------------------------------------------------------ GPU
+
+def ctdi_phantom(diameter: float):
+    """The phantom dxg_calibration_ctdi simulates: 0.25 cm voxels, PMMA
+    cylinder 15 cm long, chamber cylinders of 1 cm x 10 cm (index 0 centre,
+    1..4 at 1 cm depth)."""
+    h = 0.25
+    nxy = 2 * math.ceil((0.5 * diameter + 1.0) / h)
+    nz = 2 * math.ceil((7.5 + 0.5) / h)
+    ax = [(np.arange(n) + 0.5 - 0.5 * n) * h for n in (nz, nxy, nxy)]
+    Z, Y, X = np.meshgrid(*ax, indexing="ij")
+    R = 0.5 * diameter
+    pmma = (X ** 2 + Y ** 2 <= R * R) & (np.abs(Z) <= 7.5)
+    material = pmma.astype(np.uint8)
+    density = np.where(pmma, 1.19, 0.001205)
+    centres = [(0, 0), (R - 1, 0), (0, R - 1), (-(R - 1), 0), (0, -(R - 1))]
+    chambers = [pmma & (np.abs(Z) <= 5.0) & ((X - cx) ** 2 + (Y - cy) ** 2 <= 0.25) for cx, cy in centres]
+    mats = [{6: 0.000124, 7: 0.755268, 8: 0.231781, 18: 0.012827}, {1: 0.080538, 6: 0.599848, 8: 0.319614}]
+    return (nxy, nxy, nz), (h, h, h), density.ravel(), material.ravel(), mats, chambers
+
+
+@pytest.mark.gpu
+def test_calibrated_axial_scan_reproduces_ctdiw():
+    """A sequential (axial) scan calibrated to CTDIw = 10 mGy, run on the CTDI
+    phantom itself with another particle count, measures CTDIw = 10 mGy."""
+    beam = dx.CTSequentialBeam((0, 0, 0))
+    beam.setCollimation(4.0)
+    beam.setStepAngleDeg(10.0)
+    beam.setBowtieFilter(read_filters()[DEFAULT_CT_BOWTIE])
+    beam.setNumberOfParticlesPerExposure(300_000)
+    beam.setCTDIw(10.0)
+    beam.setCTDIdiameter(32.0)
+    f = beam.calibrationFactor()
+    assert f > 0
+    dims, spacing, dens, mat, mats, chambers = ctdi_phantom(32.0)
+    world = dx.World()
+    grid = world.addItem(dx.AAVoxelGrid(correction=1))
+    grid.setData(dims, dens, mat, [dx.Material.byWeight(m) for m in mats])
+    grid.setSpacing(spacing)
+    world.build()
+    dx.Transport(seed=77)(world, beam, None, True)
+    dose = grid.doseScored()[0].reshape(dims[::-1])
+    mass = dens.reshape(dims[::-1])
+    ctdi = [float((dose[c] * mass[c]).sum() / mass[c].sum()) * 10.0 / 4.0 for c in chambers]
+    ctdiw = ctdi[0] / 3 + 2 / 3 * np.mean(ctdi[1:])
+    print("CTDI100 per chamber", ctdi, "CTDIw", ctdiw)
+    assert ctdiw == pytest.approx(10.0, rel=0.02)
+    # uncalibrated runs leave the raw dose
+    grid.clearScores()
+    dx.Transport(seed=78)(world, beam, None, False)
+    raw = grid.doseScored()[0].reshape(dims[::-1])
+    assert float(raw[chambers[0]].mean()) == pytest.approx(float(dose[chambers[0]].mean()) / f, rel=0.05)
+
+
+@pytest.mark.gpu
+def test_dap_calibrated_dose_is_independent_of_particle_count():
+    from opendxmc_amd import phantoms
+    ph = phantoms.water_cube(32, 0.5)
+    world = dx.World()
+    grid = world.addItem(dx.AAVoxelGrid(correction=1))
+    grid.setData(ph.dims, ph.density, ph.material, [dx.Material.byWeight(m) for m in ph.materials])
+    grid.setSpacing(ph.spacing)
+    world.build()
+    beam = dx.DXBeam({13: 2.0})
+    beam.tube.voltage = 80.0
+    beam.setCollimation((10.0, 10.0))
+    beam.setDAPvalue(50.0)
+    totals = []
+    for n, seed in ((200_000, 1), (800_000, 2)):
+        grid.clearScores()
+        beam.setNumberOfParticlesPerExposure(n)
+        dx.Transport(seed=seed)(world, beam, None, True)
+        d = grid.doseScored()[0]
+        totals.append(float((d * ph.density).sum()))
+    assert totals[1] == pytest.approx(totals[0], rel=0.01)
