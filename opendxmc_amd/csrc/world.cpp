@@ -354,9 +354,11 @@ int dxg_world_build(dxg_world* w)
         HIP_TRY(w->d_fl_cum.upload(w->tables.fl_cum, w->stream));
         HIP_TRY(w->d_fl_line.upload(w->tables.fl_line, w->stream));
     }
-    // macro-cell majorants are opt-in (DXG_CELL = cell edge in voxels); the
-    // global majorant is the default
-    w->cell = 0;
+    // Macro-cell majorants pay off once the voxel words no longer sit in the
+    // 256 MB last-level cache: grids of >= 32 M voxels (128 MB of words) use
+    // cells of 32^3 voxels, smaller grids the global majorant alone.
+    // DXG_CELL = cell edge in voxels overrides (0 = global majorant).
+    w->cell = n >= (32ull << 20) ? 32 : 0;
     if (const char* env = std::getenv("DXG_CELL"))
         w->cell = std::clamp(std::atoi(env), 0, 64);
     if (w->cell > 0) {
