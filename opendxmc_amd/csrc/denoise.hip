@@ -120,8 +120,9 @@ struct ConvArgs {
     int y32_channels;        // 1 or 32
     int relu;
     int nb, nz, ny, nx;
-    int wave_tiles;          // nb*nz*ny*(nx/64)
+    int wave_tiles;          // tiles to compute: nb*nz*ny*(nx/64), or the length of `tiles`
     int chunk;               // work-group tiles (4 wave tiles) per work-group
+    const int* tiles;        // null: every tile; else the (ascending) tile indices to compute
 };
 
 // Weight fragment (tap, out channel co, 16-B chunk q) lives at chunk
@@ -170,9 +171,10 @@ __global__ __launch_bounds__(256) void conv3d_kernel(ConvArgs a)
     const int nxs = a.nx >> 6;
 
     for (int wt = begin; wt < end; ++wt) {
-        const int t = wt * 4 + wave;
-        if (t >= a.wave_tiles)
+        const int k = wt * 4 + wave;
+        if (k >= a.wave_tiles)
             break;
+        const int t = a.tiles ? a.tiles[k] : k;
         int r = t / nxs;
         const int x0 = (t - r * nxs) * 64;
         const int y = r % a.ny;
@@ -429,30 +431,50 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a)
         a.part_b[static_cast<size_t>(blockIdx.x) * 32 + tid] = bsh[tid];
 }
 
-// Sum the per-work-group partials into one layer's gradient in parameter
-// layout ([co][tap][ci] weights, then 32 biases).
-__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float* part, const float* part_b, int parts, float* grad)
+// Sum the per-work-group partials into one layer's gradient, in two fixed-
+// order stages: slice y of the partials into sums[y], then the slices into the
+// parameter layout ([co][tap][ci] weights, then 32 biases).
+constexpr int kReduceSlices = 8;
+
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float* part, const float* part_b, int parts, float* sums)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
+    const int per = (parts + kReduceSlices - 1) / kReduceSlices;
+    const int p0 = blockIdx.y * per, p1 = min(parts, p0 + per);
     if (i < kLayerW) {
         float s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-        int p = 0;
-        for (; p + 4 <= parts; p += 4) {
+        int p = p0;
+        for (; p + 4 <= p1; p += 4) {
             s0 += part[static_cast<size_t>(p) * kLayerW + i];
             s1 += part[static_cast<size_t>(p + 1) * kLayerW + i];
             s2 += part[static_cast<size_t>(p + 2) * kLayerW + i];
             s3 += part[static_cast<size_t>(p + 3) * kLayerW + i];
         }
-        for (; p < parts; ++p)
+        for (; p < p1; ++p)
             s0 += part[static_cast<size_t>(p) * kLayerW + i];
-        const int tap = i >> 10, co = (i >> 5) & 31, ci = i & 31;
-        grad[(co * kTaps + tap) * kC + ci] = (s0 + s1) + (s2 + s3);
+        sums[blockIdx.y * kLayerP + i] = (s0 + s1) + (s2 + s3);
     } else if (i < kLayerP) {
         const int co = i - kLayerW;
         float s = 0;
-        for (int p = 0; p < parts; ++p)
+        for (int p = p0; p < p1; ++p)
             s += part_b[p * 32 + co];
-        grad[kLayerW + co] = s;
+        sums[blockIdx.y * kLayerP + i] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void wgrad_finish_kernel(const float* sums, float* grad)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kLayerP)
+        return;
+    float s = 0;
+    for (int k = 0; k < kReduceSlices; ++k)
+        s += sums[k * kLayerP + i];
+    if (i < kLayerW) {
+        const int tap = i >> 10, co = (i >> 5) & 31, ci = i & 31;
+        grad[(co * kTaps + tap) * kC + ci] = s;
+    } else {
+        grad[i] = s;
     }
 }
 
@@ -634,15 +656,15 @@ __global__ __launch_bounds__(256) void world_features_kernel(WorldFeatArgs a)
 }
 
 // Denoised dose (mGy) from the predicted relative dose.
-__global__ __launch_bounds__(256) void world_dose_kernel(const float* pred, int nx, int nxp, size_t nv, const double* acc,
-                                                         double scale, double* out)
+__global__ __launch_bounds__(256) void world_dose_kernel(const float* feat0, const float* corr, int nx, int nxp, size_t nv,
+                                                         const double* acc, double scale, double* out)
 {
     const size_t v = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
     if (v >= nv)
         return;
     const size_t row = v / nx;
-    const int x = static_cast<int>(v - row * nx);
-    out[v] = fmax(static_cast<double>(pred[row * nxp + x]), 0.0) * doseRef(acc) * scale;
+    const size_t i = row * nxp + (v - row * nx);
+    out[v] = fmax(static_cast<double>(feat0[i] + corr[i]), 0.0) * doseRef(acc) * scale;
 }
 
 // ---------------------------------------------------------------- host side
@@ -700,7 +722,8 @@ hipError_t launchConv(ConvArgs a, const Dims& d, hipStream_t s)
     a.nz = d.nz;
     a.ny = d.ny;
     a.nx = d.nx;
-    a.wave_tiles = d.nb * d.nz * d.ny * (d.nx / 64);
+    if (!a.tiles)
+        a.wave_tiles = d.nb * d.nz * d.ny * (d.nx / 64);
     const int wg_tiles = (a.wave_tiles + 3) / 4;
     const int grid = roundUp8(std::min(wg_tiles, 2 * computeUnits()));
     a.chunk = (wg_tiles + grid - 1) / grid;
@@ -717,7 +740,7 @@ int wgradGrid(const Dims& d)
     return roundUp8(std::min(units, 2 * computeUnits()));
 }
 
-hipError_t launchWgrad(const uint16_t* x, const uint16_t* g, float* part, float* part_b, float* grad, const Dims& d,
+hipError_t launchWgrad(const uint16_t* x, const uint16_t* g, float* part, float* part_b, float* sums, float* grad, const Dims& d,
                        hipStream_t s)
 {
     WgradArgs a {};
@@ -733,8 +756,62 @@ hipError_t launchWgrad(const uint16_t* x, const uint16_t* g, float* part, float*
     const int grid = wgradGrid(d);
     a.chunk = (a.units + grid - 1) / grid;
     hipLaunchKernelGGL(wgrad_kernel, dim3(grid), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((kLayerP + 255) / 256), dim3(256), 0, s, part, part_b, grid, grad);
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((kLayerP + 255) / 256, kReduceSlices), dim3(256), 0, s, part, part_b, grid, sums);
+    hipLaunchKernelGGL(wgrad_finish_kernel, dim3((kLayerP + 255) / 256), dim3(256), 0, s, sums, grad);
     return hipGetLastError();
+}
+
+// Receptive-field radius of the 18 convolutions, plus one voxel: a voxel of
+// the body depends on nothing farther than this.
+constexpr int kReach = kLayers + 1;
+
+// Wave tiles (64 voxels along x of one row) of a padded world volume that lie
+// within kReach voxels (Chebyshev) of a voxel with material != 0, ascending.
+// Body-voxel predictions computed on these tiles alone equal the full
+// computation: outside them the activations stay zero, and that difference
+// moves inward by one voxel per layer, so it never reaches the body.
+std::vector<int> bodyTiles(const uint8_t* mat, int nx, int ny, int nz, int nxp)
+{
+    const int nxt = nxp / 64;
+    const size_t rows = static_cast<size_t>(nz) * ny;
+    std::vector<uint8_t> b(rows * nxt, 0);
+    for (size_t r = 0; r < rows; ++r) {
+        const uint8_t* row = mat + r * nx;
+        for (int x = 0; x < nx; ++x)
+            if (row[x]) {
+                b[r * nxt + x / 64] = 1;
+                x = (x / 64 + 1) * 64 - 1;   // the rest of this tile is decided
+            }
+    }
+    // x: kReach < 64, so one tile either side
+    std::vector<uint8_t> c(b.size(), 0);
+    for (size_t r = 0; r < rows; ++r)
+        for (int t = 0; t < nxt; ++t)
+            c[r * nxt + t] = b[r * nxt + t] | (t > 0 ? b[r * nxt + t - 1] : 0) | (t + 1 < nxt ? b[r * nxt + t + 1] : 0);
+    // y and z: box dilation by kReach through running counts
+    auto dilate = [&](std::vector<uint8_t>& src, int n, size_t stride, size_t outer_n, size_t outer_stride, size_t inner_n) {
+        std::vector<uint8_t> dst(src.size(), 0);
+        std::vector<int> pre(static_cast<size_t>(n) + 1);
+        for (size_t o = 0; o < outer_n; ++o)
+            for (size_t i = 0; i < inner_n; ++i) {
+                const size_t base = o * outer_stride + i;
+                pre[0] = 0;
+                for (int k = 0; k < n; ++k)
+                    pre[static_cast<size_t>(k) + 1] = pre[static_cast<size_t>(k)] + src[base + k * stride];
+                for (int k = 0; k < n; ++k) {
+                    const int lo = std::max(0, k - kReach), hi = std::min(n, k + kReach + 1);
+                    dst[base + k * stride] = pre[static_cast<size_t>(hi)] - pre[static_cast<size_t>(lo)] > 0;
+                }
+            }
+        src.swap(dst);
+    };
+    dilate(c, ny, nxt, nz, static_cast<size_t>(ny) * nxt, nxt);                              // along y
+    dilate(c, nz, static_cast<size_t>(ny) * nxt, 1, 0, static_cast<size_t>(ny) * nxt);        // along z
+    std::vector<int> tiles;
+    for (size_t i = 0; i < c.size(); ++i)
+        if (c[i])
+            tiles.push_back(static_cast<int>(i));
+    return tiles;
 }
 
 template <typename T>
@@ -772,8 +849,14 @@ struct dxg_denoiser {
     DBuf<float> params, adam_m, adam_v, grads;
     DBuf<uint16_t> wf, wd;
     DBuf<uint16_t> xin, acts, gbuf;
-    DBuf<float> feat0, y32, target, mask, features, lpart, part, part_b, wemb;
+    DBuf<float> feat0, y32, target, mask, features, lpart, part, part_b, sums, wemb;
     DBuf<double> wacc, wdose;
+    DBuf<int> tiles;
+    uint64_t tiles_generation = 0;   // world generation the tile list belongs to
+    bool tiles_zeroed = false;       // acts and y32 are zero outside the tile list ...
+    const void* zeroed_acts = nullptr;   // ... in these allocations
+    const void* zeroed_y32 = nullptr;
+    int n_tiles = 0;
     uint64_t step = 0;
     bool weights_ready = false;
     double last_ms = 0.0;
@@ -818,8 +901,14 @@ struct dxg_denoiser {
 
     // Forward pass.  With keep, activations a_0..a_8 and h_0..h_7 are kept
     // for the backward pass (17 tensors); otherwise two tensors are reused.
+    // Tiles the convolutions compute (null: all); set by apply for one pass.
+    const int* tile_list = nullptr;
+    int tile_count = 0;
+
     int forward(const Dims& d, bool keep)
     {
+        if (!tile_list)
+            tiles_zeroed = false;   // a full pass writes every tile
         const size_t nv = d.voxels(), ts = nv * kC;
         HIP_TRY(acts.reserve((keep ? 17 : 2) * ts));
         auto aT = [&](int b) { return acts.p + (keep ? static_cast<size_t>(b) : 0) * ts; };
@@ -830,6 +919,8 @@ struct dxg_denoiser {
         c.bias = layerBias(0);
         c.y = aT(0);
         c.relu = 1;
+        c.tiles = tile_list;
+        c.wave_tiles = tile_count;
         HIP_TRY(launchConv(c, d, stream));
         for (int b = 0; b < kBlocks; ++b) {
             c = ConvArgs {};
@@ -838,6 +929,8 @@ struct dxg_denoiser {
             c.bias = layerBias(2 * b + 1);
             c.y = hT(b);
             c.relu = 1;
+            c.tiles = tile_list;
+            c.wave_tiles = tile_count;
             HIP_TRY(launchConv(c, d, stream));
             c = ConvArgs {};
             c.x = hT(b);
@@ -846,6 +939,8 @@ struct dxg_denoiser {
             c.res = aT(b);
             c.y = aT(b + 1);
             c.relu = 1;
+            c.tiles = tile_list;
+            c.wave_tiles = tile_count;
             HIP_TRY(launchConv(c, d, stream));
         }
         c = ConvArgs {};
@@ -854,6 +949,8 @@ struct dxg_denoiser {
         c.bias = layerBias(kLayers - 1);
         c.y32 = y32.p;
         c.y32_channels = 1;
+        c.tiles = tile_list;
+        c.wave_tiles = tile_count;
         HIP_TRY(launchConv(c, d, stream));
         return DXG_OK;
     }
@@ -916,8 +1013,9 @@ struct dxg_denoiser {
         const int grid = wgradGrid(d);
         HIP_TRY(part.reserve(static_cast<size_t>(grid) * kLayerW));
         HIP_TRY(part_b.reserve(static_cast<size_t>(grid) * 32));
+        HIP_TRY(sums.reserve(static_cast<size_t>(kReduceSlices) * kLayerP));
 
-        HIP_TRY(launchWgrad(aT(kBlocks), gout, part.p, part_b.p, layerGrad(kLayers - 1), d, stream));
+        HIP_TRY(launchWgrad(aT(kBlocks), gout, part.p, part_b.p, sums.p, layerGrad(kLayers - 1), d, stream));
         ConvArgs c {};
         c.x = gout;
         c.w = layerWd(kLayers - 1);
@@ -925,14 +1023,14 @@ struct dxg_denoiser {
         c.y = gs;
         HIP_TRY(launchConv(c, d, stream));
         for (int b = kBlocks - 1; b >= 0; --b) {
-            HIP_TRY(launchWgrad(hT(b), gs, part.p, part_b.p, layerGrad(2 * b + 2), d, stream));
+            HIP_TRY(launchWgrad(hT(b), gs, part.p, part_b.p, sums.p, layerGrad(2 * b + 2), d, stream));
             c = ConvArgs {};
             c.x = gs;
             c.w = layerWd(2 * b + 2);
             c.gate = hT(b);
             c.y = gh;
             HIP_TRY(launchConv(c, d, stream));
-            HIP_TRY(launchWgrad(aT(b), gh, part.p, part_b.p, layerGrad(2 * b + 1), d, stream));
+            HIP_TRY(launchWgrad(aT(b), gh, part.p, part_b.p, sums.p, layerGrad(2 * b + 1), d, stream));
             c = ConvArgs {};
             c.x = gh;
             c.w = layerWd(2 * b + 1);
@@ -941,7 +1039,7 @@ struct dxg_denoiser {
             c.y = gs;
             HIP_TRY(launchConv(c, d, stream));
         }
-        HIP_TRY(launchWgrad(xin.p, gs, part.p, part_b.p, layerGrad(0), d, stream));
+        HIP_TRY(launchWgrad(xin.p, gs, part.p, part_b.p, sums.p, layerGrad(0), d, stream));
         return DXG_OK;
     }
 };
@@ -1143,6 +1241,7 @@ int dxg_denoiser_infer(dxg_denoiser* d, const float* features, const uint64_t di
     if (int rc = d->forward(dm, false); rc != DXG_OK)
         return rc;
     hipLaunchKernelGGL(finalize_kernel, dim3((nv + 255) / 256), dim3(256), 0, d->stream, d->y32.p, d->feat0.p, d->y32.p, nv);
+    d->tiles_zeroed = false;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(d->ev1, d->stream));
     HIP_TRY(hipMemcpyAsync(prediction, d->y32.p, nv * sizeof(float), hipMemcpyDeviceToHost, d->stream));
@@ -1182,16 +1281,43 @@ int dxg_denoiser_apply(dxg_denoiser* d, const dxg_world* w, double* dose_mgy)
     if (!d->weights_ready)
         if (int rc = d->prepWeights(); rc != DXG_OK)
             return rc;
-    if (int rc = d->forward(dm, false); rc != DXG_OK)
-        return rc;
     const size_t nvp = dm.voxels();
     const size_t nv = static_cast<size_t>(v.nx) * v.ny * v.nz;
-    hipLaunchKernelGGL(finalize_kernel, dim3((nvp + 255) / 256), dim3(256), 0, d->stream, d->y32.p, d->feat0.p, d->y32.p, nvp);
+    // only the tiles within the network's reach of the body (DXG_DENOISE_SKIP=0: all)
+    const char* env = std::getenv("DXG_DENOISE_SKIP");
+    const bool skip = !(env && env[0] == '0');
+    if (skip) {
+        if (d->tiles_generation != v.generation || !d->tiles.p) {
+            const std::vector<int> t = bodyTiles(v.material_host, v.nx, v.ny, v.nz, dm.nx);
+            HIP_TRY(d->tiles.reserve(std::max<size_t>(t.size(), 1)));
+            if (!t.empty())
+                HIP_TRY(hipMemcpyAsync(d->tiles.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, d->stream));
+            d->n_tiles = static_cast<int>(t.size());
+            d->tiles_generation = v.generation;
+            d->tiles_zeroed = false;
+        }
+        const size_t ts = nvp * kC;
+        HIP_TRY(d->acts.reserve(2 * ts));
+        if (!d->tiles_zeroed || d->zeroed_acts != d->acts.p || d->zeroed_y32 != d->y32.p) {
+            HIP_TRY(hipMemsetAsync(d->acts.p, 0, 2 * ts * sizeof(uint16_t), d->stream));
+            HIP_TRY(hipMemsetAsync(d->y32.p, 0, nvp * sizeof(float), d->stream));
+            d->tiles_zeroed = true;
+            d->zeroed_acts = d->acts.p;
+            d->zeroed_y32 = d->y32.p;
+        }
+        d->tile_list = d->tiles.p;
+        d->tile_count = d->n_tiles;
+    }
+    const int frc = d->n_tiles > 0 || !skip ? d->forward(dm, false) : DXG_OK;
+    d->tile_list = nullptr;
+    d->tile_count = 0;
+    if (frc != DXG_OK)
+        return frc;
     HIP_TRY(d->wdose.reserve(nv));
     const double voxel_cm3 = v.spacing[0] * v.spacing[1] * v.spacing[2];
     const double scale = dxg::kKeVToJ / (voxel_cm3 * 1e-3) * 1e3;   // keV per g/cm3 -> mGy
-    hipLaunchKernelGGL(world_dose_kernel, dim3((nv + 255) / 256), dim3(256), 0, d->stream, d->y32.p, v.nx, dm.nx, nv,
-                       d->wacc.p, scale, d->wdose.p);
+    hipLaunchKernelGGL(world_dose_kernel, dim3((nv + 255) / 256), dim3(256), 0, d->stream, d->feat0.p, d->y32.p, v.nx, dm.nx,
+                       nv, d->wacc.p, scale, d->wdose.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(d->ev1, d->stream));
     if (dose_mgy)
@@ -1245,7 +1371,9 @@ int dxg_conv3d_probe(int mode, const float* x, const float* w, const float* bias
         HIP_TRY(part.reserve(static_cast<size_t>(grid) * kLayerW));
         HIP_TRY(part_b.reserve(static_cast<size_t>(grid) * 32));
         HIP_TRY(dout.reserve(kLayerP));
-        HIP_TRY(launchWgrad(dx.p, dy.p, part.p, part_b.p, dout.p, dm, nullptr));
+        DBuf<float> sums;
+        HIP_TRY(sums.reserve(static_cast<size_t>(kReduceSlices) * kLayerP));
+        HIP_TRY(launchWgrad(dx.p, dy.p, part.p, part_b.p, sums.p, dout.p, dm, nullptr));
         HIP_TRY(hipMemcpy(out, dout.p, kLayerP * sizeof(float), hipMemcpyDeviceToHost));
         return DXG_OK;
     }

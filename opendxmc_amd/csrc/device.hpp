@@ -140,6 +140,8 @@ struct WorldView {
     uint64_t histories;
     int device;
     std::vector<float> embedding;   // per material: total mass attenuation at 60 keV over water's
+    const uint8_t* material_host;   // material index per voxel, grid order
+    uint64_t generation;            // changes whenever the world is (re)built
 };
 int worldView(const dxg_world* w, WorldView& out);
 

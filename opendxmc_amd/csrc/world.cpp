@@ -109,6 +109,7 @@ struct dxg_world {
 
     uint64_t histories = 0;
     dxg_counters counters {};
+    uint64_t generation = 0;   // world builds so far
 
     ~dxg_world()
     {
@@ -157,6 +158,8 @@ int worldView(const dxg_world* w, WorldView& out)
         out.spacing[a] = w->spacing[a];
     out.histories = w->histories;
     out.device = w->device;
+    out.material_host = w->material.data();
+    out.generation = w->generation;
     return DXG_OK;
 }
 }  // namespace dxg
@@ -376,6 +379,8 @@ int dxg_world_build(dxg_world* w)
     w->histories = 0;
     w->counters = dxg_counters {};
     w->built = true;
+    static std::atomic<uint64_t> builds { 0 };
+    w->generation = ++builds;
     return DXG_OK;
 }
 
