@@ -374,7 +374,84 @@ struct BeamTables {
     std::vector<dxg_exposure> exposures;
     std::vector<std::vector<double>> energies, weights;
     std::vector<BowtieFilter> bowties;
+
+    // A dxg_beam over these tables, valid while both objects live.
+    struct View {
+        std::vector<dxg_spectrum> spectra;
+        std::vector<dxg_bowtie> bowties;
+        dxg_beam beam {};
+    };
+    std::unique_ptr<View> view(double calibration = 0.0) const
+    {
+        auto v = std::make_unique<View>();
+        for (std::size_t i = 0; i < energies.size(); ++i)
+            v->spectra.push_back({ energies[i].data(), weights[i].data(), energies[i].size() });
+        for (const auto& b : bowties)
+            v->bowties.push_back({ b.angles().data(), b.weights().data(), b.angles().size() });
+        v->beam = { exposures.data(), exposures.size(), v->spectra.data(), v->spectra.size(),
+                    v->bowties.empty() ? nullptr : v->bowties.data(), v->bowties.size(), calibration };
+        return v;
+    }
 };
+
+namespace detail {
+// Calibration factor of a beam's tables from a closed-form entry point (DAP, air KERMA).
+inline double closedFormCalibration(const BeamTables& t, double value, int (*fn)(const dxg_beam*, double, double*))
+{
+    if (!(value > 0))
+        return 0.0;
+    const auto v = t.view();
+    double f = 0.0;
+    return fn(&v->beam, value, &f) == DXG_OK ? f : 0.0;
+}
+
+// One axial rotation at the origin of a CT tube (the CTDI phantom run).
+struct RotationTube {
+    const BeamTables* tube;   // spectrum (and bowtie) of this tube in entries 0
+    double start_angle, fov, mas;
+    bool bowtie;
+};
+
+inline double ctdiCalibration(const std::vector<RotationTube>& tubes, double step, double coll, double sdd, double ctdi_w,
+                              double diameter, std::uint64_t run_ppe)
+{
+    if (!(ctdi_w > 0))
+        return 0.0;
+    constexpr std::uint64_t kPpe = 100000;   // particles per rotation exposure
+    BeamTables rot;
+    for (const auto& rt : tubes) {
+        dxg_ct_sequential_beam p {};
+        p.scan_normal[2] = 1.0;
+        p.number_of_slices = 1;
+        p.slice_spacing_cm = 1.0;
+        p.start_angle = rt.start_angle;
+        p.step_angle = step;
+        p.collimation_cm = coll;
+        p.sdd_cm = sdd;
+        p.fov_cm = rt.fov;
+        p.particles_per_exposure = kPpe;
+        p.has_bowtie = rt.bowtie ? 1 : 0;
+        uint64_t n = 0;
+        if (dxg_ct_sequential_exposures(&p, nullptr, 0, &n) != DXG_OK)
+            return 0.0;
+        std::vector<dxg_exposure> ex(n);
+        dxg_ct_sequential_exposures(&p, ex.data(), n, &n);
+        for (auto& e : ex) {
+            e.spectrum_index = static_cast<uint32_t>(rot.energies.size());
+            e.bowtie_index = rt.bowtie ? static_cast<int32_t>(rot.bowties.size()) : -1;
+            e.weight *= rt.mas;
+            rot.exposures.push_back(e);
+        }
+        rot.energies.push_back(rt.tube->energies[0]);
+        rot.weights.push_back(rt.tube->weights[0]);
+        if (rt.bowtie)
+            rot.bowties.push_back(rt.tube->bowties[0]);
+    }
+    const auto v = rot.view();
+    double f = 0.0;
+    return dxg_calibration_ctdi(&v->beam, ctdi_w, diameter, coll, run_ppe, 12345, &f, nullptr) == DXG_OK ? f : 0.0;
+}
+}  // namespace detail
 
 class BeamBase {
 public:
@@ -389,6 +466,9 @@ public:
         t.exposures = exposures();
         return t;
     }
+    // mGy per (dose summed over a run's histories) for the beam's stated output
+    // (DAP, air KERMA, CTDI); 0 leaves the run uncalibrated.
+    virtual double calibrationFactor() const { return 0.0; }
     virtual ~BeamBase() = default;
 
 protected:
@@ -468,6 +548,7 @@ public:
         t.weights.push_back({ 1.0 });
         return t;
     }
+    double calibrationFactor() const override { return detail::closedFormCalibration(tables(), m_airKerma, dxg_calibration_air_kerma); }
 
 private:
     std::array<double, 3> m_pos, m_dir;
@@ -514,6 +595,8 @@ public:
         p.particles_per_exposure = m_ppe;
         return collect(p, dxg_dx_exposures);
     }
+
+    double calibrationFactor() const override { return detail::closedFormCalibration(tables(), m_dap, dxg_calibration_dap); }
 
 private:
     std::array<double, 3> m_pos;
@@ -565,6 +648,13 @@ public:
         if (!m_bowtie.empty())
             t.bowties.push_back(m_bowtie);
         return t;
+    }
+    // CTDIvol of a spiral = CTDIw / pitch
+    double calibrationFactor() const override
+    {
+        const BeamTables a = tables();
+        return detail::ctdiCalibration({ { &a, m_startAngle, m_fov, 1.0, !m_bowtie.empty() } }, m_stepAngle, m_coll, m_sdd,
+                                       m_ctdivol * m_pitch, m_ctdiDiameter, m_ppe);
     }
 
 protected:
@@ -661,6 +751,27 @@ public:
         return t;
     }
 
+    double calibrationFactor() const override
+    {
+        BeamTables a, b;
+        auto [ea, wa] = this->m_tube.getSpecter();
+        auto [eb, wb] = m_tubeB.getSpecter();
+        a.energies.push_back(ea);
+        a.weights.push_back(wa);
+        b.energies.push_back(eb);
+        b.weights.push_back(wb);
+        const bool bow = !this->m_bowtie.empty() || !m_bowtieB.empty();
+        if (bow) {
+            const BowtieFilter& bA = this->m_bowtie.empty() ? m_bowtieB : this->m_bowtie;
+            a.bowties.push_back(bA);
+            b.bowties.push_back(m_bowtieB.empty() ? bA : m_bowtieB);
+        }
+        return detail::ctdiCalibration({ { &a, this->m_startAngle, this->m_fov, m_masA, bow },
+                                         { &b, this->m_startAngle + m_offset, m_fovB, m_masB, bow } },
+                                       this->m_stepAngle, this->m_coll, this->m_sdd, this->m_ctdivol * this->m_pitch,
+                                       this->m_ctdiDiameter, this->m_ppe);
+    }
+
 private:
     Tube m_tubeB;
     double m_fovB = 33, m_offset = 90 * DEG_TO_RAD(), m_masA = 1, m_masB = 1;
@@ -725,6 +836,12 @@ public:
             t.bowties.push_back(m_bowtie);
         return t;
     }
+    double calibrationFactor() const override
+    {
+        const BeamTables a = tables();
+        return detail::ctdiCalibration({ { &a, m_startAngle, m_fov, 1.0, !m_bowtie.empty() } }, m_stepAngle, m_coll, m_sdd,
+                                       m_ctdiw, m_ctdiDiameter, m_ppe);
+    }
 
 private:
     std::array<double, 3> m_pos, m_normal;
@@ -778,6 +895,8 @@ public:
         return collect(p, dxg_cbct_exposures);
     }
 
+    double calibrationFactor() const override { return detail::closedFormCalibration(tables(), m_dap, dxg_calibration_dap); }
+
 private:
     std::array<double, 3> m_iso, m_axis;
     double m_sdd = 120, m_start = -90 * DEG_TO_RAD(), m_stop = 90 * DEG_TO_RAD(), m_step = 1 * DEG_TO_RAD();
@@ -803,14 +922,8 @@ public:
     bool operator()(W& world, const B& beam, TransportProgress* progress = nullptr, bool useBeamCalibration = true)
     {
         const BeamTables t = beam.tables();
-        std::vector<dxg_spectrum> spectra;
-        for (std::size_t i = 0; i < t.energies.size(); ++i)
-            spectra.push_back({ t.energies[i].data(), t.weights[i].data(), t.energies[i].size() });
-        std::vector<dxg_bowtie> bowties;
-        for (const auto& b : t.bowties)
-            bowties.push_back({ b.angles().data(), b.weights().data(), b.angles().size() });
-        dxg_beam db { t.exposures.data(), t.exposures.size(), spectra.data(), spectra.size(), bowties.data(), bowties.size() };
-        const int rc = dxg_transport_run(world.gridHandle(), &db, progress ? progress->handle() : nullptr,
+        const auto v = t.view(useBeamCalibration ? beam.calibrationFactor() : 0.0);
+        const int rc = dxg_transport_run(world.gridHandle(), &v->beam, progress ? progress->handle() : nullptr,
                                          useBeamCalibration ? 1 : 0, m_seed++);
         world.scoresChanged();
         if (rc != DXG_OK && rc != DXG_ERR_CANCELLED && progress)

@@ -85,6 +85,10 @@ def test_worker_harness_matches_python_path(tmp_path):
     beam.setStepAngleDeg(15)
     beam.setBowtieFilter(bow)
     beam.setNumberOfParticlesPerExposure(ppe)
+    # the C++ beam keeps a stored CTDIvol of 1 mGy (32 cm phantom) and worker<C>
+    # runs with useBeamCalibration: calibrate the Python run the same way
+    beam.setCTDIvol(1.0)
+    beam.setCTDIdiameter(32.0)
     assert dx.Transport(seed=seed)(world, beam)
     d, v, c = grid.doseScored()
     air = ph.material == 0
