@@ -83,7 +83,7 @@ struct dxg_batch {
     hipGraphExec_t exec = nullptr;
     // launch shape the graph was captured with
     int g_blocks = 0, g_level = -1, g_n = -1;
-    bool g_lut = false;
+    bool g_lut = false, g_cells = false;
     size_t g_lds = 0;
 
     void dropGraph()
@@ -202,16 +202,22 @@ int dxg_batch_run(dxg_batch* b, double* kernel_ms)
         level = ps[static_cast<size_t>(i)].level;
         work[static_cast<size_t>(i)] = static_cast<double>(total);
     }
+    // one kernel instance serves the batch: macro cells only when every world has them
+    bool cells = true;
+    for (const auto& p : ps)
+        cells = cells && p.cell_ratio != nullptr;
     size_t lds = 0;
     for (auto& p : ps) {
         p.lds_lut = lut ? 1 : 0;
+        if (!cells)
+            p.cell_ratio = nullptr;
         lds = std::max(lds, dxg::transportLdsBytes(p));
     }
     HIP_TRY(hipDeviceSynchronize());   // the worlds' table uploads are on their own streams
 
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
-    const int blocks = cus * dxg::transportBatchBlocksPerCU(lut, level, lds);
+    const int blocks = cus * dxg::transportBatchBlocksPerCU(lut, level, cells, lds);
     // initial block -> instance assignment proportional to each instance's histories
     std::vector<int> start(static_cast<size_t>(blocks));
     const double tot = std::accumulate(work.begin(), work.end(), 0.0);
@@ -249,11 +255,12 @@ int dxg_batch_run(dxg_batch* b, double* kernel_ms)
 
     // the reset and transport kernels read everything from d_params, so one
     // captured graph serves every run with the same launch shape
-    if (b->exec && (b->g_blocks != blocks || b->g_level != level || b->g_lut != lut || b->g_lds != lds || b->g_n != n))
+    if (b->exec && (b->g_blocks != blocks || b->g_level != level || b->g_lut != lut || b->g_cells != cells || b->g_lds != lds
+                    || b->g_n != n))
         b->dropGraph();
     if (!b->exec) {
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        dxg::launchTransportBatch(b->d_params, n, b->d_start, lut, level, lds, blocks, b->stream);
+        dxg::launchTransportBatch(b->d_params, n, b->d_start, lut, level, cells, lds, blocks, b->stream);
         hipGraph_t g = nullptr;
         const hipError_t launch_err = hipGetLastError();
         const hipError_t cap_err = hipStreamEndCapture(b->stream, &g);
@@ -268,6 +275,7 @@ int dxg_batch_run(dxg_batch* b, double* kernel_ms)
         b->g_blocks = blocks;
         b->g_level = level;
         b->g_lut = lut;
+        b->g_cells = cells;
         b->g_lds = lds;
         b->g_n = n;
     }

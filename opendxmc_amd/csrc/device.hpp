@@ -123,12 +123,12 @@ void launchReset(unsigned long long* pool, DevCounters* counters, hipStream_t s)
 int transportBlockSize();
 void launchCopy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 // Resident blocks per CU of the transport kernel at the given LDS size.
-int transportBlocksPerCU(bool lds_lut, int level, size_t lds_bytes);
+int transportBlocksPerCU(bool lds_lut, int level, bool cells, size_t lds_bytes);
 // Many instances in one persistent launch (pool and counters of every
 // instance reset first); params and start_inst are device arrays.
-void launchTransportBatch(const DevParams* params, int n_inst, const int* start_inst, bool lds_lut, int level, size_t lds,
-                          int blocks, hipStream_t s);
-int transportBatchBlocksPerCU(bool lds_lut, int level, size_t lds_bytes);
+void launchTransportBatch(const DevParams* params, int n_inst, const int* start_inst, bool lds_lut, int level, bool cells,
+                          size_t lds, int blocks, hipStream_t s);
+int transportBatchBlocksPerCU(bool lds_lut, int level, bool cells, size_t lds_bytes);
 
 // Device-resident state of a built world, for kernels outside the transport
 // (the dose denoiser reads the tally and the voxel words in place).

@@ -234,7 +234,7 @@ __device__ inline const float* stageTables(const DevParams& P, uint32_t* lds)
 // The history loop of one wavefront for one instance: claims batches of
 // [P.h_begin, P.h_end) from P.pool until none is left and every lane is done,
 // then adds the wavefront's counters and energy sums to P.counters.
-template <bool LDS_LUT, int LEVEL>
+template <bool LDS_LUT, int LEVEL, bool CELLS>
 __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_maj, const float* mu_tab, const uint32_t* s_blob,
                                              double* e_wave)
 {
@@ -360,7 +360,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         maj = s_maj[ej];
                         state = kTrack;
                         tau = -1.0f;
-                        if (P.cell_ratio)
+                        if (CELLS)
                             cellc = cellCoord(px, P.bmin[0], P.cell_inv[0], P.cnx, dx)
                                     | (cellCoord(py, P.bmin[1], P.cell_inv[1], P.cny, dy) << 10)
                                     | (cellCoord(pz, P.bmin[2], P.cell_inv[2], P.cnz, dz) << 20);
@@ -544,7 +544,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
 
         // ------------------------------------------------------------ step
         bool uncollided_exit = false;
-        if (P.cell_ratio) {
+        if (CELLS) {
             // Woodcock tracking with macro-cell majorants: a flight carries an
             // optical depth tau through cells of majorant ratio[c] * maj.  In
             // one iteration a tracking lane walks up to kMaxCross cells
@@ -666,7 +666,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
     }
 }
 
-template <bool LDS_LUT, int LEVEL>
+template <bool LDS_LUT, int LEVEL, bool CELLS>
 __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
 {
     extern __shared__ uint32_t lds[];
@@ -680,7 +680,7 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
     __syncthreads();
     const float* s_maj = reinterpret_cast<const float*>(lds);
     const uint32_t* s_blob = lds + kLutNd + (LDS_LUT ? P.n_mat * kLutNd : 0);
-    runHistories<LDS_LUT, LEVEL>(P, s_maj, mu_tab, s_blob, s_energy[wave]);
+    runHistories<LDS_LUT, LEVEL, CELLS>(P, s_maj, mu_tab, s_blob, s_energy[wave]);
 }
 
 // Many instances (world + beam) in one persistent launch.  A block starts on
@@ -688,7 +688,7 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
 // all of the block's wavefronts are done, the block moves to an instance with
 // histories left (scanning from a block-dependent offset), re-stages its LDS
 // tables and continues, until no instance has work.
-template <bool LDS_LUT, int LEVEL>
+template <bool LDS_LUT, int LEVEL, bool CELLS>
 __global__ void __launch_bounds__(kBlock) transport_batch_kernel(const DevParams* __restrict__ params, int n_inst,
                                                                  const int* __restrict__ start_inst)
 {
@@ -707,7 +707,7 @@ __global__ void __launch_bounds__(kBlock) transport_batch_kernel(const DevParams
         __syncthreads();
         const float* s_maj = reinterpret_cast<const float*>(lds);
         const uint32_t* s_blob = lds + kLutNd + (LDS_LUT ? P.n_mat * kLutNd : 0);
-        runHistories<LDS_LUT, LEVEL>(P, s_maj, mu_tab, s_blob, s_energy[wave]);
+        runHistories<LDS_LUT, LEVEL, CELLS>(P, s_maj, mu_tab, s_blob, s_energy[wave]);
         __syncthreads();   // every wavefront is done with this instance's tables
         if (threadIdx.x == 0) {
             int next = -1;
@@ -804,15 +804,17 @@ size_t transportLdsBytes(const DevParams& p)
 
 namespace {
 
-// The kernel instance for (LDS table placement, correction level).
-const void* transportInstance(bool lds_lut, int level)
+// The kernel instance for (LDS table placement, correction level, macro cells).
+const void* transportInstance(bool lds_lut, int level, bool cells)
 {
     using K = void (*)(DevParams);
-    static const K table[2][3] = {
-        { &transport_kernel<false, 0>, &transport_kernel<false, 1>, &transport_kernel<false, 2> },
-        { &transport_kernel<true, 0>, &transport_kernel<true, 1>, &transport_kernel<true, 2> },
+    static const K table[2][2][3] = {
+        { { &transport_kernel<false, 0, false>, &transport_kernel<false, 1, false>, &transport_kernel<false, 2, false> },
+          { &transport_kernel<true, 0, false>, &transport_kernel<true, 1, false>, &transport_kernel<true, 2, false> } },
+        { { &transport_kernel<false, 0, true>, &transport_kernel<false, 1, true>, &transport_kernel<false, 2, true> },
+          { &transport_kernel<true, 0, true>, &transport_kernel<true, 1, true>, &transport_kernel<true, 2, true> } },
     };
-    return reinterpret_cast<const void*>(table[lds_lut ? 1 : 0][level < 0 ? 0 : (level > 2 ? 2 : level)]);
+    return reinterpret_cast<const void*>(table[cells ? 1 : 0][lds_lut ? 1 : 0][level < 0 ? 0 : (level > 2 ? 2 : level)]);
 }
 
 }  // namespace
@@ -821,44 +823,46 @@ void launchTransport(const DevParams& p, int blocks, int threads, hipStream_t s)
 {
     const size_t lds = transportLdsBytes(p);
     void* args[] = { const_cast<DevParams*>(&p) };
-    (void)hipLaunchKernel(transportInstance(p.lds_lut != 0, p.level), dim3(static_cast<unsigned>(blocks)),
+    (void)hipLaunchKernel(transportInstance(p.lds_lut != 0, p.level, p.cell_ratio != nullptr), dim3(static_cast<unsigned>(blocks)),
                           dim3(static_cast<unsigned>(threads)), args, lds, s);
 }
 
 namespace {
 
-const void* batchInstance(bool lds_lut, int level)
+const void* batchInstance(bool lds_lut, int level, bool cells)
 {
     using K = void (*)(const DevParams*, int, const int*);
-    static const K table[2][3] = {
-        { &transport_batch_kernel<false, 0>, &transport_batch_kernel<false, 1>, &transport_batch_kernel<false, 2> },
-        { &transport_batch_kernel<true, 0>, &transport_batch_kernel<true, 1>, &transport_batch_kernel<true, 2> },
+    static const K table[2][2][3] = {
+        { { &transport_batch_kernel<false, 0, false>, &transport_batch_kernel<false, 1, false>, &transport_batch_kernel<false, 2, false> },
+          { &transport_batch_kernel<true, 0, false>, &transport_batch_kernel<true, 1, false>, &transport_batch_kernel<true, 2, false> } },
+        { { &transport_batch_kernel<false, 0, true>, &transport_batch_kernel<false, 1, true>, &transport_batch_kernel<false, 2, true> },
+          { &transport_batch_kernel<true, 0, true>, &transport_batch_kernel<true, 1, true>, &transport_batch_kernel<true, 2, true> } },
     };
-    return reinterpret_cast<const void*>(table[lds_lut ? 1 : 0][level < 0 ? 0 : (level > 2 ? 2 : level)]);
+    return reinterpret_cast<const void*>(table[cells ? 1 : 0][lds_lut ? 1 : 0][level < 0 ? 0 : (level > 2 ? 2 : level)]);
 }
 
 }  // namespace
 
-void launchTransportBatch(const DevParams* params, int n_inst, const int* start_inst, bool lds_lut, int level, size_t lds,
-                          int blocks, hipStream_t s)
+void launchTransportBatch(const DevParams* params, int n_inst, const int* start_inst, bool lds_lut, int level, bool cells,
+                          size_t lds, int blocks, hipStream_t s)
 {
     hipLaunchKernelGGL(batch_reset_kernel, dim3(static_cast<unsigned>((n_inst + 63) / 64)), dim3(64), 0, s, params, n_inst);
     void* args[] = { const_cast<DevParams**>(&params), &n_inst, const_cast<int**>(&start_inst) };
-    (void)hipLaunchKernel(batchInstance(lds_lut, level), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), args, lds, s);
+    (void)hipLaunchKernel(batchInstance(lds_lut, level, cells), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), args, lds, s);
 }
 
-int transportBatchBlocksPerCU(bool lds_lut, int level, size_t lds_bytes)
+int transportBatchBlocksPerCU(bool lds_lut, int level, bool cells, size_t lds_bytes)
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batchInstance(lds_lut, level), kBlock, lds_bytes) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batchInstance(lds_lut, level, cells), kBlock, lds_bytes) != hipSuccess)
         return 1;
     return n > 0 ? n : 1;
 }
 
-int transportBlocksPerCU(bool lds_lut, int level, size_t lds_bytes)
+int transportBlocksPerCU(bool lds_lut, int level, bool cells, size_t lds_bytes)
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transportInstance(lds_lut, level), kBlock, lds_bytes) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transportInstance(lds_lut, level, cells), kBlock, lds_bytes) != hipSuccess)
         return 1;
     return n > 0 ? n : 1;
 }

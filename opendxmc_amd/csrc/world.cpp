@@ -616,7 +616,7 @@ int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, w->device));
     const int block = dxg::transportBlockSize();
-    const int per_cu = dxg::transportBlocksPerCU(p.lds_lut != 0, p.level, lds_bytes);
+    const int per_cu = dxg::transportBlocksPerCU(p.lds_lut != 0, p.level, p.cell_ratio != nullptr, lds_bytes);
     int blocks = cus * per_cu;
     if (const char* env = std::getenv("DXG_BLOCKS"))
         blocks = std::max(1, std::atoi(env));
