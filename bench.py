@@ -20,6 +20,10 @@ beside them.
 
 CPU baseline: the oracle (FP64 C++ restatement, std::thread) on a bounded
 sample of the same workload, on this host's cores.
+
+--workload ct-segmentation / organ-dose-table time the image pipelines on a
+device-resident 1024 x 512 x 512 volume instead (voxels/s; 17 B of HBM traffic
+per voxel each; CPU baseline: the oracle's numpy restatement, one thread).
 """
 from __future__ import annotations
 
@@ -128,6 +132,126 @@ WORKLOADS = {
 }
 
 
+PIPE_PROFILE = ROOT / "profiles" / "ct_segmentation_and_organ_dose_table_1024x512x512.md"
+PIPE_DIMS = (1024, 512, 512)
+PIPE_KERNELS = {"ct-segmentation": "segment_pairs_kernel", "organ-dose-table": "organ_table_kernel"}
+
+
+def pipeline_traffic(kernel: str):
+    """HBM bytes per launch of a pipeline kernel from the committed counter
+    passes; None when not profiled."""
+    try:
+        for line in PIPE_PROFILE.read_text().splitlines():
+            if line.startswith(f"traffic_bytes_per_launch_{kernel}:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def pipeline_inputs(workload: str, n: int):
+    rng = np.random.default_rng(1)
+    if workload == "ct-segmentation":
+        return {"hu": rng.uniform(-1100.0, 2500.0, n)}
+    runs = rng.integers(1, 400, n // 100 + 2)
+    return {"organ": np.repeat((np.arange(runs.size) % 60).astype(np.uint8), runs)[:n],
+            "dose": rng.uniform(0.0, 1.0, n), "density": rng.uniform(0.0, 2.0, n)}
+
+
+def pipeline_cpu_baseline(workload: str, tables, seconds_target: float = 10.0):
+    """The oracle's numpy restatement, one thread, repeated over a 2^24-voxel
+    sample of the same inputs for about seconds_target."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    n = 1 << 24
+    x = pipeline_inputs(workload, n)
+    sep = np.array(tables.separators[: tables.n_materials - 1])
+    att = np.array(tables.attenuation[: tables.n_materials])
+    done, t0 = 0, time.perf_counter()
+    while True:
+        if workload == "ct-segmentation":
+            oracle.ct_segment(sep, att, tables.slope, tables.offset, x["hu"])
+        else:
+            oracle.organ_dose_table(x["organ"], x["dose"], x["density"], (0.1, 0.1, 0.1))
+        done += n
+        dt = time.perf_counter() - t0
+        if dt >= seconds_target:
+            break
+    return {"value": done / dt, "unit": "voxels/s", "cores": 1, "kind": "port",
+            "sample": f"{done // n} passes over 2^24 voxels of the same inputs, numpy oracle, 1 thread, {dt:.1f} s"}
+
+
+def pipeline_bench(args) -> int:
+    """One step = one pass of the segmentation or dose-table kernel over a
+    device-resident 1024 x 512 x 512 volume."""
+    from opendxmc_amd import pipelines as P
+    from opendxmc_amd.devmem import DeviceBuffer
+
+    n = int(np.prod(PIPE_DIMS))
+    tables = P.CTSegmentationPipeline().tables()
+    x = pipeline_inputs(args.workload, n)
+    if args.workload == "ct-segmentation":
+        bufs = [DeviceBuffer.from_array(x["hu"]), DeviceBuffer(n), DeviceBuffer(8 * n)]
+
+        def step():
+            P.segment_device(tables, bufs[0].ptr, n, bufs[1].ptr, bufs[2].ptr)
+        which, dtype = 0, "f64"
+        description = ("CT number -> material / density segmentation (ctsegmentationpipeline.cpp), 5 materials at "
+                       "120 kV 0.9 mm Al, CT numbers uniform in [-1100, 2500)")
+    else:
+        bufs = [DeviceBuffer.from_array(x["organ"]), DeviceBuffer.from_array(x["dose"]),
+                DeviceBuffer.from_array(x["density"])]
+
+        def step():
+            P.organ_dose_table_device(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, n, (0.1, 0.1, 0.1))
+        which, dtype = 1, "f64"
+        description = ("organ dose table (dosetablepipeline.cpp), 60 organs in runs of 1..400 voxels, FP64 dose "
+                       "and density")
+    del x
+    for _ in range(args.warmup):
+        step()
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        kernel_ms += P.last_ms(which)
+    wall = time.perf_counter() - t0
+    per_launch_ms = kernel_ms / max(args.steps, 1)
+    achieved = 17.0 * n / (per_launch_ms * 1e-3) / 1e9
+    result = {
+        "metric": "voxels/s",
+        "value": n * args.steps / wall,
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": {"workload": f"{description}, {PIPE_DIMS[0]}x{PIPE_DIMS[1]}x{PIPE_DIMS[2]} voxels resident in HBM"},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": pipeline_traffic(PIPE_KERNELS[args.workload]),
+            "traffic_source": str(PIPE_PROFILE.relative_to(ROOT)) + " (FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch)",
+            "kernel": PIPE_KERNELS[args.workload],
+            "kernel_ms_per_launch": per_launch_ms,
+            "bytes_per_voxel": 17,
+        },
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = pipeline_cpu_baseline(args.workload, tables)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def cpu_baseline(ph, beam, seconds_target: float = 15.0):
     """Time the oracle on a bounded sample: every exposure, fewer particles."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -168,7 +292,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="ct-spiral-256")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(PIPE_KERNELS), default="ct-spiral-256")
     ap.add_argument("--histories", type=float, default=None, help="histories per step (default per workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -176,6 +300,9 @@ def main() -> int:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (histories shard trivially "
               f"across GPUs, but this build measures one MI355X)", file=sys.stderr)
         return 2
+
+    if args.workload in PIPE_KERNELS:
+        return pipeline_bench(args)
 
     import opendxmc_amd as dx
 

@@ -121,6 +121,29 @@ EXPOSURE_DTYPE = np.dtype([
 assert EXPOSURE_DTYPE.itemsize == C.sizeof(Exposure) == 112
 
 
+SEG_MAX_MATERIALS = 128
+
+
+class CTSegmentation(C.Structure):
+    """dxg_ct_segmentation (include/dxmc_pipelines.h)."""
+    _fields_ = [("n_materials", C.c_uint32), ("reserved", C.c_uint32),
+                ("hu", C.c_double * SEG_MAX_MATERIALS), ("separators", C.c_double * (SEG_MAX_MATERIALS - 1)),
+                ("attenuation", C.c_double * SEG_MAX_MATERIALS),
+                ("water_attenuation", C.c_double), ("air_attenuation", C.c_double),
+                ("water_density", C.c_double), ("air_density", C.c_double),
+                ("slope", C.c_double), ("offset", C.c_double)]
+
+
+class OrganDose(C.Structure):
+    """dxg_organ_dose (include/dxmc_pipelines.h)."""
+    _fields_ = [("n_voxels", u64), ("volume_cm3", C.c_double), ("mass_g", C.c_double), ("dose", C.c_double)]
+
+
+ORGAN_DOSE_DTYPE = np.dtype([("n_voxels", np.uint64), ("volume_cm3", np.float64), ("mass_g", np.float64),
+                             ("dose", np.float64)])
+assert ORGAN_DOSE_DTYPE.itemsize == C.sizeof(OrganDose) == 32
+
+
 class NativeError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[{code}] {msg}")
@@ -131,8 +154,8 @@ _lib = None
 
 PF = C.POINTER(C.c_float)
 
-# (name, restype, argtypes) for every entry point of include/dxmc_hip.h and
-# include/dxmc_denoise.h
+# (name, restype, argtypes) for every entry point of include/dxmc_hip.h,
+# include/dxmc_beams.h, include/dxmc_denoise.h and include/dxmc_pipelines.h
 SIGNATURES = [
     ("dxg_last_error", C.c_char_p, []),
     ("dxg_world_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
@@ -186,6 +209,14 @@ SIGNATURES = [
     ("dxg_denoiser_apply", C.c_int, [C.c_void_p, C.c_void_p, PD]),
     ("dxg_conv3d_probe", C.c_int, [C.c_int, PF, PF, PF, u64, C.POINTER(u64), PF]),
     ("dxg_mfma_probe", C.c_int, [PF, PF, PF]),
+    ("dxg_ct_segmentation_tables", C.c_int, [C.POINTER(Tube), C.POINTER(Material), PD, u64, C.POINTER(Material),
+                                             C.c_double, C.POINTER(Material), C.c_double, C.POINTER(CTSegmentation)]),
+    ("dxg_ct_segment", C.c_int, [C.POINTER(CTSegmentation), PD, u64, C.POINTER(C.c_uint8), PD]),
+    ("dxg_ct_segment_device", C.c_int, [C.POINTER(CTSegmentation), C.c_void_p, u64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("dxg_organ_dose_table", C.c_int, [C.POINTER(C.c_uint8), PD, PD, u64, PD, C.POINTER(OrganDose)]),
+    ("dxg_organ_dose_table_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, u64, PD, C.POINTER(OrganDose),
+                                              C.c_void_p]),
+    ("dxg_pipeline_last_ms", C.c_double, [C.c_int]),
 ]
 
 

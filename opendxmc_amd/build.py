@@ -33,7 +33,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-command-line-argument",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
-SOURCES = ["physics.cpp", "world.cpp", "beams.cpp", "calibration.cpp", "batch.cpp", "transport.hip", "denoise.hip"]
+SOURCES = ["physics.cpp", "world.cpp", "beams.cpp", "calibration.cpp", "batch.cpp", "transport.hip", "denoise.hip", "pipelines.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -10,6 +10,12 @@ factor applied to the transport kernel's counters.
 
 times the denoiser: inference on a random volume (nz ny nx) and training steps
 on a batch of four 64^3 patches; each convolution is 55,296 FLOP per voxel.
+
+    python -m opendxmc_amd.profiling pipelines --dims 1024 512 512 --iters 5
+
+times CT segmentation and the organ dose table on device-resident volumes
+(17 bytes of HBM traffic per voxel each): organ indices in runs of 1..400
+voxels (mean 200), then one organ drawn at random per voxel (the worst case).
 """
 from __future__ import annotations
 
@@ -57,9 +63,38 @@ def denoise_probe(dims, iters: int, train_steps: int, batch: int = 4) -> None:
                   f" loss {loss:.4f}", flush=True)
 
 
+def pipelines_probe(dims, iters: int) -> None:
+    from . import pipelines as P
+    from .devmem import DeviceBuffer
+
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(1)
+    tables = P.CTSegmentationPipeline().tables()
+    hu = DeviceBuffer.from_array(rng.uniform(-1100.0, 2500.0, n))
+    mat, dens = DeviceBuffer(n), DeviceBuffer(8 * n)
+    for i in range(iters):
+        P.segment_device(tables, hu.ptr, n, mat.ptr, dens.ptr)
+        ms = P.last_ms(0)
+        print(f"segment {tuple(dims)} #{i}: {ms:.3f} ms, {17 * n / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
+    hu.free()
+    dose = DeviceBuffer.from_array(rng.uniform(0.0, 1.0, n))
+    runs = rng.integers(1, 400, n // 100 + 2)
+    organs = {
+        "runs": np.repeat((np.arange(runs.size) % 60).astype(np.uint8), runs)[:n],
+        "random": rng.integers(0, 256, n, dtype=np.uint8),
+    }
+    sp = (0.1, 0.1, 0.1)
+    for kind, organ in organs.items():
+        mat.upload(organ)
+        for i in range(iters):
+            P.organ_dose_table_device(mat.ptr, dose.ptr, dens.ptr, n, sp)
+            ms = P.last_ms(1)
+            print(f"dose table {kind} {tuple(dims)} #{i}: {ms:.3f} ms, {17 * n / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "denoise"])
+    ap.add_argument("what", choices=["copy", "denoise", "pipelines"])
     ap.add_argument("--mib", type=int, default=256)
     ap.add_argument("--iters", type=int, default=5)
     ap.add_argument("--dims", type=int, nargs=3, default=(400, 512, 512))
@@ -69,6 +104,8 @@ def main() -> None:
         ms = copy_probe(a.mib, a.iters)
         gbs = 2 * (a.mib << 20) / (ms * 1e-3) / 1e9
         print(f"copy {a.mib} MiB x {a.iters}: {ms:.3f} ms per launch, {gbs:.0f} GB/s (read+write)")
+    elif a.what == "pipelines":
+        pipelines_probe(a.dims, a.iters)
     else:
         denoise_probe(a.dims, a.iters, a.train_steps)
 

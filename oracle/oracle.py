@@ -196,3 +196,81 @@ def transport(dims, spacing, density, material_arr, materials: list[dict[int, fl
     if rc != 0:
         raise RuntimeError(f"oracle_transport failed ({rc}): {_err()}")
     return se, se2, nev, cnt.as_dict()
+
+
+# ------------------------------------------------------------- image pipelines
+
+def mean_ct_numbers(energy, weight, materials, densities, water, water_density, air, air_density) -> dict:
+    """getMeanCTNumbers (ctsegmentationpipeline.cpp:61-115): spectrum averages
+    of the per-energy CT number 1000 (mu_m - mu_w) / (mu_w - mu_a), with
+    mu = (photoelectric + incoherent + coherent) x density, and of mu/rho;
+    plus the constants of the per-voxel density map (:146-149)."""
+    energy = np.asarray(energy, dtype=np.float64)
+    weight = np.asarray(weight, dtype=np.float64)
+
+    def mu(comp):
+        a = attenuation(comp, energy)
+        return a[:, 0] + a[:, 1] + a[:, 2]
+
+    mw, ma = mu(water), mu(air)
+    uw, ua = mw * water_density, ma * air_density
+    hu, att = [], []
+    for comp, rho in zip(materials, densities):
+        mm = mu(comp)
+        hu.append(1000.0 * float(np.sum(weight * (mm * rho - uw) / (uw - ua))))
+        att.append(float(np.sum(weight * mm)))
+    aw, aa = float(np.sum(weight * mw)), float(np.sum(weight * ma))
+    hu = np.array(hu)
+    return {
+        "hu": hu,
+        "separators": (hu[:-1] + hu[1:]) / 2.0,
+        "attenuation": np.array(att),
+        "water_attenuation": aw,
+        "air_attenuation": aa,
+        "slope": aw * water_density - aa * air_density,
+        "offset": aw * water_density,
+    }
+
+
+def ct_segment(separators, attenuation_, slope: float, offset: float, hu) -> tuple[np.ndarray, np.ndarray]:
+    """The per-voxel map of ctsegmentationpipeline.cpp:134-152: material = the
+    first i with hu < separators[i] (the last material when none), density =
+    std::max(((slope hu) / 1000 + offset) / attenuation[material], 0.0)."""
+    hu = np.asarray(hu, dtype=np.float64)
+    sep = np.asarray(separators, dtype=np.float64)
+    idx = np.full(hu.shape, len(sep), dtype=np.uint8)
+    for i in range(len(sep) - 1, -1, -1):
+        idx[hu < sep[i]] = i
+    v = (slope * hu / 1000.0 + offset) / np.asarray(attenuation_, dtype=np.float64)[idx]
+    return idx, np.where(v < 0.0, 0.0, v)
+
+
+ORGAN_DOSE_DTYPE = np.dtype([("n_voxels", np.uint64), ("volume_cm3", np.float64), ("mass_g", np.float64),
+                             ("dose", np.float64)])
+
+
+def organ_dose_table(organ, dose, density, spacing) -> np.ndarray:
+    """DoseTablePipeline::updateImageData (dosetablepipeline.cpp:62-86) for all
+    256 organ indices: voxel count, count x voxel volume, voxel volume x sum of
+    density, and (sum of dose x (voxel volume x density)) / mass.  Each organ's
+    sums are numpy pairwise sums over its voxels."""
+    organ = np.asarray(organ, dtype=np.uint8).ravel()
+    density = np.asarray(density, dtype=np.float64).ravel()
+    dose = np.asarray(dose, dtype=np.float64).ravel()
+    vv = float(spacing[0]) * float(spacing[1]) * float(spacing[2])
+    energy = dose * (vv * density)
+    order = np.argsort(organ, kind="stable")
+    counts = np.bincount(organ, minlength=256)
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    e_sorted, d_sorted = energy[order], density[order]
+    out = np.zeros(256, dtype=ORGAN_DOSE_DTYPE)
+    for i in range(256):
+        c = int(counts[i])
+        if c == 0:
+            continue
+        s = slice(starts[i], starts[i + 1])
+        mass = vv * np.sum(d_sorted[s])
+        with np.errstate(divide="ignore", invalid="ignore"):
+            dose_i = np.sum(e_sorted[s]) / mass      # 0/0 -> NaN, as in C++
+        out[i] = (c, c * vv, mass, dose_i)
+    return out
