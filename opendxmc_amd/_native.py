@@ -155,7 +155,7 @@ _lib = None
 PF = C.POINTER(C.c_float)
 
 # (name, restype, argtypes) for every entry point of include/dxmc_hip.h,
-# include/dxmc_beams.h, include/dxmc_denoise.h and include/dxmc_pipelines.h
+# include/dxmc_beams.h, include/dxmc_denoise.h, include/dxmc_pipelines.h and include/dxmc_h5.h
 SIGNATURES = [
     ("dxg_last_error", C.c_char_p, []),
     ("dxg_world_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
@@ -217,6 +217,22 @@ SIGNATURES = [
     ("dxg_organ_dose_table_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, u64, PD, C.POINTER(OrganDose),
                                               C.c_void_p]),
     ("dxg_pipeline_last_ms", C.c_double, [C.c_int]),
+    ("dxg_h5_create", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("dxg_h5_open", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("dxg_h5_close", C.c_int, [C.c_void_p]),
+    ("dxg_h5_create_group", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("dxg_h5_kind", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("dxg_h5_write_array", C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(u64), C.c_void_p, C.c_int]),
+    ("dxg_h5_write_strings", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_char_p), u64]),
+    ("dxg_h5_write_attribute", C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int, C.c_void_p, u64, C.c_int]),
+    ("dxg_h5_dataset_shape", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(u64), C.POINTER(C.c_int),
+                                       C.POINTER(u64)]),
+    ("dxg_h5_read_array", C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_void_p, u64]),
+    ("dxg_h5_read_strings", C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, u64, C.POINTER(u64), C.POINTER(u64)]),
+    ("dxg_h5_read_attribute", C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int, C.c_void_p, u64, C.POINTER(u64),
+                                        C.POINTER(C.c_int)]),
+    ("dxg_h5_list", C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, u64, C.POINTER(u64), C.POINTER(u64)]),
+    ("dxg_h5_last_deflate_ms", C.c_double, [C.POINTER(C.c_int)]),
 ]
 
 

@@ -33,7 +33,12 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-command-line-argument",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
-SOURCES = ["physics.cpp", "world.cpp", "beams.cpp", "calibration.cpp", "batch.cpp", "transport.hip", "denoise.hip", "pipelines.hip"]
+SOURCES = ["physics.cpp", "world.cpp", "beams.cpp", "calibration.cpp", "batch.cpp", "transport.hip", "denoise.hip", "pipelines.hip",
+           "h5io.cpp"]
+# h5io.cpp compiles against the HDF5 headers only; the library itself is
+# opened with dlopen when a project file is first touched.
+HDF5_INCLUDE = Path(os.environ.get("DXG_HDF5_INCLUDE", "/opt/conda/include"))
+EXTRA_FLAGS = {"h5io.cpp": ["-isystem", str(HDF5_INCLUDE)]}
 
 
 def _run(cmd: list[str]) -> None:
@@ -62,10 +67,10 @@ def build_library() -> Path:
             continue
         obj = BUILD / (src.replace(".", "_") + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        _run([HIPCC, *HIP_FLAGS, *lang, "-c", str(path), "-o", str(obj)])
+        _run([HIPCC, *HIP_FLAGS, *EXTRA_FLAGS.get(src, []), *lang, "-c", str(path), "-o", str(obj)])
         objs.append(str(obj))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp), "-ldl"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp), "-ldl", "-lz", "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 
