@@ -323,6 +323,7 @@ def main() -> int:
         transport(world, beam, None, True)
         print(f"warmup {i} done", file=sys.stderr, flush=True)
     steps = bytes_alg = kernel_ms = launches = hist = deposits = crossings = 0
+    phase = [0, 0, 0]
     t0 = time.perf_counter()
     for i in range(args.steps):
         transport.seed = i + 1
@@ -334,6 +335,7 @@ def main() -> int:
         kernel_ms += c["kernel_ms"]
         launches += c["kernel_launches"]
         hist += c["histories"]
+        phase = [a + b for a, b in zip(phase, c["phase_cycles"])]
         print(f"step {i}: {c['histories']} histories, kernel {c['kernel_ms']:.1f} ms", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     bytes_alg = 5.0 * steps + 24.0 * deposits
@@ -379,6 +381,8 @@ def main() -> int:
             "bytes_per_history": bytes_alg / max(hist, 1),
         },
     }
+    if sum(phase) > 0:   # libraries built with DXG_PHASE_CLOCKS
+        result["config"]["phase_cycle_share"] = {k: v / sum(phase) for k, v in zip(("refill", "interact", "step"), phase)}
     t2 = committed_time_to_2pct()
     if t2 is not None:
         result["time_to_2pct"] = t2

@@ -107,6 +107,8 @@ typedef struct dxg_counters {
     uint64_t kernel_launches;
     uint64_t uncollided;       /* histories leaving the grid with no real interaction */
     uint64_t cell_crossings;   /* macro-cell boundary crossings (majorant changes, no voxel read) */
+    uint64_t phase_cycles[3];  /* wavefront clock cycles spent refilling, interacting and stepping
+                                  (libraries built with DXG_PHASE_CLOCKS; zero otherwise) */
 } dxg_counters;
 
 const char* dxg_last_error(void);

@@ -7,11 +7,13 @@ CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libdxmc_hip.so"
+# DXG_LIBRARY points at another build of the library (same C-ABI)
+LIB_PATH = Path(os.environ.get("DXG_LIBRARY") or Path(__file__).resolve().parent / "libdxmc_hip.so")
 
 DXG_OK = 0
 DXG_ERR_ARGUMENT = -1
@@ -66,11 +68,13 @@ class Counters(C.Structure):
         ("histories", u64), ("steps", u64), ("real_events", u64), ("deposits", u64),
         ("emitted_energy", C.c_double), ("escaped_energy", C.c_double),
         ("deposited_energy", C.c_double), ("kernel_ms", C.c_double), ("kernel_launches", u64),
-        ("uncollided", u64), ("cell_crossings", u64),
+        ("uncollided", u64), ("cell_crossings", u64), ("phase_cycles", u64 * 3),
     ]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["phase_cycles"] = list(self.phase_cycles)
+        return d
 
 
 class Tube(C.Structure):

@@ -34,6 +34,7 @@ struct DevCounters {
     double deposited;
     unsigned long long uncollided;   // histories that left the grid with no real interaction
     unsigned long long crossings;    // macro-cell boundary crossings (no voxel read)
+    unsigned long long phase_cycles[3];   // wavefront clock cycles in refill, interact, step (DXG_PHASE_CLOCKS builds)
 };
 
 // Packed voxel word: the FP32 density with its 8 low mantissa bits replaced
@@ -89,9 +90,19 @@ struct DevParams {
     int cnx, cny, cnz;
     float cell_len[3];
     float cell_inv[3];
+    // the same bounds as bytes, cell_ratio[c] <= cell_q[c] * cell_qscale,
+    // staged in LDS after the beam blob when cell_lds (a read per crossing
+    // from LDS instead of a dependent global load)
+    const uint8_t* cell_q;
+    float cell_qscale;
+    int cell_lds;
 
-    // tallies: per voxel {sum w dE, sum (w dE)^2, events (u64 bits), unused}
+    // tallies of voxel i: sum w dE at tally[i * tally_vs], sum (w dE)^2 one
+    // tally_ks further, events (u64 bits) two further -- three arrays
+    // (tally_vs 1, tally_ks = voxels) or four doubles per voxel (4, 1)
     double* tally;
+    unsigned long long tally_ks;
+    unsigned int tally_vs;
     float dose_scale;   // beam calibration applied to every deposit (1: none)
 
     // beam
@@ -99,6 +110,7 @@ struct DevParams {
     const unsigned long long* ex_start;   // first beam-history of each exposure, n_ex entries
     int n_ex;
     unsigned long long ppe;               // particles per exposure when uniform, else 0
+    double inv_ppe;                       // 1 / ppe (exposure index without 64-bit division)
     const uint32_t* blob;                 // BeamBlob words
     BeamBlob bb;
 
@@ -134,7 +146,9 @@ int transportBatchBlocksPerCU(bool lds_lut, int level, bool cells, size_t lds_by
 // (the dose denoiser reads the tally and the voxel words in place).
 struct WorldView {
     const uint32_t* voxels;    // packed words (packVoxel)
-    const double* tally;       // 4 doubles per voxel: sum w dE, sum (w dE)^2, events, unused
+    const double* tally;       // sum w dE, sum (w dE)^2, events (u64 bits) of voxel i at i * tally_vs + k * tally_ks
+    unsigned long long tally_ks;
+    unsigned int tally_vs;
     int nx, ny, nz;
     double spacing[3];         // cm
     uint64_t histories;

@@ -50,13 +50,36 @@ constexpr float kInvHc = 1.0f / 12.398419843320026f;
 constexpr float kInvMeC2 = 1.0f / 510.99895f;
 constexpr float kCutoff = 1.0f;
 constexpr float kTwoPi = 6.283185307179586f;
-constexpr int kBlock = 256;
+// Work-group size and the occupancy target of the transport kernels
+// (compile-time).  Six wavefronts per SIMD is what the LDS tables allow
+// (about 26 KB per work-group of 256); the target keeps the macro-cell
+// instances at 80 VGPRs (81-82 otherwise: five wavefronts).
+#ifndef DXG_TRANSPORT_BLOCK
+#define DXG_TRANSPORT_BLOCK 256
+#endif
+#ifndef DXG_TRANSPORT_WAVES_PER_EU
+#define DXG_TRANSPORT_WAVES_PER_EU 6
+#endif
+constexpr int kBlock = DXG_TRANSPORT_BLOCK;
+#define DXG_TRANSPORT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(DXG_TRANSPORT_WAVES_PER_EU)))
 constexpr int kMaxCross = 8;    // macro-cell crossings a lane walks per iteration
 constexpr int kIaShellsd = 5;    // impulse-approximation shells per material
 constexpr int kIaInvNd = 2048;   // unit Compton profile inverse-CDF nodes
 constexpr int kFlMaxd = 12;      // fluorescing elements per material
 
 enum LaneState : int { kDead = 0, kTrack = 1, kInteract = 2, kDone = 3 };
+
+// The hardware's approximations (v_log_f32, v_exp_f32, v_rcp_f32, v_sqrt_f32;
+// about 1 ulp) for sampling.  The library calls expand to ~10-20 instructions
+// each for denormal handling and correct rounding, which sampling does not
+// need: a flight length -ln(u)/mu cost 22 instructions of a ~70-instruction
+// Woodcock step.  Arguments are normal, positive floats (u >= 2^-24).
+constexpr float kLn2f = 0.693147181f;
+constexpr float kLog2ef = 1.442695041f;
+__device__ inline float hwLog(float x) { return __builtin_amdgcn_logf(x) * kLn2f; }
+__device__ inline float hwExp(float x) { return __builtin_amdgcn_exp2f(x * kLog2ef); }
+__device__ inline float hwRcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float hwSqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 struct Rng {
     uint32_t s0, s1, s2, s3;
@@ -96,25 +119,27 @@ __device__ inline float rngU(Rng& r)
     return static_cast<float>((out >> 8) + 1u) * (1.0f / 16777216.0f);
 }
 
-// Slab entry of a ray into the grid box.  Identical operation order to the
-// oracle's FP32 traversal (bit-exact check).
+// Slab entry of a ray into the grid box.  EXACT: identical operation order to
+// the oracle's FP32 traversal (bit-exact check); else the reciprocals are the
+// hardware's (sampling only).
+template <bool EXACT = true>
 __device__ inline bool boxEntry(const DevParams& P, float& px, float& py, float& pz, float dx, float dy, float dz)
 {
     float t0 = 0.0f, t1 = 3.0e38f;
     {
-        const float id = 1.0f / dx;
+        const float id = EXACT ? 1.0f / dx : hwRcp(dx);
         const float ta = (P.bmin[0] - px) * id, tb = (P.bmax[0] - px) * id;
         t0 = fmaxf(t0, fminf(ta, tb));
         t1 = fminf(t1, fmaxf(ta, tb));
     }
     {
-        const float id = 1.0f / dy;
+        const float id = EXACT ? 1.0f / dy : hwRcp(dy);
         const float ta = (P.bmin[1] - py) * id, tb = (P.bmax[1] - py) * id;
         t0 = fmaxf(t0, fminf(ta, tb));
         t1 = fminf(t1, fmaxf(ta, tb));
     }
     {
-        const float id = 1.0f / dz;
+        const float id = EXACT ? 1.0f / dz : hwRcp(dz);
         const float ta = (P.bmin[2] - pz) * id, tb = (P.bmax[2] - pz) * id;
         t0 = fmaxf(t0, fminf(ta, tb));
         t1 = fminf(t1, fmaxf(ta, tb));
@@ -141,7 +166,7 @@ __device__ inline int voxelOf(const DevParams& P, float px, float py, float pz)
 
 __device__ inline void rotate(float& dx, float& dy, float& dz, float cost, float phi)
 {
-    const float sint = sqrtf(fmaxf(0.0f, 1.0f - cost * cost));
+    const float sint = hwSqrt(fmaxf(0.0f, 1.0f - cost * cost));
     float sp, cp;
     __sincosf(phi, &sp, &cp);
     float nx, ny, nz;
@@ -150,8 +175,8 @@ __device__ inline void rotate(float& dx, float& dy, float& dz, float cost, float
         ny = sint * sp;
         nz = dz > 0.0f ? cost : -cost;
     } else {
-        const float t = sqrtf(1.0f - dz * dz);
-        const float it = 1.0f / t;
+        const float t = hwSqrt(1.0f - dz * dz);
+        const float it = hwRcp(t);
         nx = dx * cost + sint * (dx * dz * cp - dy * sp) * it;
         ny = dy * cost + sint * (dy * dz * cp + dx * sp) * it;
         nz = dz * cost - t * sint * cp;
@@ -165,10 +190,10 @@ __device__ inline void rotate(float& dx, float& dy, float& dz, float cost, float
 __device__ inline void tallyAdd(const DevParams& P, int idx, float e)
 {
     const double de = static_cast<double>(e) * static_cast<double>(P.dose_scale);
-    double* t = P.tally + 4 * static_cast<size_t>(idx);
+    double* t = P.tally + static_cast<size_t>(idx) * P.tally_vs;
     atomicAdd(t, de);
-    atomicAdd(t + 1, de * de);
-    atomicAdd(reinterpret_cast<unsigned long long*>(t + 2), 1ull);
+    atomicAdd(t + P.tally_ks, de * de);
+    atomicAdd(reinterpret_cast<unsigned long long*>(t + 2 * P.tally_ks), 1ull);
 }
 
 __device__ inline float lerpTab(const float* tab, int j, float t)
@@ -193,13 +218,13 @@ __device__ inline int cellCoord(float p, float bmin, float inv, int n, float d)
 }
 
 // Path length to the exit face of cell c along one axis (+inf when the
-// direction is parallel to the faces; never negative).
-__device__ inline float exitDistance(float p, float d, float bmin, float len, int c)
+// direction is parallel to the faces; never negative); inv_d = 1 / d.
+__device__ inline float exitDistance(float p, float d, float inv_d, float bmin, float len, int c)
 {
     if (d == 0.0f)
         return __builtin_inff();
     const float face = bmin + static_cast<float>(c + (d > 0.0f ? 1 : 0)) * len;
-    return fmaxf(__fdividef(face - p, d), 0.0f);
+    return fmaxf((face - p) * inv_d, 0.0f);
 }
 
 // Energy bin of the attenuation tables and the bin's majorant.
@@ -228,6 +253,12 @@ __device__ inline const float* stageTables(const DevParams& P, uint32_t* lds)
             s_mu[i] = P.mu_tot[i];
     for (int i = threadIdx.x; i < P.bb.words; i += blockDim.x)
         s_blob[i] = P.blob[i];
+    if (P.cell_ratio && P.cell_lds) {
+        const int words = (P.cnx * P.cny * P.cnz + 3) / 4;
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(P.cell_q);
+        for (int i = threadIdx.x; i < words; i += blockDim.x)
+            s_blob[P.bb.words + i] = src[i];
+    }
     return LDS_LUT ? s_mu : P.mu_tot;
 }
 
@@ -245,6 +276,12 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
     int state = kDead;
     float tau = -1.0f;   // optical depth left in the current flight (macro cells; < 0: draw a new one)
     int cellc = 0;       // macro cell of the photon: cx | cy << 10 | cz << 20
+    int mat = 0;         // material of the voxel at the photon (read with its word in the step)
+    const uint8_t* s_cellq = reinterpret_cast<const uint8_t*>(s_blob + P.bb.words);
+    // majorant ratio of macro cell c (LDS bytes or the global float table)
+    auto cellRatio = [&](int c) {
+        return P.cell_lds ? static_cast<float>(s_cellq[c]) * P.cell_qscale : P.cell_ratio[c];
+    };
     Rng rng { 1, 2, 3, 4 };
     bool collided = false;
 
@@ -252,8 +289,19 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
     unsigned long long n_steps = 0, n_real = 0, n_dep = 0, n_hist = 0, n_unc = 0, n_cross = 0;
     // wavefront-private batch of pool histories [cur, end), claimed P.batch at a time
     unsigned long long cur = 0, end = 0;
+#ifdef DXG_PHASE_CLOCKS
+    // wavefront clock cycles per phase (refill, interact, step): diagnostics builds only
+    unsigned long long phase_cycles[3] = { 0, 0, 0 };
+    unsigned long long t_iter = 0;
+#define DXG_PHASE_END(k) phase_cycles[k] += __builtin_amdgcn_s_memtime() - t_iter
+#else
+#define DXG_PHASE_END(k)
+#endif
 
     for (;;) {
+#ifdef DXG_PHASE_CLOCKS
+        t_iter = __builtin_amdgcn_s_memtime();
+#endif
         const unsigned long long m_dead = __ballot(state == kDead);
         const unsigned long long m_int = __ballot(state == kInteract);
         const unsigned long long m_track = __ballot(state == kTrack);
@@ -291,8 +339,14 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     rngSeed(rng, P.seed, h + P.rng_offset);
                     // exposure containing beam history h
                     int ei;
-                    if (P.ppe) {
-                        ei = static_cast<int>(min(h / P.ppe, static_cast<unsigned long long>(P.n_ex - 1)));
+                    if (P.ppe) {   // h / ppe from the double reciprocal, corrected by one either way
+                        long long q = static_cast<long long>(static_cast<double>(h) * P.inv_ppe);
+                        const unsigned long long lo = static_cast<unsigned long long>(q) * P.ppe;
+                        if (lo > h)
+                            --q;
+                        else if (lo + P.ppe <= h)
+                            ++q;
+                        ei = static_cast<int>(min(static_cast<unsigned long long>(q), static_cast<unsigned long long>(P.n_ex - 1)));
                     } else {
                         int lo = 0, hi = P.n_ex - 1;
                         while (lo < hi) {
@@ -309,7 +363,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     const float ax = ex.half_angle[0] * (2.0f * rngU(rng) - 1.0f);
                     const float ay = ex.half_angle[1] * (2.0f * rngU(rng) - 1.0f);
                     const float sx = __sinf(ax), sy = __sinf(ay);
-                    const float sz = sqrtf(fmaxf(0.0f, 1.0f - sx * sx - sy * sy));
+                    const float sz = hwSqrt(fmaxf(0.0f, 1.0f - sx * sx - sy * sy));
                     dx = ex.c0[0] * sx + ex.c1[0] * sy + ex.axis[0] * sz;
                     dy = ex.c0[1] * sx + ex.c1[1] * sy + ex.axis[1] * sz;
                     dz = ex.c0[2] * sx + ex.c1[2] * sy + ex.axis[2] * sz;
@@ -353,7 +407,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     }
                     const double we = static_cast<double>(w) * static_cast<double>(E);
                     atomicAdd(&e_wave[0], we);
-                    if (boxEntry(P, px, py, pz, dx, dy, dz)) {
+                    if (boxEntry<false>(P, px, py, pz, dx, dy, dz)) {
                         const float f = (E - kLutE0f) * kLutInvDEf;
                         ej = min(static_cast<int>(f), kLutNd - 2);
                         et = f - static_cast<float>(ej);
@@ -370,6 +424,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 }
             }
             n_hist += __popcll(__ballot(started));
+            DXG_PHASE_END(0);
             continue;
         }
 
@@ -381,7 +436,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 collided = true;
                 state = kTrack;
                 tau = -1.0f;   // the photon stays in its macro cell; its next flight is drawn anew
-                const int m = static_cast<int>(P.voxels[vi] & 0xFFu);
+                const int m = mat;
                 const float4 a = P.mu_parts[m * kLutNd + ej];
                 const float4 b = P.mu_parts[m * kLutNd + ej + 1];
                 const float c_pe = __fmaf_rn(et, b.x - a.x, a.x);
@@ -415,7 +470,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         // the fluorescence photon leaves the absorption point isotropically
                         E = e_out;
                         const float ct = 2.0f * rngU(rng) - 1.0f;
-                        const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+                        const float st = hwSqrt(fmaxf(0.0f, 1.0f - ct * ct));
                         float sp, cp;
                         __sincosf(kTwoPi * rngU(rng), &sp, &cp);
                         dx = st * cp;
@@ -430,8 +485,8 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     // level 2 adds the impulse approximation: a shell drawn by electron
                     // fraction, p_z from its Compton profile, the Doppler-shifted E'
                     const float k = E * kInvMeC2;
-                    const float e0 = 1.0f / (1.0f + 2.0f * k);
-                    const float a1 = __logf(1.0f + 2.0f * k);
+                    const float e0 = hwRcp(1.0f + 2.0f * k);
+                    const float a1 = hwLog(1.0f + 2.0f * k);
                     const float a2 = 0.5f * (1.0f - e0 * e0);
                     float cost, enew;
                     for (;;) {
@@ -439,10 +494,10 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         for (;;) {
                             const float r1 = rngU(rng), r2 = rngU(rng), r3 = rngU(rng);
                             if (r1 * (a1 + a2) < a1)
-                                eps = __expf(-a1 * r2);
+                                eps = hwExp(-a1 * r2);
                             else
-                                eps = sqrtf(e0 * e0 + (1.0f - e0 * e0) * r2);
-                            const float t = (1.0f - eps) / (k * eps);
+                                eps = hwSqrt(e0 * e0 + (1.0f - e0 * e0) * r2);
+                            const float t = (1.0f - eps) * hwRcp(k * eps);
                             const float sin2 = t * (2.0f - t);
                             const float g = 1.0f - eps * sin2 / (1.0f + eps * eps);
                             if (r3 > g)
@@ -450,7 +505,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                             cost = 1.0f - t;
                             if (LEVEL == 0)
                                 break;
-                            const float x = E * kInvHc * sqrtf(fmaxf(0.0f, 0.5f * (1.0f - cost)));
+                            const float x = E * kInvHc * hwSqrt(fmaxf(0.0f, 0.5f * (1.0f - cost)));
                             const float fx = x * (static_cast<float>(kSfNd - 1) / kXMaxf);
                             float sv = 1.0f;
                             if (fx < static_cast<float>(kSfNd - 1)) {
@@ -484,7 +539,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         const float Q = (K - cost) * (K - cost) + (1.0f - cost * cost) * (1.0f - p2);
                         if (!(den > 0.0f))
                             continue;
-                        enew = k * (h + pz * sqrtf(Q)) / den * (1.0f / kInvMeC2);
+                        enew = k * (h + pz * hwSqrt(Q)) * hwRcp(den) * (1.0f / kInvMeC2);
                         if (enew > 0.0f && E - enew >= s.y)
                             break;
                     }
@@ -511,9 +566,9 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     float cost;
                     if (LEVEL == 0) {
                         const float q = 4.0f - 8.0f * rngU(rng);
-                        const float aa = -0.5f * q + sqrtf(0.25f * q * q + 1.0f);
+                        const float aa = -0.5f * q + hwSqrt(0.25f * q * q + 1.0f);
                         const float u = cbrtf(aa);
-                        cost = fminf(1.0f, fmaxf(-1.0f, u - 1.0f / u));
+                        cost = fminf(1.0f, fmaxf(-1.0f, u - hwRcp(u)));
                     } else {
                         const float xm = E * kInvHc;
                         const float umax = xm * xm;
@@ -539,6 +594,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 atomicAdd(&e_wave[2], e_scored);
             n_real += n_int;
             n_dep += __popcll(__ballot(deps > 0)) + __popcll(__ballot(deps > 1));
+            DXG_PHASE_END(1);
             continue;
         }
 
@@ -556,15 +612,16 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
             int crossed = 0;
             if (state == kTrack) {
                 if (tau < 0.0f)
-                    tau = -__logf(rngU(rng));
+                    tau = -hwLog(rngU(rng));
                 int cx = cellc & 1023, cy = (cellc >> 10) & 1023, cz = cellc >> 20;
-                float mu_c = P.cell_ratio[cx + P.cnx * (cy + P.cny * cz)] * maj;
-                float tmx = exitDistance(px, dx, P.bmin[0], P.cell_len[0], cx);
-                float tmy = exitDistance(py, dy, P.bmin[1], P.cell_len[1], cy);
-                float tmz = exitDistance(pz, dz, P.bmin[2], P.cell_len[2], cz);
-                const float ddx = dx != 0.0f ? __fdividef(P.cell_len[0], fabsf(dx)) : __builtin_inff();
-                const float ddy = dy != 0.0f ? __fdividef(P.cell_len[1], fabsf(dy)) : __builtin_inff();
-                const float ddz = dz != 0.0f ? __fdividef(P.cell_len[2], fabsf(dz)) : __builtin_inff();
+                float mu_c = cellRatio(cx + P.cnx * (cy + P.cny * cz)) * maj;
+                const float idx = hwRcp(dx), idy = hwRcp(dy), idz = hwRcp(dz);
+                float tmx = exitDistance(px, dx, idx, P.bmin[0], P.cell_len[0], cx);
+                float tmy = exitDistance(py, dy, idy, P.bmin[1], P.cell_len[1], cy);
+                float tmz = exitDistance(pz, dz, idz, P.bmin[2], P.cell_len[2], cz);
+                const float ddx = dx != 0.0f ? P.cell_len[0] * fabsf(idx) : __builtin_inff();
+                const float ddy = dy != 0.0f ? P.cell_len[1] * fabsf(idy) : __builtin_inff();
+                const float ddz = dz != 0.0f ? P.cell_len[2] * fabsf(idz) : __builtin_inff();
                 float t0 = 0.0f;   // path walked in this iteration up to the current cell's entry face
                 float s = 0.0f;    // path to move the photon by at the end of the iteration
                 bool out = false;
@@ -573,7 +630,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     const float seg = t1 - t0;
                     if (tau < mu_c * seg) {
                         tentative = true;
-                        s = t0 + __fdividef(tau, mu_c);
+                        s = t0 + tau * hwRcp(mu_c);
                         tau = -1.0f;
                         break;
                     }
@@ -600,7 +657,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         s = t0;
                         break;
                     }
-                    mu_c = P.cell_ratio[cx + P.cnx * (cy + P.cny * cz)] * maj;
+                    mu_c = cellRatio(cx + P.cnx * (cy + P.cny * cz)) * maj;
                 }
                 px = __fmaf_rn(s, dx, px);
                 py = __fmaf_rn(s, dy, py);
@@ -618,6 +675,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     const int m = static_cast<int>(word & 0xFFu);
                     const float rho = asFloat(word & 0xFFFFFF00u);
                     const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
+                    mat = m;
                     if (!(rngU(rng) * mu_c > mu))
                         state = kInteract;
                 }
@@ -626,10 +684,11 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
             for (int b = 0; b < 4; ++b)   // crossed <= kMaxCross < 16: sum it bit by bit
                 n_cross += static_cast<unsigned long long>(__popcll(__ballot((crossed >> b) & 1))) << b;
             n_unc += __popcll(__ballot(uncollided_exit));
+            DXG_PHASE_END(2);
             continue;
         }
         if (state == kTrack) {
-            const float s = __fdividef(-__logf(rngU(rng)), maj);
+            const float s = -hwLog(rngU(rng)) * hwRcp(maj);
             px = __fmaf_rn(s, dx, px);
             py = __fmaf_rn(s, dy, py);
             pz = __fmaf_rn(s, dz, pz);
@@ -643,13 +702,16 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 const int m = static_cast<int>(word & 0xFFu);
                 const float rho = asFloat(word & 0xFFFFFF00u);
                 const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
+                mat = m;
                 if (!(rngU(rng) * maj > mu))
                     state = kInteract;   // real collision; virtual ones keep tracking
             }
         }
         n_steps += n_track;
         n_unc += __popcll(__ballot(uncollided_exit));
+        DXG_PHASE_END(2);
     }
+#undef DXG_PHASE_END
 
     // ---- one atomic per wavefront and counter
     const double e_emit = e_wave[0], e_esc = e_wave[1], e_dep = e_wave[2];
@@ -660,6 +722,10 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
         atomicAdd(&P.counters->histories, n_hist);
         atomicAdd(&P.counters->uncollided, n_unc);
         atomicAdd(&P.counters->crossings, n_cross);
+#ifdef DXG_PHASE_CLOCKS
+        for (int k = 0; k < 3; ++k)
+            atomicAdd(&P.counters->phase_cycles[k], phase_cycles[k]);
+#endif
         atomicAdd(&P.counters->emitted, e_emit);
         atomicAdd(&P.counters->escaped, e_esc);
         atomicAdd(&P.counters->deposited, e_dep);
@@ -667,7 +733,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
 }
 
 template <bool LDS_LUT, int LEVEL, bool CELLS>
-__global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
+__global__ void __launch_bounds__(kBlock) DXG_TRANSPORT_OCCUPANCY transport_kernel(const DevParams P)
 {
     extern __shared__ uint32_t lds[];
     const float* mu_tab = stageTables<LDS_LUT>(P, lds);
@@ -689,7 +755,7 @@ __global__ void __launch_bounds__(kBlock) transport_kernel(const DevParams P)
 // histories left (scanning from a block-dependent offset), re-stages its LDS
 // tables and continues, until no instance has work.
 template <bool LDS_LUT, int LEVEL, bool CELLS>
-__global__ void __launch_bounds__(kBlock) transport_batch_kernel(const DevParams* __restrict__ params, int n_inst,
+__global__ void __launch_bounds__(kBlock) DXG_TRANSPORT_OCCUPANCY transport_batch_kernel(const DevParams* __restrict__ params, int n_inst,
                                                                  const int* __restrict__ start_inst)
 {
     extern __shared__ uint32_t lds[];
@@ -743,6 +809,8 @@ __global__ void batch_reset_kernel(const DevParams* __restrict__ params, int n_i
     c->deposited = 0;
     c->uncollided = 0;
     c->crossings = 0;
+    for (int k = 0; k < 3; ++k)
+        c->phase_cycles[k] = 0;
 }
 
 __global__ void traverse_kernel(const DevParams P, uint64_t n_rays, const float* __restrict__ pos, const float* __restrict__ dir,
@@ -787,6 +855,8 @@ __global__ void reset_kernel(unsigned long long* pool, DevCounters* c)
         c->deposited = 0;
         c->uncollided = 0;
         c->crossings = 0;
+        for (int k = 0; k < 3; ++k)
+            c->phase_cycles[k] = 0;
     }
 }
 
@@ -798,8 +868,9 @@ __global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__
 
 size_t transportLdsBytes(const DevParams& p)
 {
+    const size_t cells = p.cell_ratio && p.cell_lds ? (static_cast<size_t>(p.cnx) * p.cny * p.cnz + 3) / 4 : 0;
     return sizeof(uint32_t) * (static_cast<size_t>(kLutNd) + (p.lds_lut ? static_cast<size_t>(p.n_mat) * kLutNd : 0)
-                               + static_cast<size_t>(p.bb.words));
+                               + static_cast<size_t>(p.bb.words) + cells);
 }
 
 namespace {
