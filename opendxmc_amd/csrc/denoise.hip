@@ -144,12 +144,21 @@ __device__ inline u32x4 dpp4(u32x4 old, u32x4 src)
     return r;
 }
 
+// Wavefronts per convolution work-group; the layer's weights (55 KB of LDS)
+// allow two work-groups per CU.  Four (two wavefronts per SIMD) measured
+// faster than six (three per SIMD): 539 against 451 TFLOP/s over the 18
+// layers of a 128x256x256 volume.
+#ifndef DXG_CONV_WAVES
+#define DXG_CONV_WAVES 4
+#endif
+constexpr int kConvWaves = DXG_CONV_WAVES;
+
 template <bool SHIFT>
-__global__ __launch_bounds__(256) void conv3d_kernel(ConvArgs a)
+__global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
 {
     __shared__ u32x4 wl[kTaps * 32 * 4];   // 55,296 B
     const int tid = threadIdx.x;
-    for (int i = tid; i < kTaps * 32 * 4; i += 256) {
+    for (int i = tid; i < kTaps * 32 * 4; i += 64 * kConvWaves) {
         const int row = i >> 2;
         wl[wSlot(row >> 5, row & 31, i & 3)] = reinterpret_cast<const u32x4*>(a.w)[i];
     }
@@ -165,13 +174,13 @@ __global__ __launch_bounds__(256) void conv3d_kernel(ConvArgs a)
     // XCDs, so map them to contiguous slabs of rows per XCD.
     const int g = gridDim.x;
     const int logical = (blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
-    const int wg_tiles = (a.wave_tiles + 3) >> 2;
+    const int wg_tiles = (a.wave_tiles + kConvWaves - 1) / kConvWaves;
     const int begin = logical * a.chunk;
     const int end = min(begin + a.chunk, wg_tiles);
     const int nxs = a.nx >> 6;
 
     for (int wt = begin; wt < end; ++wt) {
-        const int k = wt * 4 + wave;
+        const int k = wt * kConvWaves + wave;
         if (k >= a.wave_tiles)
             break;
         const int t = a.tiles ? a.tiles[k] : k;
@@ -187,7 +196,70 @@ __global__ __launch_bounds__(256) void conv3d_kernel(ConvArgs a)
             for (int j = 0; j < 4; ++j)
                 acc[m][j] = f32x4 { 0, 0, 0, 0 };
 
-        for (int dz = -1; dz <= 1; ++dz) {
+        if (SHIFT) {
+            // The nine (dz, dy) input rows in turn, each loaded once.  Tile j,
+            // lane n holds voxel x0 + 4n + j (the four tiles interleave), so the
+            // x-neighbour operands of tiles 1-3 / 0-2 are simply the
+            // neighbouring tile's registers; only tile 0 at dx = -1 and tile 3
+            // at dx = +1 take their voxel from the next lane of the same 16-lane
+            // row (one DPP move per register), with the halo voxel x0 - 1 /
+            // x0 + 64 in the lane that falls off the row.  The next row's loads
+            // are issued before this row's 24 MFMAs (two buffers, alternating).
+            const u32x4 zero = { 0, 0, 0, 0 };
+            u32x4 buf[2][4], lbuf[2], rbuf[2];
+            bool have[2] = { false, false };
+            auto loadRow = [&](int q, int b) {
+                const int zz = z + q / 3 - 1, yy = y + q % 3 - 1;
+                have[b] = static_cast<unsigned>(zz) < static_cast<unsigned>(a.nz) && static_cast<unsigned>(yy) < static_cast<unsigned>(a.ny);
+                if (!have[b])
+                    return;
+                const size_t rowb = (static_cast<size_t>(n * a.nz + zz) * a.ny + yy) * a.nx;
+                const uint16_t* base = a.x + (rowb + x0 + 4 * lr) * kC + 8 * lq;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    buf[b][j] = load16(base + j * kC);
+                lbuf[b] = zero;
+                rbuf[b] = zero;
+                if (lr == 0 && x0 > 0)
+                    lbuf[b] = load16(a.x + (rowb + x0 - 1) * kC + 8 * lq);
+                if (lr == 15 && x0 + 64 < a.nx)
+                    rbuf[b] = load16(a.x + (rowb + x0 + 64) * kC + 8 * lq);
+            };
+            loadRow(0, 0);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const int b = q & 1;
+                if (q < 8)
+                    loadRow(q + 1, b ^ 1);
+                if (!have[b])
+                    continue;
+                const u32x4(&c)[4] = buf[b];
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    bf16x8 bop[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (dx == 0)
+                            bop[j] = asOperand(c[j]);
+                        else if (dx < 0)
+                            bop[j] = asOperand(j > 0 ? c[j - 1] : dpp4<kRowShr1>(lbuf[b], c[3]));   // lane n <- lane n-1
+                        else
+                            bop[j] = asOperand(j < 3 ? c[j + 1] : dpp4<kRowShl1>(rbuf[b], c[0]));   // lane n <- lane n+1
+                    }
+                    const int tap = q * 3 + dx + 1;
+                    bf16x8 aop[2];
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+                        aop[m] = asOperand(wl[wSlot(tap, 16 * m + lr, lq)]);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop[m], bop[j], acc[m][j], 0, 0, 0);
+                }
+            }
+        }
+        for (int dz = -1; dz <= 1 && !SHIFT; ++dz) {
             const int zz = z + dz;
             if (static_cast<unsigned>(zz) >= static_cast<unsigned>(a.nz))
                 continue;
@@ -197,49 +269,6 @@ __global__ __launch_bounds__(256) void conv3d_kernel(ConvArgs a)
                     continue;
                 const size_t rowb = (static_cast<size_t>(n * a.nz + zz) * a.ny + yy) * a.nx;
                 const int tapb = (dz + 1) * 9 + (dy + 1) * 3;
-                if (SHIFT) {
-                    // One load per voxel tile; the dx = -1 / +1 operands are the
-                    // same row moved by one lane inside each 16-lane DPP row,
-                    // with the voxel that crosses a tile edge taken from the
-                    // neighbouring tile (or the halo voxel at x0 - 1 / x0 + 64).
-                    u32x4 c[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        c[j] = load16(a.x + (rowb + x0 + 16 * j + lr) * kC + 8 * lq);
-                    const u32x4 zero = { 0, 0, 0, 0 };
-                    u32x4 left = zero, right = zero;
-                    if (lr == 15 && x0 > 0)
-                        left = load16(a.x + (rowb + x0 - 1) * kC + 8 * lq);
-                    if (lr == 0 && x0 + 64 < a.nx)
-                        right = load16(a.x + (rowb + x0 + 64) * kC + 8 * lq);
-#pragma unroll
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        bf16x8 bop[4];
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            if (dx == 0) {
-                                bop[j] = asOperand(c[j]);
-                            } else if (dx < 0) {
-                                const u32x4 edge = dpp4<kRowShl15>(zero, j == 0 ? left : c[j - 1]);   // lane 0 <- lane 15
-                                bop[j] = asOperand(dpp4<kRowShr1>(edge, c[j]));                      // lane i <- lane i-1
-                            } else {
-                                const u32x4 edge = dpp4<kRowShr15>(zero, j == 3 ? right : c[j + 1]);  // lane 15 <- lane 0
-                                bop[j] = asOperand(dpp4<kRowShl1>(edge, c[j]));                      // lane i <- lane i+1
-                            }
-                        }
-                        const int tap = tapb + dx + 1;
-                        bf16x8 aop[2];
-#pragma unroll
-                        for (int m = 0; m < 2; ++m)
-                            aop[m] = asOperand(wl[wSlot(tap, 16 * m + lr, lq)]);
-#pragma unroll
-                        for (int m = 0; m < 2; ++m)
-#pragma unroll
-                            for (int j = 0; j < 4; ++j)
-                                acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop[m], bop[j], acc[m][j], 0, 0, 0);
-                    }
-                    continue;
-                }
 #pragma unroll
                 for (int dx = -1; dx <= 1; ++dx) {
                     bf16x8 bop[4];
@@ -268,7 +297,7 @@ __global__ __launch_bounds__(256) void conv3d_kernel(ConvArgs a)
         const size_t orow = (static_cast<size_t>(n * a.nz + z) * a.ny + y) * a.nx;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const size_t v = orow + x0 + 16 * j + lr;
+            const size_t v = orow + x0 + (SHIFT ? 4 * lr + j : 16 * j + lr);
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
                 const int co0 = 16 * m + 4 * lq;
@@ -726,13 +755,13 @@ hipError_t launchConv(ConvArgs a, const Dims& d, hipStream_t s)
     a.nx = d.nx;
     if (!a.tiles)
         a.wave_tiles = d.nb * d.nz * d.ny * (d.nx / 64);
-    const int wg_tiles = (a.wave_tiles + 3) / 4;
+    const int wg_tiles = (a.wave_tiles + kConvWaves - 1) / kConvWaves;
     const int grid = roundUp8(std::min(wg_tiles, 2 * computeUnits()));
     a.chunk = (wg_tiles + grid - 1) / grid;
     if (convShift())
-        hipLaunchKernelGGL(conv3d_kernel<true>, dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(conv3d_kernel<true>, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
     else
-        hipLaunchKernelGGL(conv3d_kernel<false>, dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(conv3d_kernel<false>, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
     return hipGetLastError();
 }
 
