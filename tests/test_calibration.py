@@ -144,7 +144,7 @@ def test_calibrated_axial_scan_reproduces_ctdiw():
     grid.setSpacing(spacing)
     world.build()
     dx.Transport(seed=77)(world, beam, None, True)
-    dose = grid.doseScored()[0].reshape(dims[::-1])
+    dose, dose_var, _ = (a.reshape(dims[::-1]) for a in grid.doseScored())
     mass = dens.reshape(dims[::-1])
     ctdi = [float((dose[c] * mass[c]).sum() / mass[c].sum()) * 10.0 / 4.0 for c in chambers]
     ctdiw = ctdi[0] / 3 + 2 / 3 * np.mean(ctdi[1:])
@@ -153,8 +153,18 @@ def test_calibrated_axial_scan_reproduces_ctdiw():
     # uncalibrated runs leave the raw dose
     grid.clearScores()
     dx.Transport(seed=78)(world, beam, None, False)
-    raw = grid.doseScored()[0].reshape(dims[::-1])
-    assert float(raw[chambers[0]].mean()) == pytest.approx(float(dose[chambers[0]].mean()) / f, rel=0.05)
+    raw, raw_var, _ = (a.reshape(dims[::-1]) for a in grid.doseScored())
+    # centre chamber mean of both runs, with standard errors from the scored
+    # variances (voxels taken as independent)
+    c0 = chambers[0]
+    n0 = int(c0.sum())
+    m_cal, se_cal = float(dose[c0].mean()) / f, float(np.sqrt(dose_var[c0].sum())) / n0 / f
+    m_raw, se_raw = float(raw[c0].mean()), float(np.sqrt(raw_var[c0].sum())) / n0
+    se = float(np.hypot(se_cal, se_raw))
+    z = (m_raw - m_cal) / se
+    print("centre chamber raw", m_raw, "+-", se_raw, "calibrated / f", m_cal, "+-", se_cal, "z", z)
+    assert se < 0.05 * m_cal
+    assert abs(z) < 4.0
 
 
 @pytest.mark.gpu
