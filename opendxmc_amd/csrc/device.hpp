@@ -70,6 +70,7 @@ struct DevParams {
     float bmax[3];
     float inv_spacing[3];
     int level;   // correction level 0 or 1
+    int index24;   // nx < 2^24 and ny * nz < 2^24: voxel indices from 24-bit multiplies
 
     // attenuation tables (see physics.hpp DeviceTables)
     const float* mu_tot;

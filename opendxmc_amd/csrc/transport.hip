@@ -73,7 +73,7 @@ enum LaneState : int { kDead = 0, kTrack = 1, kInteract = 2, kDone = 3 };
 // about 1 ulp) for sampling.  The library calls expand to ~10-20 instructions
 // each for denormal handling and correct rounding, which sampling does not
 // need: a flight length -ln(u)/mu cost 22 instructions of a ~70-instruction
-// Woodcock step.  Arguments are normal, positive floats (u >= 2^-24).
+// Woodcock step.  Arguments are normal, positive floats (u >= 2^-23).
 constexpr float kLn2f = 0.693147181f;
 constexpr float kLog2ef = 1.442695041f;
 __device__ inline float hwLog(float x) { return __builtin_amdgcn_logf(x) * kLn2f; }
@@ -105,7 +105,9 @@ __device__ inline void rngSeed(Rng& r, uint64_t seed, uint64_t history)
         r.s0 = 1;
 }
 
-// xoshiro128+; uniform in (0, 1]
+// xoshiro128+; uniform in (0, 1] on a grid of 2^-23: the top 23 bits become
+// the mantissa of a float in [1, 2) (shift, or) and 2 - f lands in (0, 1]
+// (one subtract) -- three instructions where a convert-and-scale takes four.
 __device__ inline float rngU(Rng& r)
 {
     const uint32_t out = r.s0 + r.s3;
@@ -116,7 +118,7 @@ __device__ inline float rngU(Rng& r)
     r.s0 ^= r.s3;
     r.s2 ^= t;
     r.s3 = (r.s3 << 11) | (r.s3 >> 21);
-    return static_cast<float>((out >> 8) + 1u) * (1.0f / 16777216.0f);
+    return 2.0f - __uint_as_float((out >> 9) | 0x3F800000u);
 }
 
 // Slab entry of a ray into the grid box.  EXACT: identical operation order to
@@ -161,6 +163,9 @@ __device__ inline int voxelOf(const DevParams& P, float px, float py, float pz)
     if (static_cast<unsigned>(ix) >= static_cast<unsigned>(P.nx) || static_cast<unsigned>(iy) >= static_cast<unsigned>(P.ny)
         || static_cast<unsigned>(iz) >= static_cast<unsigned>(P.nz))
         return -1;
+    if (P.index24)   // full-rate v_mad_u32_u24 (the 32-bit multiply is quarter rate); products stay below 2^31
+        return static_cast<int>(__umul24(static_cast<unsigned>(P.nx), static_cast<unsigned>(iy) + __umul24(static_cast<unsigned>(P.ny), static_cast<unsigned>(iz))))
+               + ix;
     return ix + P.nx * (iy + P.ny * iz);
 }
 
@@ -227,13 +232,18 @@ __device__ inline float exitDistance(float p, float d, float inv_d, float bmin, 
     return fmaxf((face - p) * inv_d, 0.0f);
 }
 
-// Energy bin of the attenuation tables and the bin's majorant.
+// Energy bin of the attenuation tables and the bin's majorant.  Macro-cell
+// tracking scales the majorant per cell, so CELLS keeps it as is; the global
+// tracker keeps its reciprocal (the mean free path between tentative
+// collisions), so a step multiplies instead of taking a quarter-rate
+// reciprocal: the reciprocal is taken once per energy change.
+template <bool CELLS>
 __device__ inline void energyBin(float E, const float* s_maj, int& ej, float& et, float& maj)
 {
     const float f = (E - kLutE0f) * kLutInvDEf;
     ej = min(static_cast<int>(f), kLutNd - 2);
     et = f - static_cast<float>(ej);
-    maj = s_maj[ej];
+    maj = CELLS ? s_maj[ej] : hwRcp(s_maj[ej]);
 }
 
 }  // namespace
@@ -271,7 +281,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
 {
     const int lane = static_cast<int>(__lane_id());
     float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 1, E = 0, w = 0;
-    float maj = 1, et = 0;
+    float maj = 1, et = 0;   // majorant at the photon's energy (its reciprocal without CELLS: energyBin)
     int ej = 0, vi = 0;
     int state = kDead;
     float tau = -1.0f;   // optical depth left in the current flight (macro cells; < 0: draw a new one)
@@ -408,10 +418,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     const double we = static_cast<double>(w) * static_cast<double>(E);
                     atomicAdd(&e_wave[0], we);
                     if (boxEntry<false>(P, px, py, pz, dx, dy, dz)) {
-                        const float f = (E - kLutE0f) * kLutInvDEf;
-                        ej = min(static_cast<int>(f), kLutNd - 2);
-                        et = f - static_cast<float>(ej);
-                        maj = s_maj[ej];
+                        energyBin<CELLS>(E, s_maj, ej, et, maj);
                         state = kTrack;
                         tau = -1.0f;
                         if (CELLS)
@@ -476,7 +483,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         dx = st * cp;
                         dy = st * sp;
                         dz = ct;
-                        energyBin(E, s_maj, ej, et, maj);
+                        energyBin<CELLS>(E, s_maj, ej, et, maj);
                     } else {
                         state = kDead;
                     }
@@ -556,10 +563,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         e_scored += rest;
                         state = kDead;
                     } else {
-                        const float f = (E - kLutE0f) * kLutInvDEf;
-                        ej = min(static_cast<int>(f), kLutNd - 2);
-                        et = f - static_cast<float>(ej);
-                        maj = s_maj[ej];
+                        energyBin<CELLS>(E, s_maj, ej, et, maj);
                     }
                 } else {
                     // Rayleigh
@@ -688,7 +692,8 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
             continue;
         }
         if (state == kTrack) {
-            const float s = -hwLog(rngU(rng)) * hwRcp(maj);
+            const float mfp = maj;   // the global instance keeps 1 / majorant
+            const float s = -hwLog(rngU(rng)) * mfp;
             px = __fmaf_rn(s, dx, px);
             py = __fmaf_rn(s, dy, py);
             pz = __fmaf_rn(s, dz, pz);
@@ -703,7 +708,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 const float rho = asFloat(word & 0xFFFFFF00u);
                 const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
                 mat = m;
-                if (!(rngU(rng) * maj > mu))
+                if (!(rngU(rng) > mu * mfp))
                     state = kInteract;   // real collision; virtual ones keep tracking
             }
         }

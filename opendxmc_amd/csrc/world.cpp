@@ -192,6 +192,7 @@ void fillParams(const dxg_world* w, dxg::DevParams& p)
         p.inv_spacing[a] = 1.0f / static_cast<float>(w->spacing[a]);
     }
     p.level = w->level;
+    p.index24 = w->dims[0] < (1u << 24) && w->dims[1] * w->dims[2] < (1u << 24);
     p.mu_tot = w->d_mu_tot.ptr;
     p.mu_parts = w->d_mu_parts.ptr;
     p.majorant = w->d_majorant.ptr;
