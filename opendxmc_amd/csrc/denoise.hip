@@ -597,7 +597,7 @@ __global__ __launch_bounds__(64) void zero2_kernel(double* p)
 }
 
 // acc[0] += sum over body voxels (material != 0) of sum_wdE / density, acc[1] += their count.
-__global__ __launch_bounds__(256) void dose_mean_kernel(const uint32_t* voxels, const double* tally, unsigned tally_vs, size_t nv, double* acc)
+__global__ __launch_bounds__(256) void dose_mean_kernel(const uint32_t* voxels, const double* tally, size_t nv, double* acc)
 {
     __shared__ double red[2][4];
     double s = 0.0, c = 0.0;
@@ -606,7 +606,7 @@ __global__ __launch_bounds__(256) void dose_mean_kernel(const uint32_t* voxels, 
         if ((word & 0xFFu) != 0) {
             const float rho = wordDensity(word);
             if (rho > 0.0f)
-                s += tally[v * tally_vs] / static_cast<double>(rho);
+                s += tally[v * dxg::kTallyRecord] / static_cast<double>(rho);
             c += 1.0;
         }
     }
@@ -633,9 +633,7 @@ __device__ inline double doseRef(const double* acc)
 
 struct WorldFeatArgs {
     const uint32_t* voxels;
-    const double* tally;    // sum of voxel i at i * tally_vs, its sum of squares tally_ks further
-    unsigned long long tally_ks;
-    unsigned tally_vs;
+    const double* tally;    // records of dxg::kTallyRecord doubles: sum, sum of squares, ...
     const float* emb;
     int n_emb;
     int nx, nxp;            // grid x extent and its padded extent
@@ -663,7 +661,7 @@ __global__ __launch_bounds__(256) void world_features_kernel(WorldFeatArgs a)
         const uint32_t word = a.voxels[i];
         const uint32_t m = word & 0xFFu;
         const float rho = wordDensity(word);
-        const double e = a.tally[i * a.tally_vs], e2 = a.tally[i * a.tally_vs + a.tally_ks];
+        const double e = a.tally[i * dxg::kTallyRecord], e2 = a.tally[i * dxg::kTallyRecord + 1];
         const double d = rho > 0.0f ? e / static_cast<double>(rho) : 0.0;
         f[0] = static_cast<float>(d / doseRef(a.acc));
         f[1] = rho;
@@ -1012,12 +1010,10 @@ struct dxg_denoiser {
         HIP_TRY(hipMemcpyAsync(wemb.p, v.embedding.data(), v.embedding.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(zero2_kernel, dim3(1), dim3(64), 0, stream, wacc.p);
         const int mblocks = static_cast<int>(std::min<size_t>((nv + 255) / 256, 4096));
-        hipLaunchKernelGGL(dose_mean_kernel, dim3(mblocks), dim3(256), 0, stream, v.voxels, v.tally, v.tally_vs, nv, wacc.p);
+        hipLaunchKernelGGL(dose_mean_kernel, dim3(mblocks), dim3(256), 0, stream, v.voxels, v.tally, nv, wacc.p);
         WorldFeatArgs a {};
         a.voxels = v.voxels;
         a.tally = v.tally;
-        a.tally_ks = v.tally_ks;
-        a.tally_vs = v.tally_vs;
         a.emb = wemb.p;
         a.n_emb = static_cast<int>(v.embedding.size());
         a.nx = v.nx;

@@ -61,6 +61,13 @@ struct BeamBlob {
     int words;
 };
 
+// The tally of voxel i is the 32-byte record tally[kTallyRecord * i + k]:
+// k = 0 sum of w dE, 1 sum of (w dE)^2, 2 number of events (an FP64 integer,
+// exact below 2^53), 3 unused.  The three values of one deposit sit in one
+// aligned 32-byte block, so one atomic instruction can update all three from
+// three lanes as a single memory request.
+constexpr int kTallyRecord = 4;
+
 // Everything one transport launch reads.  Passed by value as a kernel argument.
 struct DevParams {
     // voxel grid of packed words, flat index x + nx (y + ny z)
@@ -98,12 +105,8 @@ struct DevParams {
     float cell_qscale;
     int cell_lds;
 
-    // tallies of voxel i: sum w dE at tally[i * tally_vs], sum (w dE)^2 one
-    // tally_ks further, events (u64 bits) two further -- three arrays
-    // (tally_vs 1, tally_ks = voxels) or four doubles per voxel (4, 1)
+    // tally: one record of kTallyRecord doubles per voxel
     double* tally;
-    unsigned long long tally_ks;
-    unsigned int tally_vs;
     float dose_scale;   // beam calibration applied to every deposit (1: none)
 
     // beam
@@ -147,9 +150,7 @@ int transportBatchBlocksPerCU(bool lds_lut, int level, bool cells, size_t lds_by
 // (the dose denoiser reads the tally and the voxel words in place).
 struct WorldView {
     const uint32_t* voxels;    // packed words (packVoxel)
-    const double* tally;       // sum w dE, sum (w dE)^2, events (u64 bits) of voxel i at i * tally_vs + k * tally_ks
-    unsigned long long tally_ks;
-    unsigned int tally_vs;
+    const double* tally;       // one record of kTallyRecord doubles per voxel
     int nx, ny, nz;
     double spacing[3];         // cm
     uint64_t histories;

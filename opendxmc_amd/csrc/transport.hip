@@ -192,13 +192,42 @@ __device__ inline void rotate(float& dx, float& dy, float& dz, float cost, float
     dz = nz * inv;
 }
 
-__device__ inline void tallyAdd(const DevParams& P, int idx, float e)
+// Scores the deposits of one interaction phase: each lane with deps > 0 made
+// deps (1 or 2) deposits e1 [, e2] at its voxel vi, and its record gains
+// (sum w dE, sum (w dE)^2, events) = (d1 + d2, d1^2 + d2^2, deps) with
+// d = e * dose_scale.  Wavefront-collective: every lane of the wavefront calls
+// it.  FP64 atomics are executed beyond L2 at a fixed rate of requests, and
+// one atomic instruction's lanes that fall in one 64-byte block travel as one
+// request; so a record update is spread over three consecutive lanes of one
+// instruction (one value each, all in the record's aligned 32 bytes) -- one
+// request per update where three separate atomics made three.
+__device__ inline void tallyFlush(const DevParams& P, int deps, int vi, float e1, float e2)
 {
-    const double de = static_cast<double>(e) * static_cast<double>(P.dose_scale);
-    double* t = P.tally + static_cast<size_t>(idx) * P.tally_vs;
-    atomicAdd(t, de);
-    atomicAdd(t + P.tally_ks, de * de);
-    atomicAdd(reinterpret_cast<unsigned long long*>(t + 2 * P.tally_ks), 1ull);
+    const unsigned long long m = __ballot(deps > 0);
+    const int k = __popcll(m);
+    if (k == 0)
+        return;
+    const int lane = static_cast<int>(__lane_id());
+    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+    // lane q < k learns the lane of update q: updating lanes go to their rank,
+    // the others to k and up, a permutation, so every destination is written once
+    const int dest = deps > 0 ? rank : k + (lane - rank);
+    const int src_of = __builtin_amdgcn_ds_permute(dest * 4, lane);
+    const double scale = static_cast<double>(P.dose_scale);
+    for (int base = 0; base < 3 * k; base += 64) {   // wavefront-uniform rounds of 64 slots
+        const int slot = base + lane;                  // slot = 3 * update + value
+        const int q = slot / 3;
+        const int c = slot - 3 * q;
+        const int src = __builtin_amdgcn_ds_bpermute((q & 63) * 4, src_of);
+        const int idx = __builtin_amdgcn_ds_bpermute(src * 4, vi);
+        const int n = __builtin_amdgcn_ds_bpermute(src * 4, deps);
+        const double d1 = static_cast<double>(__int_as_float(__builtin_amdgcn_ds_bpermute(src * 4, __float_as_int(e1)))) * scale;
+        const double d2 = static_cast<double>(__int_as_float(__builtin_amdgcn_ds_bpermute(src * 4, __float_as_int(e2)))) * scale;
+        if (slot < 3 * k) {
+            const double v = c == 0 ? d1 + d2 : (c == 1 ? d1 * d1 + d2 * d2 : static_cast<double>(n));
+            atomicAdd(P.tally + static_cast<size_t>(idx) * kTallyRecord + c, v);
+        }
+    }
 }
 
 __device__ inline float lerpTab(const float* tab, int j, float t)
@@ -438,6 +467,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
         // -------------------------------------------------------- interact
         if (n_int > 0 && (n_int >= P.interact_threshold || n_track == 0)) {
             int deps = 0;
+            float e1 = 0.0f, e2 = 0.0f;   // this phase's deposits at vi (tallyFlush)
             double e_scored = 0.0;
             if (state == kInteract) {
                 collided = true;
@@ -470,7 +500,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         }
                     }
                     const float de = w * (E - e_out);
-                    tallyAdd(P, vi, de);
+                    e1 = de;
                     deps = 1;
                     e_scored = de;
                     if (e_out > 0.0f) {
@@ -551,14 +581,14 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                             break;
                     }
                     const float de = w * (E - enew);
-                    tallyAdd(P, vi, de);
+                    e1 = de;
                     deps = 1;
                     e_scored = de;
                     E = enew;
                     rotate(dx, dy, dz, cost, kTwoPi * rngU(rng));
                     if (E < kCutoff) {
                         const float rest = w * E;
-                        tallyAdd(P, vi, rest);
+                        e2 = rest;
                         deps = 2;
                         e_scored += rest;
                         state = kDead;
@@ -594,6 +624,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     rotate(dx, dy, dz, cost, kTwoPi * rngU(rng));
                 }
             }
+            tallyFlush(P, deps, vi, e1, e2);
             if (deps > 0)
                 atomicAdd(&e_wave[2], e_scored);
             n_real += n_int;

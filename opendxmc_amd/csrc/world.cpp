@@ -97,14 +97,8 @@ struct dxg_world {
     DevBuf<float> d_majorant, d_s, d_rfwd, d_rinv;
     DevBuf<float4> d_ia_shell;
     DevBuf<float> d_ia_inv, d_fl_cum, d_fl_line;
-    DevBuf<double> d_tally;
-    // tally layout: three arrays for grids under 32 M voxels -- a deposit's
-    // three atomics then land on three cache lines that L2 serves in parallel
-    // (CT spiral 256x256x200: 18.2 ms per 1e8 histories against 20.0 ms) --
-    // four doubles per voxel above, where three line fills from HBM per
-    // deposit cost more than they save (512x512x400: 25.9 ms against 30.7 ms)
-    bool tally_soa = false;
-    size_t tallyDoubles() const { return (tally_soa ? 3 : 4) * static_cast<size_t>(dims[0] * dims[1] * dims[2]); }
+    DevBuf<double> d_tally;   // one record of dxg::kTallyRecord doubles per voxel
+    size_t tallyDoubles() const { return dxg::kTallyRecord * static_cast<size_t>(dims[0] * dims[1] * dims[2]); }
     DevBuf<float> d_cells;   // macro-cell majorant ratios
     DevBuf<uint8_t> d_cells_q;   // the same, as bytes of cell_qscale (padded to 4)
     float cell_qscale = 0.0f;
@@ -160,8 +154,6 @@ int worldView(const dxg_world* w, WorldView& out)
     }
     out.voxels = w->d_voxels.ptr;
     out.tally = w->d_tally.ptr;
-    out.tally_ks = w->tally_soa ? w->size() : 1;
-    out.tally_vs = w->tally_soa ? 1 : 4;
     out.nx = static_cast<int>(w->dims[0]);
     out.ny = static_cast<int>(w->dims[1]);
     out.nz = static_cast<int>(w->dims[2]);
@@ -204,8 +196,6 @@ void fillParams(const dxg_world* w, dxg::DevParams& p)
     p.fl_cum = w->d_fl_cum.ptr;
     p.fl_line = w->d_fl_line.ptr;
     p.tally = w->d_tally.ptr;
-    p.tally_ks = w->tally_soa ? w->size() : 1;
-    p.tally_vs = w->tally_soa ? 1 : 4;
     p.pool = w->d_pool.ptr;
     p.counters = w->d_counters.ptr;
     if (w->cell > 0) {
@@ -403,7 +393,6 @@ int dxg_world_build(dxg_world* w)
             HIP_TRY(w->d_cells_q.upload(q, w->stream));
         }
     }
-    w->tally_soa = n < (32ull << 20);
     HIP_TRY(w->d_tally.alloc(w->tallyDoubles()));
     HIP_TRY(hipMemsetAsync(w->d_tally.ptr, 0, w->tallyDoubles() * sizeof(double), w->stream));
     HIP_TRY(w->d_pool.alloc(1));
@@ -736,27 +725,17 @@ int dxg_read_energy_scores(const dxg_world* w, double* sum_e, double* sum_e2, ui
     if (!w || !w->built)
         return fail(DXG_ERR_STATE, "dxg_read_energy_scores: world not built");
     const uint64_t n = w->size();
-    const size_t bytes = n * sizeof(double);
-    if (w->tally_soa) {   // three arrays: straight into the caller's buffers
-        if (sum_e)
-            HIP_TRY(hipMemcpyAsync(sum_e, w->d_tally.ptr, bytes, hipMemcpyDeviceToHost, w->stream));
-        if (sum_e2)
-            HIP_TRY(hipMemcpyAsync(sum_e2, w->d_tally.ptr + n, bytes, hipMemcpyDeviceToHost, w->stream));
-        if (n_events)
-            HIP_TRY(hipMemcpyAsync(n_events, w->d_tally.ptr + 2 * n, bytes, hipMemcpyDeviceToHost, w->stream));
-        HIP_TRY(hipStreamSynchronize(w->stream));
-        return DXG_OK;
-    }
-    std::vector<double> t(4 * n);
+    constexpr int R = dxg::kTallyRecord;
+    std::vector<double> t(R * n);
     HIP_TRY(hipMemcpyAsync(t.data(), w->d_tally.ptr, t.size() * sizeof(double), hipMemcpyDeviceToHost, w->stream));
     HIP_TRY(hipStreamSynchronize(w->stream));
     for (uint64_t i = 0; i < n; ++i) {
         if (sum_e)
-            sum_e[i] = t[4 * i];
+            sum_e[i] = t[R * i];
         if (sum_e2)
-            sum_e2[i] = t[4 * i + 1];
-        if (n_events)
-            std::memcpy(&n_events[i], &t[4 * i + 2], sizeof(uint64_t));
+            sum_e2[i] = t[R * i + 1];
+        if (n_events)   // an FP64 integer (exact below 2^53)
+            n_events[i] = static_cast<uint64_t>(t[R * i + 2]);
     }
     return DXG_OK;
 }
