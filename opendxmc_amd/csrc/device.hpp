@@ -76,6 +76,7 @@ struct DevParams {
     float bmin[3];
     float bmax[3];
     float inv_spacing[3];
+    float vox_off[3];   // -bmin * inv_spacing: voxel coordinate = fma(p, inv_spacing, vox_off) on sampling paths
     int level;   // correction level 0 or 1
     int index24;   // nx < 2^24 and ny * nz < 2^24: voxel indices from 24-bit multiplies
 

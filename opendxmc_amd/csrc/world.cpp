@@ -182,6 +182,7 @@ void fillParams(const dxg_world* w, dxg::DevParams& p)
         p.bmin[a] = -half;
         p.bmax[a] = half;
         p.inv_spacing[a] = 1.0f / static_cast<float>(w->spacing[a]);
+        p.vox_off[a] = -p.bmin[a] * p.inv_spacing[a];
     }
     p.level = w->level;
     p.index24 = w->dims[0] < (1u << 24) && w->dims[1] * w->dims[2] < (1u << 24);
