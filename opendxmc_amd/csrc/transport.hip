@@ -347,6 +347,11 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
     auto cellRatio = [&](int c) {
         return P.cell_lds ? static_cast<float>(s_cellq[c]) * P.cell_qscale : P.cell_ratio[c];
     };
+    // flat macro-cell index (cell counts are below 1024 per axis: full-rate 24-bit multiplies)
+    auto cellIndex = [](const DevParams& Q, int cx, int cy, int cz) {
+        return static_cast<int>(__umul24(static_cast<unsigned>(Q.cnx), static_cast<unsigned>(cy) + __umul24(static_cast<unsigned>(Q.cny), static_cast<unsigned>(cz))))
+               + cx;
+    };
     Rng rng { 1, 2, 3, 4 };
     bool collided = false;
 
@@ -674,14 +679,11 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 if (tau < 0.0f)
                     tau = -hwLog(rngU(rng));
                 int cx = cellc & 1023, cy = (cellc >> 10) & 1023, cz = cellc >> 20;
-                float mu_c = cellRatio(cx + P.cnx * (cy + P.cny * cz)) * maj;
+                float mu_c = cellRatio(cellIndex(P, cx, cy, cz)) * maj;
                 const float idx = hwRcp(dx), idy = hwRcp(dy), idz = hwRcp(dz);
                 float tmx = exitDistance(px, dx, idx, P.bmin[0], P.cell_len[0], cx);
                 float tmy = exitDistance(py, dy, idy, P.bmin[1], P.cell_len[1], cy);
                 float tmz = exitDistance(pz, dz, idz, P.bmin[2], P.cell_len[2], cz);
-                const float ddx = dx != 0.0f ? P.cell_len[0] * fabsf(idx) : __builtin_inff();
-                const float ddy = dy != 0.0f ? P.cell_len[1] * fabsf(idy) : __builtin_inff();
-                const float ddz = dz != 0.0f ? P.cell_len[2] * fabsf(idz) : __builtin_inff();
                 float t0 = 0.0f;   // path walked in this iteration up to the current cell's entry face
                 float s = 0.0f;    // path to move the photon by at the end of the iteration
                 bool out = false;
@@ -703,13 +705,13 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     ++crossed;
                     if (tmx == t1) {
                         cx += dx > 0.0f ? 1 : -1;
-                        tmx += ddx;
+                        tmx += P.cell_len[0] * fabsf(idx);   // one cell further (dx != 0 here: tmx was finite)
                     } else if (tmy == t1) {
                         cy += dy > 0.0f ? 1 : -1;
-                        tmy += ddy;
+                        tmy += P.cell_len[1] * fabsf(idy);   // one cell further (dy != 0 here: tmy was finite)
                     } else {
                         cz += dz > 0.0f ? 1 : -1;
-                        tmz += ddz;
+                        tmz += P.cell_len[2] * fabsf(idz);   // one cell further (dz != 0 here: tmz was finite)
                     }
                     if (static_cast<unsigned>(cx) >= static_cast<unsigned>(P.cnx) || static_cast<unsigned>(cy) >= static_cast<unsigned>(P.cny)
                         || static_cast<unsigned>(cz) >= static_cast<unsigned>(P.cnz)) {
@@ -717,7 +719,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         s = t0;
                         break;
                     }
-                    mu_c = cellRatio(cx + P.cnx * (cy + P.cny * cz)) * maj;
+                    mu_c = cellRatio(cellIndex(P, cx, cy, cz)) * maj;
                 }
                 px = __fmaf_rn(s, dx, px);
                 py = __fmaf_rn(s, dy, py);
