@@ -78,6 +78,12 @@ struct DevParams {
     float inv_spacing[3];
     float vox_off[3];   // -bmin * inv_spacing: voxel coordinate = fma(p, inv_spacing, vox_off) on sampling paths
     int level;   // correction level 0 or 1
+    // outer medium (global-majorant tracking): every voxel outside the box
+    // [outer_lo, outer_hi] holds the packed word outer_word; outer_on is 0 when
+    // the box is the whole grid
+    int outer_on;
+    uint32_t outer_word;
+    float outer_lo[3], outer_hi[3];
     int index24;   // nx < 2^24 and ny * nz < 2^24: voxel indices from 24-bit multiplies
 
     // attenuation tables (see physics.hpp DeviceTables)

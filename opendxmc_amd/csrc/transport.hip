@@ -195,6 +195,25 @@ __device__ inline int voxelOfFast(const DevParams& P, float px, float py, float 
     return ix + P.nx * (iy + P.ny * iz);
 }
 
+// Whether a point lies in the closed box [lo, hi].
+__device__ inline bool inBox(const float* lo, const float* hi, float px, float py, float pz)
+{
+    return px >= lo[0] && px <= hi[0] && py >= lo[1] && py <= hi[1] && pz >= lo[2] && pz <= hi[2];
+}
+
+// Narrows [t0, t1] to the part of the ray p + t d inside the slab [lo, hi] of
+// one axis (id = 1 / d).
+__device__ inline void clipSlab(float p, float d, float id, float lo, float hi, float& t0, float& t1)
+{
+    if (d != 0.0f) {
+        const float ta = (lo - p) * id, tb = (hi - p) * id;
+        t0 = fmaxf(t0, fminf(ta, tb));
+        t1 = fminf(t1, fmaxf(ta, tb));
+    } else if (p < lo || p > hi) {
+        t0 = __builtin_inff();
+    }
+}
+
 __device__ inline void rotate(float& dx, float& dy, float& dz, float cost, float phi)
 {
     const float sint = hwSqrt(fmaxf(0.0f, 1.0f - cost * cost));
@@ -342,6 +361,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
     float tau = -1.0f;   // optical depth left in the current flight (macro cells; < 0: draw a new one)
     int cellc = 0;       // macro cell of the photon: cx | cy << 10 | cz << 20
     int mat = 0;         // material of the voxel at the photon (read with its word in the step)
+    bool outer = false;  // global tracking: the photon is in the outer medium, outside the box of the rest
     const uint8_t* s_cellq = reinterpret_cast<const uint8_t*>(s_blob + P.bb.words);
     // majorant ratio of macro cell c (LDS bytes or the global float table)
     auto cellRatio = [&](int c) {
@@ -480,6 +500,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     if (boxEntry<false>(P, px, py, pz, dx, dy, dz)) {
                         energyBin<CELLS>(E, s_maj, ej, et, maj);
                         state = kTrack;
+                        outer = !CELLS && P.outer_on && !inBox(P.outer_lo, P.outer_hi, px, py, pz);
                         tau = -1.0f;
                         if (CELLS)
                             cellc = cellCoord(px, P.bmin[0], P.cell_inv[0], P.cnx, dx)
@@ -749,7 +770,50 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
             DXG_PHASE_END(2);
             continue;
         }
-        if (state == kTrack) {
+        const bool woodcock = state == kTrack && !outer;
+        if (state == kTrack && outer) {
+            // In the outer medium (every voxel outside the box holds outer_word):
+            // the flight is sampled exactly with that medium's attenuation and
+            // ends at a real collision, at the box's face (the photon continues
+            // there with Woodcock steps: flights are memoryless, so the rest is
+            // drawn anew) or at the grid's edge.
+            const int ma = static_cast<int>(P.outer_word & 0xFFu);
+            const float mu_a = lerpTab(mu_tab + ma * kLutNd, ej, et) * asFloat(P.outer_word & 0xFFFFFF00u);
+            const float s = -hwLog(rngU(rng)) * hwRcp(mu_a);
+            const float ix = hwRcp(dx), iy = hwRcp(dy), iz = hwRcp(dz);
+            float t0 = 0.0f, t1 = __builtin_inff();
+            clipSlab(px, dx, ix, P.outer_lo[0], P.outer_hi[0], t0, t1);
+            clipSlab(py, dy, iy, P.outer_lo[1], P.outer_hi[1], t0, t1);
+            clipSlab(pz, dz, iz, P.outer_lo[2], P.outer_hi[2], t0, t1);
+            const float t_box = t0 <= t1 ? t0 : __builtin_inff();
+            float t_edge = 0.0f, t_far = __builtin_inff();   // the grid: where the ray leaves it
+            clipSlab(px, dx, ix, P.bmin[0], P.bmax[0], t_edge, t_far);
+            clipSlab(py, dy, iy, P.bmin[1], P.bmax[1], t_edge, t_far);
+            clipSlab(pz, dz, iz, P.bmin[2], P.bmax[2], t_edge, t_far);
+            bool escape = false;
+            if (s < fminf(t_box, t_far)) {
+                px = __fmaf_rn(s, dx, px);
+                py = __fmaf_rn(s, dy, py);
+                pz = __fmaf_rn(s, dz, pz);
+                vi = voxelOfFast(P, px, py, pz);
+                escape = vi < 0;   // rounding at the grid's face
+                mat = ma;
+                state = kInteract;
+            } else if (t_box < t_far) {
+                px = __fmaf_rn(t_box, dx, px);
+                py = __fmaf_rn(t_box, dy, py);
+                pz = __fmaf_rn(t_box, dz, pz);
+                outer = false;
+            } else {
+                escape = true;
+            }
+            if (escape) {
+                atomicAdd(&e_wave[1], static_cast<double>(w) * static_cast<double>(E));
+                if (!collided)
+                    atomicAdd(&e_wave[3], 1.0);   // uncollided exit
+                state = kDead;
+            }
+        } else if (woodcock) {
             const float mfp = maj;   // the global instance keeps 1 / majorant
             const float s = -hwLog(rngU(rng)) * mfp;
             px = __fmaf_rn(s, dx, px);
@@ -767,11 +831,13 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 const float rho = asFloat(word & 0xFFFFFF00u);
                 const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
                 mat = m;
+                if (P.outer_on && word == P.outer_word)   // outer medium: inside the box or out of it
+                    outer = !inBox(P.outer_lo, P.outer_hi, px, py, pz);
                 if (!(rngU(rng) > mu * mfp))
                     state = kInteract;   // real collision; virtual ones keep tracking
             }
         }
-        n_steps += n_track;
+        n_steps += __popcll(__ballot(woodcock));
         DXG_PHASE_END(2);
     }
 #undef DXG_PHASE_END

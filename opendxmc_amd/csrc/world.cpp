@@ -92,6 +92,12 @@ struct dxg_world {
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf<uint32_t> d_voxels;
+    // box of the voxels whose word differs from the grid's corner voxel (the
+    // outer medium), voxel index bounds inclusive; outer_on false when it is
+    // the whole grid
+    bool outer_on = false;
+    uint32_t outer_word = 0;
+    uint64_t outer_box[6] = { 0, 0, 0, 0, 0, 0 };
     DevBuf<float> d_mu_tot;
     DevBuf<float4> d_mu_parts;
     DevBuf<float> d_majorant, d_s, d_rfwd, d_rinv;
@@ -185,6 +191,22 @@ void fillParams(const dxg_world* w, dxg::DevParams& p)
         p.vox_off[a] = -p.bmin[a] * p.inv_spacing[a];
     }
     p.level = w->level;
+    p.outer_on = w->outer_on && w->cell == 0 ? 1 : 0;
+    if (const char* env = std::getenv("DXG_OUTER"))   // 0: Woodcock steps through the outer medium too
+        p.outer_on = std::atoi(env) != 0 ? p.outer_on : 0;
+    p.outer_word = w->outer_word;
+    for (int a = 0; a < 3; ++a) {
+        // the box in world coordinates, widened by 1e-4 voxel so that rounding
+        // in the voxel lookup never puts a point outside it into a voxel inside
+        const double sp = w->spacing[a], b0 = -0.5 * static_cast<double>(w->dims[a]) * sp;
+        if (w->outer_box[a] > w->outer_box[3 + a]) {   // empty box
+            p.outer_lo[a] = 1.0f;
+            p.outer_hi[a] = -1.0f;
+        } else {
+            p.outer_lo[a] = static_cast<float>(b0 + (static_cast<double>(w->outer_box[a]) - 1e-4) * sp);
+            p.outer_hi[a] = static_cast<float>(b0 + (static_cast<double>(w->outer_box[3 + a]) + 1.0 + 1e-4) * sp);
+        }
+    }
     p.index24 = w->dims[0] < (1u << 24) && w->dims[1] * w->dims[2] < (1u << 24);
     p.mu_tot = w->d_mu_tot.ptr;
     p.mu_parts = w->d_mu_parts.ptr;
@@ -347,6 +369,33 @@ int dxg_world_build(dxg_world* w)
     for (uint64_t i = 0; i < n; ++i)
         packed[i] = dxg::packVoxel(static_cast<float>(w->density[i]), w->material[i]);
     HIP_TRY(w->d_voxels.upload(packed, w->stream));
+    {
+        // The uniform outer medium around the object (usually air): photons
+        // outside the box of the other voxels sample their flights exactly in
+        // it instead of taking Woodcock steps through it.
+        const uint64_t nx = w->dims[0], ny = w->dims[1], nz = w->dims[2];
+        const uint32_t ow = packed[0];
+        uint64_t lo[3] = { nx, ny, nz }, hi[3] = { 0, 0, 0 };
+        bool any = false;
+        for (uint64_t z = 0, i = 0; z < nz; ++z)
+            for (uint64_t y = 0; y < ny; ++y)
+                for (uint64_t x = 0; x < nx; ++x, ++i)
+                    if (packed[i] != ow) {
+                        any = true;
+                        lo[0] = std::min(lo[0], x), hi[0] = std::max(hi[0], x);
+                        lo[1] = std::min(lo[1], y), hi[1] = std::max(hi[1], y);
+                        lo[2] = std::min(lo[2], z), hi[2] = std::max(hi[2], z);
+                    }
+        if (!any)   // a uniform grid: an empty box (first index above the last)
+            for (int a = 0; a < 3; ++a)
+                lo[a] = 1, hi[a] = 0;
+        w->outer_word = ow;
+        w->outer_on = !(any && lo[0] == 0 && lo[1] == 0 && lo[2] == 0 && hi[0] == nx - 1 && hi[1] == ny - 1 && hi[2] == nz - 1);
+        for (int a = 0; a < 3; ++a) {
+            w->outer_box[a] = lo[a];
+            w->outer_box[3 + a] = hi[a];
+        }
+    }
     HIP_TRY(w->d_mu_tot.upload(w->tables.mu_tot, w->stream));
     {
         std::vector<float4> parts(w->tables.mu_parts.size() / 4);

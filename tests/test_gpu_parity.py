@@ -262,3 +262,57 @@ def test_woodcock_beer_lambert(level):
     se = math.sqrt(expected * (1 - expected) / n)
     print(f"uncollided {got:.6f} expected {expected:.6f} ({(got - expected) / se:+.2f} SE)", flush=True)
     assert abs(got - expected) < 4 * se + 2e-4 * expected   # + FP32 table interpolation allowance
+
+
+def test_outer_medium_flights_match_woodcock_steps():
+    """Outside the box of its non-air voxels the body phantom is uniform air.
+    Photons there sample their flights exactly in air (to a real collision,
+    the box's face or the grid's edge) instead of taking Woodcock steps with
+    the bone-set majorant.  Both must sample the same transport: the two runs
+    agree voxel by voxel within their standard errors, in total energy, in the
+    energy deposited in the outer air and in the fraction of photons that
+    leave without interacting; the exact flights take fewer Woodcock steps."""
+    ph = small_body()
+    beam = dx.CTSpiralBeam((0, 0, -12), (0, 0, 12))
+    beam.setCollimation(4.0)
+    beam.setStepAngleDeg(15)
+    beam.setPitch(1.0)
+    beam.setTubeVoltage(120)
+    beam.addTubeFiltrationMaterial(13, 6.0)
+    beam.setBowtieFilter(read_filters()[DEFAULT_CT_BOWTIE])
+    beam.setNumberOfParticlesPerExposure(4_000_000 // beam.numberOfExposures())
+    world, grid = make_world(ph, 1)
+    outer = (ph.material == ph.material.flat[0]) & (ph.density == ph.density.flat[0])
+    assert 0 < outer.sum() < outer.size
+    runs = {}
+    old = os.environ.get("DXG_OUTER")
+    try:
+        for mode, env in (("exact", "1"), ("woodcock", "0")):
+            os.environ["DXG_OUTER"] = env
+            grid.clearScores()
+            assert dx.Transport(seed=31 if mode == "exact" else 32)(world, beam, None, False)
+            e, e2, _ = grid.energyScored()
+            runs[mode] = (e, e2, grid.histories(), grid.counters())
+    finally:
+        if old is None:
+            del os.environ["DXG_OUTER"]
+        else:
+            os.environ["DXG_OUTER"] = old
+    (e1, s1, h1, c1), (e0, s0, h0, c0) = runs["exact"], runs["woodcock"]
+    v = compare((e1, s1), (e0, s0), h1, h0)
+    print(v, flush=True)
+    assert v.ok, str(v)
+    print("woodcock steps per history", c1["steps"] / h1, "against", c0["steps"] / h0, flush=True)
+    assert c1["steps"] / h1 < 0.9 * c0["steps"] / h0
+    for c in (c1, c0):
+        assert abs(c["deposited_energy"] + c["escaped_energy"] - c["emitted_energy"]) <= 1e-5 * c["emitted_energy"]
+    # energy deposited in the outer air, per history, within 4 standard errors
+    a1, a0 = e1[outer].sum() / h1, e0[outer].sum() / h0
+    va = (s1[outer].sum() / h1 - a1 * a1) / h1 + (s0[outer].sum() / h0 - a0 * a0) / h0
+    print("outer-air energy per history", a1, a0, "z", (a1 - a0) / math.sqrt(va), flush=True)
+    assert a1 > 0 and abs(a1 - a0) <= 4.0 * math.sqrt(va)
+    # photons leaving without any interaction (binomial standard error)
+    u1, u0 = c1["uncollided"] / h1, c0["uncollided"] / h0
+    su = math.sqrt(u1 * (1 - u1) / h1 + u0 * (1 - u0) / h0)
+    print("uncollided fraction", u1, u0, "z", (u1 - u0) / su, flush=True)
+    assert abs(u1 - u0) <= 4.0 * su
