@@ -414,12 +414,23 @@ int dxg_world_build(dxg_world* w)
         HIP_TRY(w->d_fl_cum.upload(w->tables.fl_cum, w->stream));
         HIP_TRY(w->d_fl_line.upload(w->tables.fl_line, w->stream));
     }
-    // Macro-cell majorants pay off once the voxel words no longer sit in the
-    // 256 MB last-level cache: grids of >= 32 M voxels (128 MB of words) use
-    // cells of 64^3 voxels (measured on 512x512x400: 16^3 3.7e9, 32^3 4.0e9,
-    // 64^3 4.5e9, 128^3 4.3e9 histories/s), smaller grids the global majorant
-    // alone.  DXG_CELL = cell edge in voxels overrides (0 = global majorant).
-    w->cell = n >= (32ull << 20) ? 64 : 0;
+    // Global majorant with exact flights through the outer medium, or
+    // macro-cell majorants.  Cells pay off for grids whose voxel words no
+    // longer sit in the 256 MB last-level cache (>= 32 M voxels, 128 MB) and
+    // that the outer medium does not surround: on a 512x512x400 CT body
+    // (the box of the non-air voxels holds 42 % of the grid) the global
+    // majorant with exact air flights ran 5.6e9 histories/s against 4.7e9 for
+    // 64^3 cells (16^3 3.7e9, 32^3 4.0e9, 128^3 4.3e9).  So grids of >= 32 M
+    // voxels use 64^3 cells when the box holds more than 3/4 of the grid.
+    // DXG_CELL = cell edge in voxels overrides (0 = global majorant).
+    double box_frac = 1.0;
+    if (w->outer_on) {
+        box_frac = 0.0;
+        if (w->outer_box[0] <= w->outer_box[3])
+            box_frac = static_cast<double>(w->outer_box[3] - w->outer_box[0] + 1) * static_cast<double>(w->outer_box[4] - w->outer_box[1] + 1)
+                       * static_cast<double>(w->outer_box[5] - w->outer_box[2] + 1) / static_cast<double>(n);
+    }
+    w->cell = n >= (32ull << 20) && box_frac > 0.75 ? 64 : 0;
     if (const char* env = std::getenv("DXG_CELL"))
         w->cell = std::clamp(std::atoi(env), 0, 1024);
     if (w->cell > 0) {

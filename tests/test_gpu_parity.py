@@ -316,3 +316,34 @@ def test_outer_medium_flights_match_woodcock_steps():
     su = math.sqrt(u1 * (1 - u1) / h1 + u0 * (1 - u0) / h0)
     print("uncollided fraction", u1, u0, "z", (u1 - u0) / su, flush=True)
     assert abs(u1 - u0) <= 4.0 * su
+
+
+@pytest.mark.parametrize("edge", [8, 16])
+def test_macro_cell_tracking_matches_oracle(oracle, edge):
+    """Macro-cell majorants (forced on a small grid with DXG_CELL at build
+    time): the walk through cells of ratio r_c * majorant, tentative collisions
+    inside a cell accepted with mu / mu_c, gives the oracle's doses; the walk
+    does cross cells."""
+    ph = small_body()
+    beam = dx.CTSpiralBeam((0, 0, -12), (0, 0, 12))
+    beam.setCollimation(4.0)
+    beam.setStepAngleDeg(15)
+    beam.setPitch(1.0)
+    beam.setTubeVoltage(120)
+    beam.addTubeFiltrationMaterial(13, 6.0)
+    beam.setBowtieFilter(read_filters()[DEFAULT_CT_BOWTIE])
+    beam.setNumberOfParticlesPerExposure(3_000_000 // beam.numberOfExposures())
+    old = os.environ.get("DXG_CELL")
+    os.environ["DXG_CELL"] = str(edge)
+    try:
+        g, o, hg, ho, gc, oc = run_both(ph, beam, 1, oracle=oracle)
+    finally:
+        if old is None:
+            del os.environ["DXG_CELL"]
+        else:
+            os.environ["DXG_CELL"] = old
+    v = compare(g, o, hg, ho)
+    print(v, "crossings per history", gc["cell_crossings"] / hg, flush=True)
+    assert v.ok, str(v)
+    assert gc["cell_crossings"] > 0
+    assert abs(gc["deposited_energy"] + gc["escaped_energy"] - gc["emitted_energy"]) <= 1e-5 * gc["emitted_energy"]
