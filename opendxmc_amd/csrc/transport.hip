@@ -785,7 +785,8 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
             clipSlab(px, dx, ix, P.outer_lo[0], P.outer_hi[0], t0, t1);
             clipSlab(py, dy, iy, P.outer_lo[1], P.outer_hi[1], t0, t1);
             clipSlab(pz, dz, iz, P.outer_lo[2], P.outer_hi[2], t0, t1);
-            const float t_box = t0 <= t1 ? t0 : __builtin_inff();
+            // an empty box (a uniform grid: lo above hi) is never entered
+            const float t_box = t0 <= t1 && P.outer_lo[0] <= P.outer_hi[0] ? t0 : __builtin_inff();
             float t_edge = 0.0f, t_far = __builtin_inff();   // the grid: where the ray leaves it
             clipSlab(px, dx, ix, P.bmin[0], P.bmax[0], t_edge, t_far);
             clipSlab(py, dy, iy, P.bmin[1], P.bmax[1], t_edge, t_far);
