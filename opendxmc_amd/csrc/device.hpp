@@ -133,6 +133,7 @@ struct DevParams {
     DevCounters* counters;
     int refill_threshold;            // dead lanes that trigger a wavefront refill
     int interact_threshold;          // parked lanes that trigger the interaction phase
+    int outer_threshold;             // lanes outside the box that trigger the outer-flight phase
     int lds_lut;                     // attenuation table staged in LDS
     unsigned long long batch;        // histories a wavefront claims per pool atomic (>= 64)
 };

@@ -14,6 +14,9 @@
 //              sample their source particle together;
 //   * interact once interact_threshold lanes are parked (or no lane tracks):
 //              photoelectric / Compton / Rayleigh for all parked lanes at once;
+//   * outer    once outer_threshold lanes wait outside the box of the
+//              non-outer voxels (or no lane tracks): exact flights through the
+//              uniform outer medium, to a collision, the box or the grid's edge;
 //   * step     otherwise: every TRACK lane advances one Woodcock step -- one
 //              4-byte gather of the packed voxel word (density + material id),
 //              one LDS read of the attenuation at the photon's energy bin.
@@ -67,7 +70,7 @@ constexpr int kIaShellsd = 5;    // impulse-approximation shells per material
 constexpr int kIaInvNd = 2048;   // unit Compton profile inverse-CDF nodes
 constexpr int kFlMaxd = 12;      // fluorescing elements per material
 
-enum LaneState : int { kDead = 0, kTrack = 1, kInteract = 2, kDone = 3 };
+enum LaneState : int { kDead = 0, kTrack = 1, kInteract = 2, kDone = 3, kOuter = 4 };
 
 // The hardware's approximations (v_log_f32, v_exp_f32, v_rcp_f32, v_sqrt_f32;
 // about 1 ulp) for sampling.  The library calls expand to ~10-20 instructions
@@ -395,12 +398,13 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
         const unsigned long long m_dead = __ballot(state == kDead);
         const unsigned long long m_int = __ballot(state == kInteract);
         const unsigned long long m_track = __ballot(state == kTrack);
-        const int n_dead = __popcll(m_dead), n_int = __popcll(m_int), n_track = __popcll(m_track);
-        if (n_dead + n_int + n_track == 0)
+        const unsigned long long m_outer = __ballot(state == kOuter);
+        const int n_dead = __popcll(m_dead), n_int = __popcll(m_int), n_track = __popcll(m_track), n_outer = __popcll(m_outer);
+        if (n_dead + n_int + n_track + n_outer == 0)
             break;
 
         // ------------------------------------------------------------ refill
-        if (n_dead > 0 && (n_dead >= P.refill_threshold || (n_track == 0 && n_int == 0))) {
+        if (n_dead > 0 && (n_dead >= P.refill_threshold || (n_track == 0 && n_int == 0 && n_outer == 0))) {
             const unsigned long long rank = __popcll(m_dead & ((1ull << lane) - 1ull));
             const unsigned long long avail = end - cur;
             unsigned long long mine;
@@ -499,8 +503,8 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     atomicAdd(&e_wave[0], we);
                     if (boxEntry<false>(P, px, py, pz, dx, dy, dz)) {
                         energyBin<CELLS>(E, s_maj, ej, et, maj);
-                        state = kTrack;
                         outer = !CELLS && P.outer_on && !inBox(P.outer_lo, P.outer_hi, px, py, pz);
+                        state = outer ? kOuter : kTrack;
                         tau = -1.0f;
                         if (CELLS)
                             cellc = cellCoord(px, P.bmin[0], P.cell_inv[0], P.cnx, dx)
@@ -523,7 +527,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
             double e_scored = 0.0;
             if (state == kInteract) {
                 collided = true;
-                state = kTrack;
+                state = outer ? kOuter : kTrack;   // the photon stays where it is
                 tau = -1.0f;   // the photon stays in its macro cell; its next flight is drawn anew
                 const int m = mat;
                 const float4 a = P.mu_parts[m * kLutNd + ej];
@@ -770,51 +774,62 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
             DXG_PHASE_END(2);
             continue;
         }
-        const bool woodcock = state == kTrack && !outer;
-        if (state == kTrack && outer) {
-            // In the outer medium (every voxel outside the box holds outer_word):
-            // the flight is sampled exactly with that medium's attenuation and
-            // ends at a real collision, at the box's face (the photon continues
-            // there with Woodcock steps: flights are memoryless, so the rest is
-            // drawn anew) or at the grid's edge.
-            const int ma = static_cast<int>(P.outer_word & 0xFFu);
-            const float mu_a = lerpTab(mu_tab + ma * kLutNd, ej, et) * asFloat(P.outer_word & 0xFFFFFF00u);
-            const float s = -hwLog(rngU(rng)) * hwRcp(mu_a);
-            const float ix = hwRcp(dx), iy = hwRcp(dy), iz = hwRcp(dz);
-            float t0 = 0.0f, t1 = __builtin_inff();
-            clipSlab(px, dx, ix, P.outer_lo[0], P.outer_hi[0], t0, t1);
-            clipSlab(py, dy, iy, P.outer_lo[1], P.outer_hi[1], t0, t1);
-            clipSlab(pz, dz, iz, P.outer_lo[2], P.outer_hi[2], t0, t1);
-            // an empty box (a uniform grid: lo above hi) is never entered
-            const float t_box = t0 <= t1 && P.outer_lo[0] <= P.outer_hi[0] ? t0 : __builtin_inff();
-            float t_edge = 0.0f, t_far = __builtin_inff();   // the grid: where the ray leaves it
-            clipSlab(px, dx, ix, P.bmin[0], P.bmax[0], t_edge, t_far);
-            clipSlab(py, dy, iy, P.bmin[1], P.bmax[1], t_edge, t_far);
-            clipSlab(pz, dz, iz, P.bmin[2], P.bmax[2], t_edge, t_far);
-            bool escape = false;
-            if (s < fminf(t_box, t_far)) {
-                px = __fmaf_rn(s, dx, px);
-                py = __fmaf_rn(s, dy, py);
-                pz = __fmaf_rn(s, dz, pz);
-                vi = voxelOfFast(P, px, py, pz);
-                escape = vi < 0;   // rounding at the grid's face
-                mat = ma;
-                state = kInteract;
-            } else if (t_box < t_far) {
-                px = __fmaf_rn(t_box, dx, px);
-                py = __fmaf_rn(t_box, dy, py);
-                pz = __fmaf_rn(t_box, dz, pz);
-                outer = false;
-            } else {
-                escape = true;
+        // ----------------------------------------------- outer-medium flights
+        // Lanes outside the box of the non-outer voxels (global tracking) wait
+        // like parked collisions and fly together once outer_threshold of them
+        // are waiting or no lane tracks, so a step never runs this code for a
+        // few lanes beside the Woodcock steps of the rest.
+        if (n_outer > 0 && (n_outer >= P.outer_threshold || n_track == 0)) {
+            if (state == kOuter) {
+                // In the outer medium (every voxel outside the box holds outer_word):
+                // the flight is sampled exactly with that medium's attenuation and
+                // ends at a real collision, at the box's face (the photon continues
+                // there with Woodcock steps: flights are memoryless, so the rest is
+                // drawn anew) or at the grid's edge.
+                const int ma = static_cast<int>(P.outer_word & 0xFFu);
+                const float mu_a = lerpTab(mu_tab + ma * kLutNd, ej, et) * asFloat(P.outer_word & 0xFFFFFF00u);
+                const float s = -hwLog(rngU(rng)) * hwRcp(mu_a);
+                const float ix = hwRcp(dx), iy = hwRcp(dy), iz = hwRcp(dz);
+                float t0 = 0.0f, t1 = __builtin_inff();
+                clipSlab(px, dx, ix, P.outer_lo[0], P.outer_hi[0], t0, t1);
+                clipSlab(py, dy, iy, P.outer_lo[1], P.outer_hi[1], t0, t1);
+                clipSlab(pz, dz, iz, P.outer_lo[2], P.outer_hi[2], t0, t1);
+                // an empty box (a uniform grid: lo above hi) is never entered
+                const float t_box = t0 <= t1 && P.outer_lo[0] <= P.outer_hi[0] ? t0 : __builtin_inff();
+                float t_edge = 0.0f, t_far = __builtin_inff();   // the grid: where the ray leaves it
+                clipSlab(px, dx, ix, P.bmin[0], P.bmax[0], t_edge, t_far);
+                clipSlab(py, dy, iy, P.bmin[1], P.bmax[1], t_edge, t_far);
+                clipSlab(pz, dz, iz, P.bmin[2], P.bmax[2], t_edge, t_far);
+                bool escape = false;
+                if (s < fminf(t_box, t_far)) {
+                    px = __fmaf_rn(s, dx, px);
+                    py = __fmaf_rn(s, dy, py);
+                    pz = __fmaf_rn(s, dz, pz);
+                    vi = voxelOfFast(P, px, py, pz);
+                    escape = vi < 0;   // rounding at the grid's face
+                    mat = ma;
+                    state = kInteract;
+                } else if (t_box < t_far) {
+                    px = __fmaf_rn(t_box, dx, px);
+                    py = __fmaf_rn(t_box, dy, py);
+                    pz = __fmaf_rn(t_box, dz, pz);
+                    outer = false;
+                    state = kTrack;
+                } else {
+                    escape = true;
+                }
+                if (escape) {
+                    atomicAdd(&e_wave[1], static_cast<double>(w) * static_cast<double>(E));
+                    if (!collided)
+                        atomicAdd(&e_wave[3], 1.0);   // uncollided exit
+                    state = kDead;
+                }
             }
-            if (escape) {
-                atomicAdd(&e_wave[1], static_cast<double>(w) * static_cast<double>(E));
-                if (!collided)
-                    atomicAdd(&e_wave[3], 1.0);   // uncollided exit
-                state = kDead;
-            }
-        } else if (woodcock) {
+            DXG_PHASE_END(2);
+            continue;
+        }
+
+        if (state == kTrack) {
             const float mfp = maj;   // the global instance keeps 1 / majorant
             const float s = -hwLog(rngU(rng)) * mfp;
             px = __fmaf_rn(s, dx, px);
@@ -836,9 +851,11 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     outer = !inBox(P.outer_lo, P.outer_hi, px, py, pz);
                 if (!(rngU(rng) > mu * mfp))
                     state = kInteract;   // real collision; virtual ones keep tracking
+                else if (outer)
+                    state = kOuter;      // left the box: the next flight is exact
             }
         }
-        n_steps += __popcll(__ballot(woodcock));
+        n_steps += n_track;
         DXG_PHASE_END(2);
     }
 #undef DXG_PHASE_END

@@ -660,6 +660,9 @@ int prepareRun(dxg_world* w, const dxg_beam* beam, uint64_t seed, DevParams& p, 
     p.interact_threshold = 24;
     if (const char* env = std::getenv("DXG_INTERACT"))
         p.interact_threshold = std::clamp(std::atoi(env), 1, 64);
+    p.outer_threshold = 16;
+    if (const char* env = std::getenv("DXG_OUTERT"))
+        p.outer_threshold = std::clamp(std::atoi(env), 1, 64);
     p.batch = 256;
     if (const char* env = std::getenv("DXG_BATCH"))
         p.batch = std::max<unsigned long long>(64, std::strtoull(env, nullptr, 10));
