@@ -196,15 +196,16 @@ void fillParams(const dxg_world* w, dxg::DevParams& p)
         p.outer_on = std::atoi(env) != 0 ? p.outer_on : 0;
     p.outer_word = w->outer_word;
     for (int a = 0; a < 3; ++a) {
-        // the box in world coordinates, widened by 1e-4 voxel so that rounding
-        // in the voxel lookup never puts a point outside it into a voxel inside
+        // the box in world coordinates, widened by 1e-3 voxel so that rounding
+        // in the voxel lookup (a few 1e-5 voxel on grids of up to ~1000 voxels
+        // a side) never puts a point outside it into a voxel inside
         const double sp = w->spacing[a], b0 = -0.5 * static_cast<double>(w->dims[a]) * sp;
         if (w->outer_box[a] > w->outer_box[3 + a]) {   // empty box
             p.outer_lo[a] = 1.0f;
             p.outer_hi[a] = -1.0f;
         } else {
-            p.outer_lo[a] = static_cast<float>(b0 + (static_cast<double>(w->outer_box[a]) - 1e-4) * sp);
-            p.outer_hi[a] = static_cast<float>(b0 + (static_cast<double>(w->outer_box[3 + a]) + 1.0 + 1e-4) * sp);
+            p.outer_lo[a] = static_cast<float>(b0 + (static_cast<double>(w->outer_box[a]) - 1e-3) * sp);
+            p.outer_hi[a] = static_cast<float>(b0 + (static_cast<double>(w->outer_box[3 + a]) + 1.0 + 1e-3) * sp);
         }
     }
     p.index24 = w->dims[0] < (1u << 24) && w->dims[1] * w->dims[2] < (1u << 24);
