@@ -208,13 +208,21 @@ class Tube:
                       len(self.filtration), self.energy_step)
 
     def getSpecter(self) -> tuple[np.ndarray, np.ndarray]:
-        k = _Keep()
-        t = self._struct(k)
-        n = N.u64()
-        N.check(N.lib().dxg_tube_spectrum(C.byref(t), None, None, 0, C.byref(n)))
-        e, w = np.zeros(n.value), np.zeros(n.value)
-        N.check(N.lib().dxg_tube_spectrum(C.byref(t), N.ptr(e, C.c_double), N.ptr(w, C.c_double), n.value, C.byref(n)))
-        return e, w
+        # the spectrum depends on the tube's settings only: computed once per
+        # setting (every Transport call flattens the beam's spectra again)
+        key = (float(self.voltage), float(self.anode_angle_deg),
+               tuple(sorted((int(z), float(mm)) for z, mm in self.filtration.items())), float(self.energy_step))
+        cached = self.__dict__.get("_spectrum_cache")
+        if cached is None or cached[0] != key:
+            k = _Keep()
+            t = self._struct(k)
+            n = N.u64()
+            N.check(N.lib().dxg_tube_spectrum(C.byref(t), None, None, 0, C.byref(n)))
+            e, w = np.zeros(n.value), np.zeros(n.value)
+            N.check(N.lib().dxg_tube_spectrum(C.byref(t), N.ptr(e, C.c_double), N.ptr(w, C.c_double), n.value, C.byref(n)))
+            cached = (key, e, w)
+            self.__dict__["_spectrum_cache"] = cached
+        return cached[1].copy(), cached[2].copy()
 
     def halfValueLayerAl(self) -> float:
         k = _Keep()
