@@ -378,15 +378,23 @@ int dxg_world_build(dxg_world* w)
         const uint32_t ow = packed[0];
         uint64_t lo[3] = { nx, ny, nz }, hi[3] = { 0, 0, 0 };
         bool any = false;
-        for (uint64_t z = 0, i = 0; z < nz; ++z)
-            for (uint64_t y = 0; y < ny; ++y)
-                for (uint64_t x = 0; x < nx; ++x, ++i)
-                    if (packed[i] != ow) {
-                        any = true;
-                        lo[0] = std::min(lo[0], x), hi[0] = std::max(hi[0], x);
-                        lo[1] = std::min(lo[1], y), hi[1] = std::max(hi[1], y);
-                        lo[2] = std::min(lo[2], z), hi[2] = std::max(hi[2], z);
-                    }
+        for (uint64_t z = 0; z < nz; ++z)
+            for (uint64_t y = 0; y < ny; ++y) {
+                // a row: its first and last voxel that is not the outer medium
+                const uint32_t* row = packed.data() + (z * ny + y) * nx;
+                uint64_t first = 0;
+                while (first < nx && row[first] == ow)
+                    ++first;
+                if (first == nx)
+                    continue;
+                uint64_t last = nx - 1;
+                while (row[last] == ow)
+                    --last;
+                any = true;
+                lo[0] = std::min(lo[0], first), hi[0] = std::max(hi[0], last);
+                lo[1] = std::min(lo[1], y), hi[1] = std::max(hi[1], y);
+                lo[2] = std::min(lo[2], z), hi[2] = std::max(hi[2], z);
+            }
         if (!any)   // a uniform grid: an empty box (first index above the last)
             for (int a = 0; a < 3; ++a)
                 lo[a] = 1, hi[a] = 0;
