@@ -186,11 +186,14 @@ void dxg_progress_message(const dxg_progress* p, char* buf, uint64_t cap);
  * out[4*i + {0,1,2,3}] = photoelectric, incoherent, coherent, total (cm2/g). */
 int dxg_material_attenuation(const dxg_material* m, const double* energy_kev, uint64_t n, double* out);
 
-/* ---- Deterministic traversal through the kernel's own device functions:
- * the slab entry of each ray into the grid box, then p = fma(steps[k], d, p)
- * and the voxel index floor((p - box_min) * (1/spacing)).  out_index is -1
- * outside the grid, out_material the material id or -1.  Needs a built world;
- * pos/dir are n_rays x 3 float, steps n_rays x n_steps float. */
+/* ---- Deterministic traversal through the transport kernels' own device
+ * functions (AAVoxelGrid lookup, simulationpipeline.cpp:127; index order
+ * x + y*nx + z*nx*ny, otherphantomimportpipeline.cpp:44): the refill path's
+ * slab entry of each ray into the grid box (IEEE division), then per step
+ * p = fma(steps[k], d, p) and the Woodcock step's voxel lookup
+ * floor(fma(p, 1/spacing, -box_min/spacing)) per axis, all FP32.  out_index
+ * is -1 outside the grid, out_material the material id or -1.  Needs a built
+ * world; pos/dir are n_rays x 3 float, steps n_rays x n_steps float. */
 int dxg_debug_traverse(const dxg_world* w, uint64_t n_rays, const float* pos, const float* dir,
                        uint64_t n_steps, const float* steps, int64_t* out_index, int32_t* out_material);
 

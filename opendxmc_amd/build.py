@@ -12,6 +12,7 @@ import hashlib
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -60,15 +61,18 @@ def build_physics_list(force: bool = False) -> Path:
 
 def build_library() -> Path:
     BUILD.mkdir(exist_ok=True)
-    objs = []
+    jobs = []
     for src in SOURCES:
         path = CSRC / src
         if not path.exists():
             continue
         obj = BUILD / (src.replace(".", "_") + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        _run([HIPCC, *HIP_FLAGS, *EXTRA_FLAGS.get(src, []), *lang, "-c", str(path), "-o", str(obj)])
-        objs.append(str(obj))
+        jobs.append(([HIPCC, *HIP_FLAGS, *EXTRA_FLAGS.get(src, []), *lang, "-c", str(path), "-o", str(obj)], str(obj)))
+    # one compiler per source file, at most one per CPU
+    with ThreadPoolExecutor(max_workers=max(1, min(len(jobs), os.cpu_count() or 1))) as pool:
+        list(pool.map(_run, [cmd for cmd, _ in jobs]))
+    objs = [obj for _, obj in jobs]
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp), "-ldl", "-lz", "-lpthread"])
     os.replace(tmp, LIB)

@@ -124,27 +124,26 @@ __device__ inline float rngU(Rng& r)
     return 2.0f - __uint_as_float((out >> 9) | 0x3F800000u);
 }
 
-// Slab entry of a ray into the grid box.  EXACT: identical operation order to
-// the oracle's FP32 traversal (bit-exact check); else the reciprocals are the
-// hardware's (sampling only).
-template <bool EXACT = true>
+// Slab entry of a ray into the grid box, with IEEE division (three per
+// history, at its start): the entry point is reproducible bit for bit on any
+// IEEE FP32 machine with a fused multiply-add.
 __device__ inline bool boxEntry(const DevParams& P, float& px, float& py, float& pz, float dx, float dy, float dz)
 {
     float t0 = 0.0f, t1 = 3.0e38f;
     {
-        const float id = EXACT ? 1.0f / dx : hwRcp(dx);
+        const float id = 1.0f / dx;
         const float ta = (P.bmin[0] - px) * id, tb = (P.bmax[0] - px) * id;
         t0 = fmaxf(t0, fminf(ta, tb));
         t1 = fminf(t1, fmaxf(ta, tb));
     }
     {
-        const float id = EXACT ? 1.0f / dy : hwRcp(dy);
+        const float id = 1.0f / dy;
         const float ta = (P.bmin[1] - py) * id, tb = (P.bmax[1] - py) * id;
         t0 = fmaxf(t0, fminf(ta, tb));
         t1 = fminf(t1, fmaxf(ta, tb));
     }
     {
-        const float id = EXACT ? 1.0f / dz : hwRcp(dz);
+        const float id = 1.0f / dz;
         const float ta = (P.bmin[2] - pz) * id, tb = (P.bmax[2] - pz) * id;
         t0 = fmaxf(t0, fminf(ta, tb));
         t1 = fminf(t1, fmaxf(ta, tb));
@@ -157,12 +156,35 @@ __device__ inline bool boxEntry(const DevParams& P, float& px, float& py, float&
     return true;
 }
 
-// Flat voxel index of a point, or -1 outside the grid.
-__device__ inline int voxelOf(const DevParams& P, float px, float py, float pz)
+// Moves a point by s along d: one fused multiply-add per axis.
+__device__ inline void advance(float& px, float& py, float& pz, float s, float dx, float dy, float dz)
 {
-    const int ix = static_cast<int>(floorf((px - P.bmin[0]) * P.inv_spacing[0]));
-    const int iy = static_cast<int>(floorf((py - P.bmin[1]) * P.inv_spacing[1]));
-    const int iz = static_cast<int>(floorf((pz - P.bmin[2]) * P.inv_spacing[2]));
+    px = __fmaf_rn(s, dx, px);
+    py = __fmaf_rn(s, dy, py);
+    pz = __fmaf_rn(s, dz, pz);
+}
+
+// floor(x) converted to int by the hardware's floor-convert (one instruction
+// for floor + cvt; saturates outside the int range).
+__device__ inline int floorToInt(float x)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// Flat voxel index (x + y nx + z nx ny, the AAVoxelGrid order) of a point, or
+// -1 outside the grid.  The voxel coordinate per axis is
+//   floor(fma(p, inv_spacing, vox_off)),  vox_off = -box_min * inv_spacing,
+// rounded once in FP32: a point within an ulp of a voxel face belongs to the
+// voxel this rounding picks, the same on every IEEE FP32 machine with a fused
+// multiply-add.  The only voxel lookup of the transport kernels, and the one
+// the traversal check runs.
+__device__ inline int voxelIndex(const DevParams& P, float px, float py, float pz)
+{
+    const int ix = floorToInt(__fmaf_rn(px, P.inv_spacing[0], P.vox_off[0]));
+    const int iy = floorToInt(__fmaf_rn(py, P.inv_spacing[1], P.vox_off[1]));
+    const int iz = floorToInt(__fmaf_rn(pz, P.inv_spacing[2], P.vox_off[2]));
     if (static_cast<unsigned>(ix) >= static_cast<unsigned>(P.nx) || static_cast<unsigned>(iy) >= static_cast<unsigned>(P.ny)
         || static_cast<unsigned>(iz) >= static_cast<unsigned>(P.nz))
         return -1;
@@ -172,31 +194,10 @@ __device__ inline int voxelOf(const DevParams& P, float px, float py, float pz)
     return ix + P.nx * (iy + P.ny * iz);
 }
 
-// Voxel of a point on the sampling paths: the coordinate as one FMA with a
-// precomputed offset and the hardware's floor-convert (one instruction for
-// floor + cvt), about ten instructions fewer per Woodcock step than voxelOf.
-// A point within an ulp of a voxel face may land in the neighbouring voxel,
-// which sampling does not notice; the traversal check uses voxelOf.
-__device__ inline int floorToInt(float x)
-{
-    int r;
-    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
-    return r;
-}
-
-__device__ inline int voxelOfFast(const DevParams& P, float px, float py, float pz)
-{
-    const int ix = floorToInt(__fmaf_rn(px, P.inv_spacing[0], P.vox_off[0]));
-    const int iy = floorToInt(__fmaf_rn(py, P.inv_spacing[1], P.vox_off[1]));
-    const int iz = floorToInt(__fmaf_rn(pz, P.inv_spacing[2], P.vox_off[2]));
-    if (static_cast<unsigned>(ix) >= static_cast<unsigned>(P.nx) || static_cast<unsigned>(iy) >= static_cast<unsigned>(P.ny)
-        || static_cast<unsigned>(iz) >= static_cast<unsigned>(P.nz))
-        return -1;
-    if (P.index24)
-        return static_cast<int>(__umul24(static_cast<unsigned>(P.nx), static_cast<unsigned>(iy) + __umul24(static_cast<unsigned>(P.ny), static_cast<unsigned>(iz))))
-               + ix;
-    return ix + P.nx * (iy + P.ny * iz);
-}
+// Material id and FP32 density of a packed voxel word (density with its 8 low
+// mantissa bits replaced by the material id).
+__device__ inline int wordMaterial(uint32_t word) { return static_cast<int>(word & 0xFFu); }
+__device__ inline float wordDensity(uint32_t word) { return __uint_as_float(word & 0xFFFFFF00u); }
 
 // Whether a point lies in the closed box [lo, hi].
 __device__ inline bool inBox(const float* lo, const float* hi, float px, float py, float pz)
@@ -501,7 +502,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     }
                     const double we = static_cast<double>(w) * static_cast<double>(E);
                     atomicAdd(&e_wave[0], we);
-                    if (boxEntry<false>(P, px, py, pz, dx, dy, dz)) {
+                    if (boxEntry(P, px, py, pz, dx, dy, dz)) {
                         energyBin<CELLS>(E, s_maj, ej, et, maj);
                         outer = !CELLS && P.outer_on && !inBox(P.outer_lo, P.outer_hi, px, py, pz);
                         state = outer ? kOuter : kTrack;
@@ -746,12 +747,10 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     }
                     mu_c = cellRatio(cellIndex(P, cx, cy, cz)) * maj;
                 }
-                px = __fmaf_rn(s, dx, px);
-                py = __fmaf_rn(s, dy, py);
-                pz = __fmaf_rn(s, dz, pz);
+                advance(px, py, pz, s, dx, dy, dz);
                 cellc = cx | (cy << 10) | (cz << 20);
                 if (tentative)
-                    vi = voxelOfFast(P, px, py, pz);
+                    vi = voxelIndex(P, px, py, pz);
                 if (out || (tentative && vi < 0)) {
                     atomicAdd(&e_wave[1], static_cast<double>(w) * static_cast<double>(E));
                     if (!collided)
@@ -760,8 +759,8 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     tentative = false;
                 } else if (tentative) {
                     const uint32_t word = P.voxels[vi];
-                    const int m = static_cast<int>(word & 0xFFu);
-                    const float rho = asFloat(word & 0xFFFFFF00u);
+                    const int m = wordMaterial(word);
+                    const float rho = wordDensity(word);
                     const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
                     mat = m;
                     if (!(rngU(rng) * mu_c > mu))
@@ -786,8 +785,8 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 // ends at a real collision, at the box's face (the photon continues
                 // there with Woodcock steps: flights are memoryless, so the rest is
                 // drawn anew) or at the grid's edge.
-                const int ma = static_cast<int>(P.outer_word & 0xFFu);
-                const float mu_a = lerpTab(mu_tab + ma * kLutNd, ej, et) * asFloat(P.outer_word & 0xFFFFFF00u);
+                const int ma = wordMaterial(P.outer_word);
+                const float mu_a = lerpTab(mu_tab + ma * kLutNd, ej, et) * wordDensity(P.outer_word);
                 const float s = -hwLog(rngU(rng)) * hwRcp(mu_a);
                 const float ix = hwRcp(dx), iy = hwRcp(dy), iz = hwRcp(dz);
                 float t0 = 0.0f, t1 = __builtin_inff();
@@ -802,17 +801,13 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 clipSlab(pz, dz, iz, P.bmin[2], P.bmax[2], t_edge, t_far);
                 bool escape = false;
                 if (s < fminf(t_box, t_far)) {
-                    px = __fmaf_rn(s, dx, px);
-                    py = __fmaf_rn(s, dy, py);
-                    pz = __fmaf_rn(s, dz, pz);
-                    vi = voxelOfFast(P, px, py, pz);
+                    advance(px, py, pz, s, dx, dy, dz);
+                    vi = voxelIndex(P, px, py, pz);
                     escape = vi < 0;   // rounding at the grid's face
                     mat = ma;
                     state = kInteract;
                 } else if (t_box < t_far) {
-                    px = __fmaf_rn(t_box, dx, px);
-                    py = __fmaf_rn(t_box, dy, py);
-                    pz = __fmaf_rn(t_box, dz, pz);
+                    advance(px, py, pz, t_box, dx, dy, dz);
                     outer = false;
                     state = kTrack;
                 } else {
@@ -832,10 +827,8 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
         if (state == kTrack) {
             const float mfp = maj;   // the global instance keeps 1 / majorant
             const float s = -hwLog(rngU(rng)) * mfp;
-            px = __fmaf_rn(s, dx, px);
-            py = __fmaf_rn(s, dy, py);
-            pz = __fmaf_rn(s, dz, pz);
-            vi = voxelOfFast(P, px, py, pz);
+            advance(px, py, pz, s, dx, dy, dz);
+            vi = voxelIndex(P, px, py, pz);
             if (vi < 0) {
                 atomicAdd(&e_wave[1], static_cast<double>(w) * static_cast<double>(E));
                 if (!collided)
@@ -843,8 +836,8 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 state = kDead;
             } else {
                 const uint32_t word = P.voxels[vi];
-                const int m = static_cast<int>(word & 0xFFu);
-                const float rho = asFloat(word & 0xFFFFFF00u);
+                const int m = wordMaterial(word);
+                const float rho = wordDensity(word);
                 const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
                 mat = m;
                 if (P.outer_on && word == P.outer_word)   // outer medium: inside the box or out of it
@@ -961,6 +954,10 @@ __global__ void batch_reset_kernel(const DevParams* __restrict__ params, int n_i
         c->phase_cycles[k] = 0;
 }
 
+// The traversal of the transport kernels replayed on given rays: the slab
+// entry of the refill path (boxEntry), then per step advance() and the voxel
+// lookup of the Woodcock step (voxelIndex, wordMaterial) -- the same device
+// functions, so the indices are the ones tracking visits.
 __global__ void traverse_kernel(const DevParams P, uint64_t n_rays, const float* __restrict__ pos, const float* __restrict__ dir,
                                 uint64_t n_steps, const float* __restrict__ steps, int64_t* __restrict__ out_index,
                                 int32_t* __restrict__ out_material)
@@ -975,14 +972,11 @@ __global__ void traverse_kernel(const DevParams P, uint64_t n_rays, const float*
         int64_t vi = -1;
         int32_t mm = -1;
         if (hit) {
-            const float s = steps[r * n_steps + k];
-            px = __fmaf_rn(s, dx, px);
-            py = __fmaf_rn(s, dy, py);
-            pz = __fmaf_rn(s, dz, pz);
-            const int v = voxelOf(P, px, py, pz);
+            advance(px, py, pz, steps[r * n_steps + k], dx, dy, dz);
+            const int v = voxelIndex(P, px, py, pz);
             if (v >= 0) {
                 vi = v;
-                mm = static_cast<int32_t>(P.voxels[v] & 0xFFu);
+                mm = wordMaterial(P.voxels[v]);
             }
         }
         out_index[r * n_steps + k] = vi;

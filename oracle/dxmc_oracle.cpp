@@ -890,35 +890,54 @@ void oracle_rng_stream(uint64_t seed, uint64_t history, uint64_t n, uint32_t* ou
         out[i] = r.next();
 }
 
-// FP32 deterministic traversal with the exact operation order of the HIP
-// kernel: box entry by the slab method, then p = fmaf(s, d, p) per step and
-// ix = (int)floorf((p - box_min) * inv_spacing).  out_index = -1 outside.
+// floor(x) as a saturating int conversion: NaN -> 0, below -2^31 -> INT_MIN,
+// at or above 2^31 -> INT_MAX (the hardware floor-convert's behaviour).
+static int floorSat(float x)
+{
+    if (std::isnan(x))
+        return 0;
+    const float f = std::floor(x);
+    if (f < -2147483648.0f)
+        return INT32_MIN;
+    if (f >= 2147483648.0f)
+        return INT32_MAX;
+    return static_cast<int>(f);
+}
+
+// FP32 deterministic traversal, restating the tracking's operation order
+// (the flat voxel index i = x + y nx + z nx ny of AAVoxelGrid<5, C, 255>,
+// simulationpipeline.cpp:127 and otherphantomimportpipeline.cpp:44, on a grid
+// centred on the origin, datacontainer.cpp:174-180):
+//   box:    box_min = -0.5f * (float(n) * spacing), box_max = -box_min;
+//   entry:  slab method with IEEE division, p = fmaf(t0, d, p);
+//   step:   p = fmaf(s, d, p);
+//   lookup: inv = 1.0f / spacing, off = -box_min * inv, both rounded to FP32,
+//           and per axis i = floor(fmaf(p, inv, off)) (one rounding), saturated
+//           to the int range; outside [0, n) the point is outside the grid.
+// out_index = -1 and out_material = -1 outside.
 int oracle_traverse_f32(const uint64_t dims[3], const float spacing[3], const uint8_t* material,
     uint64_t n_rays, const float* pos, const float* dir, uint64_t n_steps, const float* steps,
     int64_t* out_index, int32_t* out_material)
 {
-    float bmin[3], bmax[3], inv[3];
+    float bmin[3], bmax[3], inv[3], off[3];
     for (int a = 0; a < 3; ++a) {
         bmin[a] = -0.5f * (static_cast<float>(dims[a]) * spacing[a]);
         bmax[a] = -bmin[a];
         inv[a] = 1.0f / spacing[a];
+        off[a] = -bmin[a] * inv[a];
     }
     for (uint64_t r = 0; r < n_rays; ++r) {
         float p[3] = { pos[3 * r], pos[3 * r + 1], pos[3 * r + 2] };
         const float d[3] = { dir[3 * r], dir[3 * r + 1], dir[3 * r + 2] };
-        // slab entry (same formula as the kernel)
         float t0 = 0.0f, t1 = 3.0e38f;
-        bool hit = true;
         for (int a = 0; a < 3; ++a) {
             const float id = 1.0f / d[a];
-            float ta = (bmin[a] - p[a]) * id;
-            float tb = (bmax[a] - p[a]) * id;
-            const float lo = std::fmin(ta, tb), hi = std::fmax(ta, tb);
-            t0 = std::fmax(t0, lo);
-            t1 = std::fmin(t1, hi);
+            const float ta = (bmin[a] - p[a]) * id;
+            const float tb = (bmax[a] - p[a]) * id;
+            t0 = std::fmax(t0, std::fmin(ta, tb));
+            t1 = std::fmin(t1, std::fmax(ta, tb));
         }
-        if (!(t0 <= t1))
-            hit = false;
+        const bool hit = t0 <= t1;
         if (hit)
             for (int a = 0; a < 3; ++a)
                 p[a] = std::fmaf(t0, d[a], p[a]);
@@ -926,13 +945,14 @@ int oracle_traverse_f32(const uint64_t dims[3], const float spacing[3], const ui
             int64_t vi = -1;
             int32_t mm = -1;
             if (hit) {
-                for (int a = 0; a < 3; ++a)
+                int c[3];
+                for (int a = 0; a < 3; ++a) {
                     p[a] = std::fmaf(steps[r * n_steps + k], d[a], p[a]);
-                const int ix = static_cast<int>(std::floor((p[0] - bmin[0]) * inv[0]));
-                const int iy = static_cast<int>(std::floor((p[1] - bmin[1]) * inv[1]));
-                const int iz = static_cast<int>(std::floor((p[2] - bmin[2]) * inv[2]));
-                if (ix >= 0 && iy >= 0 && iz >= 0 && ix < static_cast<int>(dims[0]) && iy < static_cast<int>(dims[1]) && iz < static_cast<int>(dims[2])) {
-                    vi = ix + static_cast<int64_t>(dims[0]) * (iy + static_cast<int64_t>(dims[1]) * iz);
+                    c[a] = floorSat(std::fmaf(p[a], inv[a], off[a]));
+                }
+                if (c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] < static_cast<int64_t>(dims[0]) && c[1] < static_cast<int64_t>(dims[1])
+                    && c[2] < static_cast<int64_t>(dims[2])) {
+                    vi = c[0] + static_cast<int64_t>(dims[0]) * (c[1] + static_cast<int64_t>(dims[1]) * c[2]);
                     mm = material[vi];
                 }
             }

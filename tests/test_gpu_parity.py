@@ -22,6 +22,7 @@ import opendxmc_amd as dx
 from opendxmc_amd import phantoms
 from opendxmc_amd.bowtie import DEFAULT_CT_BOWTIE, read_filters
 from parity import compare
+from rays import face_rays, random_rays
 
 pytestmark = pytest.mark.gpu
 
@@ -56,32 +57,27 @@ def run_both(ph, beam, level, seed=7, oracle=None, threads=16):
     return (e, e2), (se, se2), hg, oc["histories"], counters, oc
 
 
-def test_traverse_bit_exact(oracle):
+@pytest.mark.parametrize("dims,spacing", [((13, 9, 7), (0.25, 0.5, 0.3)), ((64, 64, 64), (0.1, 0.1, 0.1)),
+                                          ((256, 256, 200), (0.15, 0.15, 0.15)), ((512, 512, 16), (0.08, 0.08, 0.08))])
+def test_traverse_bit_exact(oracle, dims, spacing):
+    """The tracking's own entry, advance and voxel lookup (boxEntry, advance,
+    voxelIndex, wordMaterial of transport.hip, replayed by traverse_kernel)
+    against the oracle's FP32 restatement: random rays, axis rays on faces,
+    and lookups at and within 3 ulps of every voxel face of each axis."""
     rng = np.random.default_rng(3)
-    dims = (13, 9, 7)
-    spacing = (0.25, 0.5, 0.3)
     n = int(np.prod(dims))
     mat = rng.integers(0, 3, n).astype(np.uint8)
     ph = phantoms.Phantom(dims, spacing, np.ones(n), mat, [phantoms.AIR, phantoms.WATER, phantoms.BONE], ["a", "w", "b"])
     _, grid = make_world(ph, 1)
-    n_rays, n_steps = 4096, 24
-    half = np.array([d * s / 2 for d, s in zip(dims, spacing)], dtype=np.float32)
-    pos = rng.uniform(-3, 3, (n_rays, 3)).astype(np.float32) * 2.0
-    dirs = rng.normal(size=(n_rays, 3))
-    # rays on voxel faces and along axes (ties)
-    k = n_rays // 4
-    dirs[:k] = 0.0
-    dirs[np.arange(k), rng.integers(0, 3, k)] = rng.choice([-1.0, 1.0], k)
-    pos[:k] = (np.round(rng.uniform(-1, 1, (k, 3)) * 4) / 4 * half).astype(np.float32)
-    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
-    dirs = dirs.astype(np.float32)
-    steps = rng.exponential(0.4, (n_rays, n_steps)).astype(np.float32)
-    steps[: k // 2] = np.float32(0.25)
-    gi, gm = grid.traverse(pos, dirs, steps)
-    oi, om = oracle.traverse_f32(dims, np.array(spacing, dtype=np.float32), mat, pos, dirs, steps)
-    assert (gi >= 0).sum() > n_rays  # the rays do visit the grid
-    np.testing.assert_array_equal(gi, oi)
-    np.testing.assert_array_equal(gm, om)
+    sp = np.array(spacing, dtype=np.float32)
+    pos, dirs, steps = random_rays(rng, dims, sp, 4096, 24)
+    fpos, fdirs, fsteps = face_rays(rng, dims, sp, 24)
+    for p, d, st in ((pos, dirs, steps), (fpos, fdirs, fsteps)):
+        gi, gm = grid.traverse(p, d, st)
+        oi, om = oracle.traverse_f32(dims, sp, mat, p, d, st)
+        assert (gi >= 0).sum() > len(p)  # the rays do visit the grid
+        np.testing.assert_array_equal(gi, oi)
+        np.testing.assert_array_equal(gm, om)
 
 
 @pytest.mark.parametrize("level", [0, 1, 2])

@@ -14,6 +14,7 @@
 from __future__ import annotations
 
 import math
+from fractions import Fraction
 import sys
 from pathlib import Path
 
@@ -25,6 +26,7 @@ sys.path.insert(0, str(Path(__file__).resolve().parent.parent / "opendxmc_amd" /
 import gen_physics_list as g  # noqa: E402
 
 from opendxmc_amd import phantoms  # noqa: E402
+from rays import face_rays, random_rays  # noqa: E402
 
 NA = 6.02214076e23
 ME_C2 = 510.99895
@@ -273,3 +275,116 @@ def test_level2_energy_bookkeeping(oracle):
                                      level=2, seed=4, threads=4)
     assert abs(c["deposited_energy"] + c["escaped_energy"] - c["emitted_energy"]) < 1e-9 * c["emitted_energy"]
     assert se.sum() == pytest.approx(c["deposited_energy"], rel=1e-12)
+
+
+# ------------------------------------------------ traversal, exact restatement
+def _round_f32(x: Fraction) -> np.float32:
+    """Round an exact rational to the nearest FP32 value (ties to even)."""
+    if x == 0:
+        return np.float32(0.0)
+    sign = -1 if x < 0 else 1
+    a = abs(x)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    # scale so that 2^23 <= m < 2^24
+    while True:
+        m = a / Fraction(2) ** (e - 23) if e >= 23 else a * Fraction(2) ** (23 - e)
+        if m < 2 ** 23:
+            e -= 1
+        elif m >= 2 ** 24:
+            e += 1
+        else:
+            break
+    q, r = divmod(m.numerator, m.denominator)
+    if 2 * r > m.denominator or (2 * r == m.denominator and q % 2 == 1):
+        q += 1
+    return np.float32(sign * q * 2.0 ** (e - 23))
+
+
+def _fma_f32(a, b, c) -> np.float32:
+    return _round_f32(Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c)))
+
+
+def _traverse_exact(dims, sp, mat, pos, dirs, steps):
+    """Independent restatement of the tracking's voxel lookup: every FP32
+    operation is evaluated exactly in rationals and rounded once (numpy float32
+    for the plain operations, _round_f32 for the fused multiply-adds)."""
+    f = np.float32
+    bmin = [f(-0.5) * (f(dims[a]) * sp[a]) for a in range(3)]
+    bmax = [-b for b in bmin]
+    inv = [f(1.0) / sp[a] for a in range(3)]
+    off = [-bmin[a] * inv[a] for a in range(3)]
+    n_rays, n_steps = steps.shape
+    idx = np.full((n_rays, n_steps), -1, dtype=np.int64)
+    mm = np.full((n_rays, n_steps), -1, dtype=np.int32)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        for r in range(n_rays):
+            p = [pos[r, a] for a in range(3)]
+            d = [dirs[r, a] for a in range(3)]
+            t0, t1 = f(0.0), f(3.0e38)
+            for a in range(3):
+                i_d = f(1.0) / d[a]
+                ta, tb = (bmin[a] - p[a]) * i_d, (bmax[a] - p[a]) * i_d
+                t0 = np.fmax(t0, np.fmin(ta, tb))
+                t1 = np.fmin(t1, np.fmax(ta, tb))
+            if not t0 <= t1:
+                continue
+            p = [_fma_f32(t0, d[a], p[a]) for a in range(3)]
+            for k in range(n_steps):
+                p = [_fma_f32(steps[r, k], d[a], p[a]) for a in range(3)]
+                c = [int(np.floor(_fma_f32(p[a], inv[a], off[a]))) for a in range(3)]
+                if all(0 <= c[a] < dims[a] for a in range(3)):
+                    idx[r, k] = c[0] + dims[0] * (c[1] + dims[1] * c[2])
+                    mm[r, k] = mat[idx[r, k]]
+    return idx, mm
+
+
+def test_traverse_f32_exact_restatement(oracle):
+    rng = np.random.default_rng(11)
+    dims = (13, 9, 7)
+    sp = np.array([0.25, 0.5, 0.3], dtype=np.float32)
+    mat = rng.integers(0, 3, int(np.prod(dims))).astype(np.uint8)
+    pos, dirs, steps = random_rays(rng, dims, sp, 160, 12)
+    idx, mm = oracle.traverse_f32(dims, sp, mat, pos, dirs, steps)
+    eidx, emm = _traverse_exact(dims, sp, mat, pos, dirs, steps)
+    assert (idx >= 0).sum() > 160
+    np.testing.assert_array_equal(idx, eidx)
+    np.testing.assert_array_equal(mm, emm)
+
+
+@pytest.mark.parametrize("dims,spacing", [((13, 9, 7), (0.25, 0.5, 0.3)), ((64, 64, 64), (0.1, 0.1, 0.1))])
+def test_traverse_f32_face_points(oracle, dims, spacing):
+    """Lookups at and within 3 ulps of every voxel face: the oracle's choice of
+    voxel is the single-rounding fma restated exactly, and differs from both
+    an unfused (x * inv + off) and a (x - box_min) * inv lookup on some of
+    these points, so either of those mistakes fails here."""
+    rng = np.random.default_rng(5)
+    sp = np.array(spacing, dtype=np.float32)
+    mat = rng.integers(0, 3, int(np.prod(dims))).astype(np.uint8)
+    pos, dirs, steps = face_rays(rng, dims, sp, 2)
+    idx, mm = oracle.traverse_f32(dims, sp, mat, pos, dirs, steps)
+    eidx, emm = _traverse_exact(dims, sp, mat, pos, dirs, steps)
+    np.testing.assert_array_equal(idx, eidx)
+    np.testing.assert_array_equal(mm, emm)
+    # the alternative roundings pick other voxels on some of these points
+    f = np.float32
+    bmin = [f(-0.5) * (f(dims[a]) * sp[a]) for a in range(3)]
+    inv = [f(1.0) / sp[a] for a in range(3)]
+    off = [-bmin[a] * inv[a] for a in range(3)]
+    n_unfused = n_shifted = 0
+    for i in range(len(pos)):
+        for a in range(3):
+            x = pos[i, a]
+            fused = int(np.floor(_fma_f32(x, inv[a], off[a])))
+            n_unfused += fused != int(np.floor(f(x * inv[a]) + off[a]))
+            n_shifted += fused != int(np.floor((x - bmin[a]) * inv[a]))
+    assert n_unfused > 0 and n_shifted > 0
+
+
+def test_round_f32_matches_numpy():
+    rng = np.random.default_rng(2)
+    for x in rng.normal(size=200) * 10.0 ** rng.integers(-8, 8, 200):
+        assert _round_f32(Fraction(float(x))) == np.float32(x)
+    # halfway cases round to even
+    one = Fraction(1)
+    assert _round_f32(one + Fraction(1, 2 ** 24)) == np.float32(1.0)
+    assert _round_f32(one + Fraction(3, 2 ** 24)) == np.float32(1.0) + np.float32(2.0 ** -22)
