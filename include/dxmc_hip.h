@@ -197,6 +197,16 @@ int dxg_material_attenuation(const dxg_material* m, const double* energy_kev, ui
 int dxg_debug_traverse(const dxg_world* w, uint64_t n_rays, const float* pos, const float* dir,
                        uint64_t n_steps, const float* steps, int64_t* out_index, int32_t* out_material);
 
+/* ---- Source particles through the transport kernels' own sampling code
+ * (Exposure::sampleParticle, the refill path; geometry of
+ * beamactorcontainer.cpp:44-69): beam histories 0..n-1 of `beam`, each keyed
+ * by (seed, history index) as dxg_transport_run keys the first run on a fresh
+ * world.  out[8*h + {0..7}] = ax, ay (collimation angles, rad), direction
+ * (3), energy (keV), weight (exposure weight x bowtie weight at |ax|) and the
+ * exposure index.  Needs a built world (its device); n at most the beam's
+ * number of particles.  Scores are not touched. */
+int dxg_debug_source(dxg_world* w, const dxg_beam* beam, uint64_t seed, uint64_t n, float* out);
+
 /* ---- Profiling aid (no counterpart in the reference): `iters` launches of a
  * plain float4 copy kernel over `bytes` bytes (multiple of 16) between two
  * fresh device buffers.  Under a hardware-counter profiler its FETCH_SIZE and

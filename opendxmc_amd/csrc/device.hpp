@@ -144,6 +144,9 @@ void launchTransport(const DevParams& p, int blocks, int threads, hipStream_t s)
 void launchTraverse(const DevParams& p, uint64_t n_rays, const float* pos, const float* dir, uint64_t n_steps,
                     const float* steps, int64_t* out_index, int32_t* out_material, hipStream_t s);
 void launchReset(unsigned long long* pool, DevCounters* counters, hipStream_t s);
+// Source particles of beam histories [0, n): 8 floats each (ax, ay, direction,
+// E, w, exposure index), from the transport kernels' sampling code.
+void launchSource(const DevParams& p, uint64_t n, float* out, hipStream_t s);
 int transportBlockSize();
 void launchCopy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 // Resident blocks per CU of the transport kernel at the given LDS size.

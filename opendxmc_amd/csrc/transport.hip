@@ -324,6 +324,84 @@ __device__ inline void energyBin(float E, const float* s_maj, int& ej, float& et
     maj = CELLS ? s_maj[ej] : hwRcp(s_maj[ej]);
 }
 
+// Exposure::sampleParticle for beam history h (rng already seeded): the
+// exposure holding h, a direction uniform in the collimation half angles
+// (ax, ay) about the exposure's frame, the energy from the spectrum's alias
+// table and the weight times the bowtie weight at |ax| (both from the beam
+// blob, staged in LDS).  The photon starts at the exposure's source position.
+__device__ inline int sampleSource(const DevParams& P, const uint32_t* s_blob, unsigned long long h, Rng& rng, float& px, float& py,
+                                   float& pz, float& dx, float& dy, float& dz, float& E, float& w, float& ax, float& ay)
+{
+    // exposure containing beam history h
+    int ei;
+    if (P.ppe) {   // h / ppe from the double reciprocal, corrected by one either way
+        long long q = static_cast<long long>(static_cast<double>(h) * P.inv_ppe);
+        const unsigned long long lo = static_cast<unsigned long long>(q) * P.ppe;
+        if (lo > h)
+            --q;
+        else if (lo + P.ppe <= h)
+            ++q;
+        ei = static_cast<int>(min(static_cast<unsigned long long>(q), static_cast<unsigned long long>(P.n_ex - 1)));
+    } else {
+        int lo = 0, hi = P.n_ex - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (P.ex_start[mid] <= h)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        ei = lo;
+    }
+    const DevExposure& ex = P.exposures[ei];
+    ax = ex.half_angle[0] * (2.0f * rngU(rng) - 1.0f);
+    ay = ex.half_angle[1] * (2.0f * rngU(rng) - 1.0f);
+    const float sx = __sinf(ax), sy = __sinf(ay);
+    const float sz = hwSqrt(fmaxf(0.0f, 1.0f - sx * sx - sy * sy));
+    dx = ex.c0[0] * sx + ex.c1[0] * sy + ex.axis[0] * sz;
+    dy = ex.c0[1] * sx + ex.c1[1] * sy + ex.axis[1] * sz;
+    dz = ex.c0[2] * sx + ex.c1[2] * sy + ex.axis[2] * sz;
+    const float inv = rsqrtf(dx * dx + dy * dy + dz * dz);
+    dx *= inv;
+    dy *= inv;
+    dz *= inv;
+    px = ex.pos[0];
+    py = ex.pos[1];
+    pz = ex.pos[2];
+    {   // spectrum: Walker alias with one uniform, from LDS
+        const int off = static_cast<int>(s_blob[P.bb.o_spec_off + ex.spectrum]);
+        const int nb = static_cast<int>(s_blob[P.bb.o_spec_n + ex.spectrum]);
+        const float f = rngU(rng) * static_cast<float>(nb);
+        int k = min(static_cast<int>(f), nb - 1);
+        const float frac = f - static_cast<float>(k);
+        if (frac >= asFloat(s_blob[P.bb.o_prob + off + k]))
+            k = static_cast<int>(s_blob[P.bb.o_alias + off + k]);
+        E = asFloat(s_blob[P.bb.o_energy + off + k]);
+    }
+    w = ex.weight;
+    if (ex.bowtie >= 0) {   // piecewise linear in |angle|, from LDS
+        const int off = static_cast<int>(s_blob[P.bb.o_bow_off + ex.bowtie]);
+        const int nb = static_cast<int>(s_blob[P.bb.o_bow_n + ex.bowtie]);
+        const float* ang = reinterpret_cast<const float*>(s_blob + P.bb.o_angle) + off;
+        const float* wt = reinterpret_cast<const float*>(s_blob + P.bb.o_weight) + off;
+        const float a = fabsf(ax);
+        float bw;
+        if (a <= ang[0]) {
+            bw = wt[0];
+        } else if (a >= ang[nb - 1]) {
+            bw = wt[nb - 1];
+        } else {
+            int k = 0;
+            while (ang[k + 1] < a)
+                ++k;
+            const float t = (a - ang[k]) / (ang[k + 1] - ang[k]);
+            bw = wt[k] + t * (wt[k + 1] - wt[k]);
+        }
+        w *= bw;
+    }
+    return ei;
+}
+
 }  // namespace
 
 // Stages one instance's tables into LDS: the majorant, the attenuation table
@@ -432,74 +510,8 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     started = true;
                     collided = false;
                     rngSeed(rng, P.seed, h + P.rng_offset);
-                    // exposure containing beam history h
-                    int ei;
-                    if (P.ppe) {   // h / ppe from the double reciprocal, corrected by one either way
-                        long long q = static_cast<long long>(static_cast<double>(h) * P.inv_ppe);
-                        const unsigned long long lo = static_cast<unsigned long long>(q) * P.ppe;
-                        if (lo > h)
-                            --q;
-                        else if (lo + P.ppe <= h)
-                            ++q;
-                        ei = static_cast<int>(min(static_cast<unsigned long long>(q), static_cast<unsigned long long>(P.n_ex - 1)));
-                    } else {
-                        int lo = 0, hi = P.n_ex - 1;
-                        while (lo < hi) {
-                            const int mid = (lo + hi + 1) >> 1;
-                            if (P.ex_start[mid] <= h)
-                                lo = mid;
-                            else
-                                hi = mid - 1;
-                        }
-                        ei = lo;
-                    }
-                    const DevExposure& ex = P.exposures[ei];
-                    // Exposure::sampleParticle
-                    const float ax = ex.half_angle[0] * (2.0f * rngU(rng) - 1.0f);
-                    const float ay = ex.half_angle[1] * (2.0f * rngU(rng) - 1.0f);
-                    const float sx = __sinf(ax), sy = __sinf(ay);
-                    const float sz = hwSqrt(fmaxf(0.0f, 1.0f - sx * sx - sy * sy));
-                    dx = ex.c0[0] * sx + ex.c1[0] * sy + ex.axis[0] * sz;
-                    dy = ex.c0[1] * sx + ex.c1[1] * sy + ex.axis[1] * sz;
-                    dz = ex.c0[2] * sx + ex.c1[2] * sy + ex.axis[2] * sz;
-                    const float inv = rsqrtf(dx * dx + dy * dy + dz * dz);
-                    dx *= inv;
-                    dy *= inv;
-                    dz *= inv;
-                    px = ex.pos[0];
-                    py = ex.pos[1];
-                    pz = ex.pos[2];
-                    {   // spectrum: Walker alias with one uniform, from LDS
-                        const int off = static_cast<int>(s_blob[P.bb.o_spec_off + ex.spectrum]);
-                        const int nb = static_cast<int>(s_blob[P.bb.o_spec_n + ex.spectrum]);
-                        const float f = rngU(rng) * static_cast<float>(nb);
-                        int k = min(static_cast<int>(f), nb - 1);
-                        const float frac = f - static_cast<float>(k);
-                        if (frac >= asFloat(s_blob[P.bb.o_prob + off + k]))
-                            k = static_cast<int>(s_blob[P.bb.o_alias + off + k]);
-                        E = asFloat(s_blob[P.bb.o_energy + off + k]);
-                    }
-                    w = ex.weight;
-                    if (ex.bowtie >= 0) {   // piecewise linear in |angle|, from LDS
-                        const int off = static_cast<int>(s_blob[P.bb.o_bow_off + ex.bowtie]);
-                        const int nb = static_cast<int>(s_blob[P.bb.o_bow_n + ex.bowtie]);
-                        const float* ang = reinterpret_cast<const float*>(s_blob + P.bb.o_angle) + off;
-                        const float* wt = reinterpret_cast<const float*>(s_blob + P.bb.o_weight) + off;
-                        const float a = fabsf(ax);
-                        float bw;
-                        if (a <= ang[0]) {
-                            bw = wt[0];
-                        } else if (a >= ang[nb - 1]) {
-                            bw = wt[nb - 1];
-                        } else {
-                            int k = 0;
-                            while (ang[k + 1] < a)
-                                ++k;
-                            const float t = (a - ang[k]) / (ang[k + 1] - ang[k]);
-                            bw = wt[k] + t * (wt[k + 1] - wt[k]);
-                        }
-                        w *= bw;
-                    }
+                    float ax, ay;
+                    (void)sampleSource(P, s_blob, h, rng, px, py, pz, dx, dy, dz, E, w, ax, ay);
                     const double we = static_cast<double>(w) * static_cast<double>(E);
                     atomicAdd(&e_wave[0], we);
                     if (boxEntry(P, px, py, pz, dx, dy, dz)) {
@@ -984,6 +996,33 @@ __global__ void traverse_kernel(const DevParams P, uint64_t n_rays, const float*
     }
 }
 
+// Exposure::sampleParticle replayed for beam histories [0, n) through the
+// transport kernels' own sampleSource, with the beam blob staged in LDS as
+// they stage it.  out[8 h + {0..7}] = ax, ay, dx, dy, dz, E, w, exposure index.
+__global__ void source_kernel(const DevParams P, uint64_t n, float* __restrict__ out)
+{
+    extern __shared__ uint32_t lds[];
+    for (int i = threadIdx.x; i < P.bb.words; i += blockDim.x)
+        lds[i] = P.blob[i];
+    __syncthreads();
+    const uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (h >= n)
+        return;
+    Rng rng;
+    rngSeed(rng, P.seed, h + P.rng_offset);
+    float px, py, pz, dx, dy, dz, E, w, ax, ay;
+    const int ei = sampleSource(P, lds, h, rng, px, py, pz, dx, dy, dz, E, w, ax, ay);
+    float* o = out + 8 * h;
+    o[0] = ax;
+    o[1] = ay;
+    o[2] = dx;
+    o[3] = dy;
+    o[4] = dz;
+    o[5] = E;
+    o[6] = w;
+    o[7] = static_cast<float>(ei);
+}
+
 __global__ void reset_kernel(unsigned long long* pool, DevCounters* c)
 {
     if (threadIdx.x == 0) {
@@ -1085,6 +1124,12 @@ void launchTraverse(const DevParams& p, uint64_t n_rays, const float* pos, const
 {
     const unsigned blocks = static_cast<unsigned>((n_rays + 255) / 256);
     hipLaunchKernelGGL(traverse_kernel, dim3(blocks), dim3(256), 0, s, p, n_rays, pos, dir, n_steps, steps, out_index, out_material);
+}
+
+void launchSource(const DevParams& p, uint64_t n, float* out, hipStream_t s)
+{
+    const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+    hipLaunchKernelGGL(source_kernel, dim3(blocks), dim3(256), sizeof(uint32_t) * static_cast<size_t>(p.bb.words), s, p, n, out);
 }
 
 void launchReset(unsigned long long* pool, DevCounters* counters, hipStream_t s)

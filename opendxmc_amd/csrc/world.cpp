@@ -940,6 +940,28 @@ int dxg_debug_traverse(const dxg_world* w, uint64_t n_rays, const float* pos, co
     return DXG_OK;
 }
 
+int dxg_debug_source(dxg_world* w, const dxg_beam* beam, uint64_t seed, uint64_t n, float* out)
+{
+    if (!out && n)
+        return fail(DXG_ERR_ARGUMENT, "dxg_debug_source: null argument");
+    dxg::DevParams p;
+    uint64_t total = 0;
+    if (int rc = dxg::prepareRun(w, beam, seed, p, total); rc != DXG_OK)
+        return rc;
+    if (n > total)
+        return fail(DXG_ERR_ARGUMENT, "dxg_debug_source: n exceeds the beam's " + std::to_string(total) + " histories");
+    if (n == 0)
+        return DXG_OK;
+    p.rng_offset = 0;
+    DevBuf<float> d;
+    HIP_TRY(d.alloc(8 * n));
+    dxg::launchSource(p, n, d.ptr, w->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, d.ptr, 8 * n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    return DXG_OK;
+}
+
 int dxg_debug_device_copy(uint64_t bytes, int iters, double* ms)
 {
     if (bytes == 0 || bytes % 16 != 0 || iters < 1)

@@ -683,6 +683,17 @@ class Transport:
         return True
 
 
+def sample_source(world: World, beam, n: int, seed: int = 1) -> np.ndarray:
+    """Source particles of beam histories 0..n-1 from the transport kernels'
+    own sampling code: an (n, 8) float32 array of ax, ay, direction (3),
+    energy (keV), weight and exposure index."""
+    k = _Keep()
+    b = beam_struct(beam, k, calibrated=False)
+    out = np.zeros((int(n), 8), dtype=np.float32)
+    N.check(N.lib().dxg_debug_source(world.items[0]._h, C.byref(b), int(seed), int(n), N.ptr(out, C.c_float)))
+    return out
+
+
 class TransportBatch:
     """Many (world, beam) instances in one persistent device launch, replayed
     from a captured HIP graph (no counterpart in the reference).  Each

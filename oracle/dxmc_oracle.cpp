@@ -437,6 +437,23 @@ struct Particle {
     double p[3], d[3], E, w;
 };
 
+// Cumulative weights of a discrete spectrum (sampled by inversion).
+Spectrum make_spectrum(const oracle_spectrum& sp)
+{
+    Spectrum s;
+    double tot = 0;
+    for (uint64_t k = 0; k < sp.n; ++k)
+        tot += sp.weight[k];
+    double c = 0;
+    for (uint64_t k = 0; k < sp.n; ++k) {
+        c += sp.weight[k] / tot;
+        s.e.push_back(sp.energy[k]);
+        s.cdf.push_back(c);
+    }
+    s.cdf.back() = 1.0;
+    return s;
+}
+
 bool box_entry(const Grid& g, Particle& ph)
 {
     double t0 = 0, t1 = 1e300;
@@ -667,17 +684,15 @@ void score(const Run& R, long long i, double e, Acc& acc)
     acc.deposited += e;
 }
 
-void history(const Run& R, uint64_t h, Acc& acc)
+// Exposure::sampleParticle: collimation angles ax, ay uniform in +-half
+// angle; direction c0 sin(ax) + c1 sin(ay) + n sqrt(1 - sin^2 ax - sin^2 ay)
+// with n = c0 x c1 (pointsFromCollimations, beamactorcontainer.cpp:44-69);
+// energy by inversion of the spectrum's cumulative weights; weight = exposure
+// weight x bowtie weight at |ax| (BowtieFilter, bowtiefilterreader.cpp:92).
+void sample_source(const Run& R, const oracle_exposure& ex, Rng& rng, Particle& ph, double& ax, double& ay)
 {
-    const Grid& g = *R.g;
-    Rng rng(R.seed, h + R.offset);
-    const size_t ei = static_cast<size_t>(std::upper_bound(R.ex_start.begin(), R.ex_start.end(), h) - R.ex_start.begin()) - 1;
-    const oracle_exposure& ex = R.ex[ei];
-
-    // ---- sample the source particle (Exposure::sampleParticle)
-    Particle ph;
-    const double ax = ex.collimation_half_angles[0] * (2.0 * rng.u() - 1.0);
-    const double ay = ex.collimation_half_angles[1] * (2.0 * rng.u() - 1.0);
+    ax = ex.collimation_half_angles[0] * (2.0 * rng.u() - 1.0);
+    ay = ex.collimation_half_angles[1] * (2.0 * rng.u() - 1.0);
     const double sx = std::sin(ax), sy = std::sin(ay);
     const double sz = std::sqrt(std::max(0.0, 1.0 - sx * sx - sy * sy));
     const double* c0 = ex.cosines;
@@ -697,6 +712,19 @@ void history(const Run& R, uint64_t h, Acc& acc)
     ph.w = ex.weight;
     if (ex.bowtie_index >= 0)
         ph.w *= R.bowties[static_cast<size_t>(ex.bowtie_index)].eval(ax);
+}
+
+void history(const Run& R, uint64_t h, Acc& acc)
+{
+    const Grid& g = *R.g;
+    Rng rng(R.seed, h + R.offset);
+    const size_t ei = static_cast<size_t>(std::upper_bound(R.ex_start.begin(), R.ex_start.end(), h) - R.ex_start.begin()) - 1;
+    const oracle_exposure& ex = R.ex[ei];
+
+    // ---- sample the source particle (Exposure::sampleParticle)
+    Particle ph;
+    double ax, ay;
+    sample_source(R, ex, rng, ph, ax, ay);
     acc.emitted += ph.w * ph.E;
 
     if (!box_entry(g, ph)) {
@@ -882,6 +910,70 @@ int oracle_material_ia(const oracle_material* mat, double* fraction, double* bin
     return static_cast<int>(m.ia_f.size());
 }
 
+// Source particles of histories 0..n-1 of one exposure (spectrum and, when
+// bowtie is not null, bowtie): out[8 h + {0..7}] = ax, ay, direction (3), E,
+// w, 0, each history keyed by (seed, h).
+int oracle_sample_source(const oracle_exposure* ex, const oracle_spectrum* spectrum, const oracle_bowtie* bowtie, uint64_t seed,
+    uint64_t n, double* out)
+{
+    if (!ex || !spectrum || spectrum->n == 0 || !out) {
+        g_err = "oracle_sample_source: null or empty argument";
+        return -1;
+    }
+    Run R;
+    R.spectra.push_back(make_spectrum(*spectrum));
+    oracle_exposure e = *ex;
+    e.spectrum_index = 0;
+    e.bowtie_index = -1;
+    if (bowtie && bowtie->n) {
+        R.bowties.push_back(make_bowtie(*bowtie));
+        e.bowtie_index = 0;
+    }
+    for (uint64_t h = 0; h < n; ++h) {
+        Rng rng(seed, h);
+        Particle ph;
+        double ax, ay;
+        sample_source(R, e, rng, ph, ax, ay);
+        double* o = out + 8 * h;
+        o[0] = ax;
+        o[1] = ay;
+        o[2] = ph.d[0];
+        o[3] = ph.d[1];
+        o[4] = ph.d[2];
+        o[5] = ph.E;
+        o[6] = ph.w;
+        o[7] = 0.0;
+    }
+    return 0;
+}
+
+// DoseScore from EnergyScore over H histories (vgrid.doseScored(i).dose() /
+// .variance(), read at simulationpipeline.cpp:178,204; mGy and mGy^2,
+// datacontainer.hpp:109): the voxel's mass m = rho V (g -> kg), one keV is
+// 1.602176634e-16 J (SI, exact), dose = 1e3 sum_e[keV] e / m [mGy]; the
+// variance of that sum estimated from the scored squares,
+// H / (H - 1) (sum_e2 - sum_e^2 / H), in the same units squared (zero for
+// H <= 1 or a massless voxel).
+int oracle_dose_scores(uint64_t n, const double* density, const double spacing[3], uint64_t histories, const double* sum_e,
+    const double* sum_e2, double* dose, double* variance)
+{
+    if (!density || !spacing || !sum_e || !sum_e2 || !dose || !variance) {
+        g_err = "oracle_dose_scores: null argument";
+        return -1;
+    }
+    constexpr double kJoulePerKeV = 1.602176634e-16;
+    const double volume_cm3 = spacing[0] * spacing[1] * spacing[2];
+    const double H = static_cast<double>(histories);
+    for (uint64_t i = 0; i < n; ++i) {
+        const double mass_kg = density[i] * volume_cm3 / 1000.0;
+        const double mgy_per_kev = mass_kg > 0 ? 1000.0 * kJoulePerKeV / mass_kg : 0.0;
+        dose[i] = sum_e[i] * mgy_per_kev;
+        const double var_kev2 = H > 1 ? std::max(0.0, (sum_e2[i] - sum_e[i] * sum_e[i] / H) * (H / (H - 1.0))) : 0.0;
+        variance[i] = var_kev2 * mgy_per_kev * mgy_per_kev;
+    }
+    return 0;
+}
+
 // The xoshiro128+ stream of one history (for RNG cross-checks).
 void oracle_rng_stream(uint64_t seed, uint64_t history, uint64_t n, uint32_t* out)
 {
@@ -1009,20 +1101,8 @@ int oracle_transport(const uint64_t dims[3], const double spacing[3], const doub
     Run R;
     R.g = &g;
     R.ex.assign(exposures, exposures + n_exposures);
-    for (uint64_t i = 0; i < n_spectra; ++i) {
-        Spectrum s;
-        double tot = 0;
-        for (uint64_t k = 0; k < spectra[i].n; ++k)
-            tot += spectra[i].weight[k];
-        double c = 0;
-        for (uint64_t k = 0; k < spectra[i].n; ++k) {
-            c += spectra[i].weight[k] / tot;
-            s.e.push_back(spectra[i].energy[k]);
-            s.cdf.push_back(c);
-        }
-        s.cdf.back() = 1.0;
-        R.spectra.push_back(std::move(s));
-    }
+    for (uint64_t i = 0; i < n_spectra; ++i)
+        R.spectra.push_back(make_spectrum(spectra[i]));
     for (uint64_t i = 0; i < n_bowties; ++i)
         R.bowties.push_back(make_bowtie(bowties[i]));
     for (auto& e : R.ex) {

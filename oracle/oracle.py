@@ -61,6 +61,8 @@ def lib() -> C.CDLL:
         h.oracle_traverse_f32.argtypes = [C.POINTER(u64), C.POINTER(C.c_float), C.POINTER(C.c_uint8), u64,
                                           C.POINTER(C.c_float), C.POINTER(C.c_float), u64, C.POINTER(C.c_float),
                                           C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+        h.oracle_dose_scores.argtypes = [u64, PD, PD, u64, PD, PD, PD, PD]
+        h.oracle_sample_source.argtypes = [C.c_void_p, C.POINTER(Spectrum), C.POINTER(Bowtie), u64, u64, PD]
         h.oracle_transport.argtypes = [C.POINTER(u64), PD, PD, C.POINTER(C.c_uint8), C.POINTER(Material), u64,
                                        C.c_void_p, u64, C.POINTER(Spectrum), u64, C.POINTER(Bowtie), u64,
                                        C.c_int, u64, u64, C.c_int, PD, PD, C.POINTER(u64), C.POINTER(Counters)]
@@ -134,6 +136,40 @@ def impulse_shells(comp: dict[int, float]):
     if n < 0:
         raise RuntimeError(_err())
     return f[:n], b[:n], p[:n]
+
+
+def sample_source(exposure: np.ndarray, spectrum: tuple[np.ndarray, np.ndarray], bowtie=None, seed: int = 1,
+                  n: int = 1000) -> np.ndarray:
+    """(n, 8) array of ax, ay, direction (3), E, w, 0 for histories 0..n-1 of
+    one exposure record (the dxg_exposure layout) with one spectrum
+    (energies, weights) and optionally one bowtie (angles, weights)."""
+    k = _Keep()
+    keep_ex = k.keep(np.ascontiguousarray(np.atleast_1d(exposure)[:1]))
+    e = k.keep(np.ascontiguousarray(spectrum[0], dtype=np.float64))
+    w = k.keep(np.ascontiguousarray(spectrum[1], dtype=np.float64))
+    sp = Spectrum(e.ctypes.data_as(PD), w.ctypes.data_as(PD), len(e))
+    bp = None
+    if bowtie is not None:
+        a = k.keep(np.ascontiguousarray(bowtie[0], dtype=np.float64))
+        bw = k.keep(np.ascontiguousarray(bowtie[1], dtype=np.float64))
+        bp = C.byref(Bowtie(a.ctypes.data_as(PD), bw.ctypes.data_as(PD), len(a)))
+    out = np.zeros((n, 8))
+    if lib().oracle_sample_source(keep_ex.ctypes.data, C.byref(sp), bp, seed, n, out.ctypes.data_as(PD)) != 0:
+        raise RuntimeError(_err())
+    return out
+
+
+def dose_scores(density, spacing, histories: int, sum_e, sum_e2) -> tuple[np.ndarray, np.ndarray]:
+    """(dose mGy, variance mGy^2) per voxel from energy sums (keV, keV^2)."""
+    rho = np.ascontiguousarray(density, dtype=np.float64).ravel()
+    sp = np.ascontiguousarray(spacing, dtype=np.float64)
+    e = np.ascontiguousarray(sum_e, dtype=np.float64).ravel()
+    e2 = np.ascontiguousarray(sum_e2, dtype=np.float64).ravel()
+    dose, var = np.zeros(len(rho)), np.zeros(len(rho))
+    if lib().oracle_dose_scores(len(rho), rho.ctypes.data_as(PD), sp.ctypes.data_as(PD), int(histories), e.ctypes.data_as(PD),
+                                e2.ctypes.data_as(PD), dose.ctypes.data_as(PD), var.ctypes.data_as(PD)) != 0:
+        raise RuntimeError(_err())
+    return dose, var
 
 
 def rng_stream(seed: int, history: int, n: int) -> np.ndarray:

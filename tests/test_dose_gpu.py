@@ -1,0 +1,103 @@
+"""DoseScore on the device (dxg_read_dose_scores; simulationpipeline.cpp:
+174-229, mGy per datacontainer.hpp:109):
+  * the conversion of the device's own energy scores equals the oracle's
+    restatement for every voxel;
+  * thin-slab primary KERMA per fluence equals E mu_tr / rho (Klein-Nishina
+    energy transfer at correction level 0), with the slab's attenuation;
+  * the reported variance matches the spread of 40 independent seeds, voxel
+    by voxel (chi-square)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import opendxmc_amd as dx
+from opendxmc_amd import phantoms
+from physics_refs import MGY_PER_KEV_G, kn_transfer_fraction
+
+pytestmark = pytest.mark.gpu
+
+def world_of(ph, level):
+    world = dx.World()
+    grid = world.addItem(dx.AAVoxelGrid(correction=level))
+    grid.setData(ph.dims, ph.density, ph.material, [dx.Material.byWeight(m) for m in ph.materials])
+    grid.setSpacing(ph.spacing)
+    world.build()
+    return world, grid
+
+
+def test_dose_conversion_matches_oracle(oracle):
+    rng = np.random.default_rng(2)
+    ph = phantoms.water_cube(16, 0.25)
+    dens = ph.density * rng.uniform(0.5, 2.0, ph.density.size)
+    ph = phantoms.Phantom(ph.dims, ph.spacing, dens, ph.material, ph.materials, ph.names)
+    world, grid = world_of(ph, 1)
+    beam = dx.PencilBeam((0.1, 0.05, -10.0), (0.1, 0.0, 1.0), 60.0)
+    beam.setNumberOfParticlesPerExposure(300_000)
+    assert dx.Transport(seed=3)(world, beam, None, False)
+    e, e2, ev = grid.energyScored()
+    d, v, ev2 = grid.doseScored()
+    od, ov = oracle.dose_scores(dens, ph.spacing, grid.histories(), e, e2)
+    np.testing.assert_array_equal(ev, ev2)
+    np.testing.assert_allclose(d, od, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(v, ov, rtol=1e-12, atol=0)
+    assert (d > 0).sum() > 100
+
+
+@pytest.mark.parametrize("energy", [30.0, 80.0])
+def test_thin_slab_kerma_per_fluence(oracle, energy):
+    """A pencil beam of H photons through one 0.02 cm water voxel (vacuum
+    around it): the primary energy imparted is H E (mu_tr / mu)(1 - exp(-mu t))
+    with mu_tr = mu_pe + mu_inc f_KN(E), so dose / fluence -> E mu_tr / rho as
+    t -> 0.  Second interactions of scattered photons in so small a voxel add
+    below 0.3 %; tolerance 1 % plus 3 reported standard errors."""
+    t = 0.02
+    ph = phantoms.Phantom((1, 1, 1), (t, t, t), np.ones(1), np.zeros(1, np.uint8), [phantoms.WATER], ["water"])
+    world, grid = world_of(ph, 0)
+    H = 100_000_000
+    beam = dx.PencilBeam((0.0, 0.0, -10.0), (0.0, 0.0, 1.0), energy)
+    beam.setNumberOfParticlesPerExposure(H)
+    assert dx.Transport(seed=11)(world, beam, None, False)
+    d, v, _ = grid.doseScored()
+    pe, inc, coh, tot = oracle.attenuation(phantoms.WATER, [energy])[0]
+    mu_tr = pe + inc * kn_transfer_fraction(energy)   # cm2/g, rho = 1
+    fluence = H / (t * t)
+    expected = fluence * energy * mu_tr * (1.0 - math.exp(-tot * t)) / (tot * t) * MGY_PER_KEV_G
+    se = math.sqrt(v[0])
+    rel = d[0] / expected - 1.0
+    print(f"E {energy} keV: dose {d[0]:.6e} mGy, expected {expected:.6e} (mu_tr/rho {mu_tr:.5f} cm2/g), "
+          f"rel {rel:+.4f}, SE {se / d[0]:.4f}", flush=True)
+    assert abs(d[0] - expected) <= 0.01 * expected + 3 * se
+
+
+def test_variance_matches_spread_of_seeds():
+    """Per voxel above 10 % of the maximum, the sample variance of 40 seeds'
+    doses against the mean reported variance: (K-1) s^2 / sigma^2 is chi-square
+    with K-1 degrees of freedom, so at most a few voxels may fall outside its
+    central 99.8 %, and the mean ratio s^2 / sigma^2 stays within 10 % of 1."""
+    ph = phantoms.water_cube(16, 0.4)
+    beam = dx.PencilBeam((0.2, 0.2, -10.0), (0, 0, 1), 60.0)
+    beam.setNumberOfParticlesPerExposure(200_000)
+    world, grid = world_of(ph, 1)
+    K = 40
+    doses, variances = [], []
+    for seed in range(K):
+        grid.clearScores()
+        dx.Transport(seed=100 + seed)(world, beam, None, False)
+        d, var, _ = grid.doseScored()
+        doses.append(d)
+        variances.append(var)
+    doses = np.array(doses)
+    sel = doses.mean(0) > 0.1 * doses.mean(0).max()
+    ratio = doses.var(0, ddof=1)[sel] / np.array(variances).mean(0)[sel]
+    q = (K - 1) * ratio
+    lo, hi = stats.chi2.ppf([0.001, 0.999], K - 1)
+    n_out = int(np.sum((q < lo) | (q > hi)))
+    print(f"{sel.sum()} voxels: mean s^2/sigma^2 {ratio.mean():.3f}, outside the 99.8 % chi-square band {n_out}",
+          flush=True)
+    assert sel.sum() >= 20
+    assert abs(ratio.mean() - 1.0) < 0.10
+    assert n_out <= max(1, int(0.03 * sel.sum()))
