@@ -17,6 +17,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <optional>
 #include <random>
 #include <string>
@@ -302,6 +303,56 @@ private:
     std::vector<double> m_weights;
 };
 
+// The organ AEC filter of CT spiral beams (organAECFilter(); settings at
+// beamsettingsmodel.cpp:349-437): weight by gantry angle, dxg_organ_aec.
+class CTOrganAECFilter {
+public:
+    bool useFilter() const { return m_f.use != 0; }
+    void setUseFilter(bool on) { m_f.use = on ? 1 : 0; }
+    bool compensateOutside() const { return m_f.compensate_outside != 0; }
+    void setCompensateOutside(bool on) { m_f.compensate_outside = on ? 1 : 0; }
+    double startAngle() const { return m_f.start_angle; }
+    double stopAngle() const { return m_f.stop_angle; }
+    double rampAngle() const { return m_f.ramp_angle; }
+    void setStartAngle(double rad) { m_f.start_angle = rad; }
+    void setStopAngle(double rad) { m_f.stop_angle = rad; }
+    void setRampAngle(double rad) { m_f.ramp_angle = rad; }
+    double startAngleDeg() const { return m_f.start_angle * 57.29577951308232; }
+    double stopAngleDeg() const { return m_f.stop_angle * 57.29577951308232; }
+    double rampAngleDeg() const { return m_f.ramp_angle * 57.29577951308232; }
+    void setStartAngleDeg(double d) { m_f.start_angle = d / 57.29577951308232; }
+    void setStopAngleDeg(double d) { m_f.stop_angle = d / 57.29577951308232; }
+    void setRampAngleDeg(double d) { m_f.ramp_angle = d / 57.29577951308232; }
+    double lowWeight() const { return m_f.low_weight; }
+    void setLowWeightFactor(double w) { m_f.low_weight = w; }
+    double maxWeight() const
+    {
+        double w = 1.0;
+        return dxg_organ_aec_max_weight(&m_f, &w) == DXG_OK ? w : 1.0;
+    }
+    const dxg_organ_aec& view() const { return m_f; }
+
+private:
+    dxg_organ_aec m_f { 0, 0, 0.0, 0.0, 0.0, 1.0 };
+};
+
+// DataContainer::calculateAECfilterFromWaterEquivalentDiameter
+// (datacontainer.cpp:42-100) for an image of dims (index x + y nx + z nx ny)
+// and spacing in cm: a CT image in HU, or densities in g/cm3.  Empty on bad input.
+inline CTAECFilter calculateAECfilterFromWaterEquivalentDiameter(const std::array<std::size_t, 3>& dims,
+                                                                 const std::array<double, 3>& spacing,
+                                                                 const std::vector<double>& image, bool isCT)
+{
+    const std::uint64_t d[3] = { dims[0], dims[1], dims[2] };
+    if (image.size() != dims[0] * dims[1] * dims[2])
+        return CTAECFilter {};
+    std::vector<double> w(dims[2]);
+    std::array<double, 3> start {}, stop {};
+    if (dxg_aec_from_wed(d, spacing.data(), image.data(), isCT ? 1 : 0, w.data(), start.data(), stop.data()) != DXG_OK)
+        return CTAECFilter {};
+    return CTAECFilter(start, stop, w);
+}
+
 // --------------------------------------------------------------------- tube
 class Tube {
 public:
@@ -412,11 +463,46 @@ struct RotationTube {
     bool bowtie;
 };
 
+inline double ctdiCalibrationRun(const std::vector<RotationTube>& tubes, double step, double coll, double sdd, double ctdi_w,
+                                 double diameter, std::uint64_t run_ppe);
+
 inline double ctdiCalibration(const std::vector<RotationTube>& tubes, double step, double coll, double sdd, double ctdi_w,
                               double diameter, std::uint64_t run_ppe)
 {
     if (!(ctdi_w > 0))
         return 0.0;
+    // The factor depends on these settings only: a CTDI phantom run per
+    // distinct setting, cached for the process (every Transport call asks).
+    std::vector<double> key { step, coll, sdd, ctdi_w, diameter, static_cast<double>(run_ppe) };
+    for (const auto& rt : tubes) {
+        key.insert(key.end(), { rt.start_angle, rt.fov, rt.mas, rt.bowtie ? 1.0 : 0.0 });
+        key.insert(key.end(), rt.tube->energies[0].begin(), rt.tube->energies[0].end());
+        key.insert(key.end(), rt.tube->weights[0].begin(), rt.tube->weights[0].end());
+        if (rt.bowtie) {
+            key.insert(key.end(), rt.tube->bowties[0].angles().begin(), rt.tube->bowties[0].angles().end());
+            key.insert(key.end(), rt.tube->bowties[0].weights().begin(), rt.tube->bowties[0].weights().end());
+        }
+        key.push_back(-1.0);   // separator between tubes
+    }
+    static std::mutex cache_mutex;
+    static std::map<std::vector<double>, double> cache;
+    {
+        std::lock_guard<std::mutex> lock(cache_mutex);
+        if (auto it = cache.find(key); it != cache.end())
+            return it->second;
+    }
+    const double factor = ctdiCalibrationRun(tubes, step, coll, sdd, ctdi_w, diameter, run_ppe);
+    if (factor > 0) {
+        std::lock_guard<std::mutex> lock(cache_mutex);
+        cache.emplace(std::move(key), factor);
+    }
+    return factor;
+}
+
+// The CTDI phantom run behind ctdiCalibration (uncached).
+inline double ctdiCalibrationRun(const std::vector<RotationTube>& tubes, double step, double coll, double sdd, double ctdi_w,
+                                 double diameter, std::uint64_t run_ppe)
+{
     constexpr std::uint64_t kPpe = 100000;   // particles per rotation exposure
     BeamTables rot;
     for (const auto& rt : tubes) {
@@ -554,7 +640,7 @@ private:
     std::array<double, 3> m_pos, m_dir;
     double m_energy;
     std::uint64_t m_nexp = 1;
-    double m_airKerma = 1;
+    double m_airKerma = 0;   // 0: uncalibrated (the Python beams' default too)
 };
 
 template <bool ENABLETRACKING = false>
@@ -603,7 +689,7 @@ private:
     std::array<std::array<double, 3>, 2> m_cos;
     std::array<double, 2> m_half { 0.1, 0.1 };
     std::uint64_t m_nexp = 1;
-    double m_dap = 1;
+    double m_dap = 0;   // 0: uncalibrated (the Python beams' default too)
 };
 
 template <bool ENABLETRACKING = false>
@@ -640,6 +726,8 @@ public:
     void setBowtieFilter(const BowtieFilter& b) { m_bowtie = b; }
     const CTAECFilter& AECFilter() const { return m_aec; }
     void setAECFilter(const CTAECFilter& a) { m_aec = a; }
+    CTOrganAECFilter& organAECFilter() { return m_organAec; }
+    const CTOrganAECFilter& organAECFilter() const { return m_organAec; }
 
     std::vector<dxg_exposure> exposures() const override { return collect(params(), dxg_ct_spiral_exposures); }
     BeamTables tables() const override
@@ -677,13 +765,15 @@ protected:
         p.relative_mas_b = 1;
         p.fov_b_cm = m_fov;
         p.aec = m_aec.view();
+        p.organ_aec = m_organAec.view();
         return p;
     }
     std::array<double, 3> m_start, m_stop;
     double m_fov = 50, m_sdd = 119, m_coll = 3.84, m_startAngle = 0, m_stepAngle = 5 * DEG_TO_RAD(), m_pitch = 1;
-    double m_ctdivol = 1, m_ctdiDiameter = 32;
+    double m_ctdivol = 0, m_ctdiDiameter = 32;   // CTDIvol 0: uncalibrated (the Python beams' default too)
     BowtieFilter m_bowtie;
     CTAECFilter m_aec;
+    CTOrganAECFilter m_organAec;
 };
 
 template <bool ENABLETRACKING = false>
@@ -847,7 +937,7 @@ private:
     std::array<double, 3> m_pos, m_normal;
     std::uint64_t m_slices = 1;
     double m_spacing = 1, m_fov = 50, m_sdd = 119, m_coll = 3.84, m_startAngle = 0, m_stepAngle = 5 * DEG_TO_RAD();
-    double m_ctdiw = 1, m_ctdiDiameter = 32;
+    double m_ctdiw = 0, m_ctdiDiameter = 32;   // CTDIw 0: uncalibrated (the Python beams' default too)
     BowtieFilter m_bowtie;
 };
 
@@ -901,7 +991,7 @@ private:
     std::array<double, 3> m_iso, m_axis;
     double m_sdd = 120, m_start = -90 * DEG_TO_RAD(), m_stop = 90 * DEG_TO_RAD(), m_step = 1 * DEG_TO_RAD();
     std::array<double, 2> m_half { 10 * DEG_TO_RAD(), 10 * DEG_TO_RAD() };
-    double m_dap = 1;
+    double m_dap = 0;   // 0: uncalibrated (the Python beams' default too)
 };
 
 // --------------------------------------------------------------- transport

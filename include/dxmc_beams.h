@@ -56,6 +56,44 @@ typedef struct dxg_aec {
     uint64_t n;
 } dxg_aec;
 
+/* The organ AEC filter of CT spiral beams (organAECFilter(), settings at
+ * beamsettingsmodel.cpp:349-437, stored at hdf5wrapper.cpp:576-580): tube
+ * current modulated by gantry angle.  Angles in radians, measured as the
+ * beam's start_angle (the gantry angle a of the exposure's tube).  Over the
+ * arc from start_angle counter-clockwise to stop_angle the weight is
+ * low_weight; over ramp_angle on each side outside the arc it rises linearly
+ * to the high weight, which holds elsewhere.  The high weight is 1, or with
+ * compensate_outside the value that keeps the mean weight over a full turn at
+ * 1: (2 pi - low (L + r)) / (2 pi - L - r) for arc L and ramp r (1 when the
+ * arc and ramps cover the whole turn).  use = 0 gives weight 1 everywhere.
+ * DXMClib's own filter is not in the reference tree: this is the build's
+ * definition from those settings. */
+typedef struct dxg_organ_aec {
+    int32_t use;
+    int32_t compensate_outside;
+    double start_angle;
+    double stop_angle;
+    double ramp_angle;
+    double low_weight;
+} dxg_organ_aec;
+
+/* The filter's weight at gantry angle `angle` (rad), and its high weight
+ * (maxWeight(), the "High weight" item of beamsettingsmodel.cpp:423-428). */
+int dxg_organ_aec_weight(const dxg_organ_aec* f, double angle, double* weight);
+int dxg_organ_aec_max_weight(const dxg_organ_aec* f, double* weight);
+
+/* CTAECFilter from the water-equivalent diameter per slice
+ * (DataContainer::calculateAECfilterFromWaterEquivalentDiameter,
+ * datacontainer.cpp:42-100): for slice z of a dims[0] x dims[1] x dims[2] image
+ * (index x + y nx + z nx ny, spacing in cm), the water-equivalent area is
+ * A_w = (mean HU / 1000 + 1) nx ny dx dy for a CT image (image_is_ct = 1) or
+ * sum(density) dx dy for a density image (g/cm3), the diameter
+ * D_w = 2 sqrt(max(A_w, 0) / pi), and the weight exp(0.2 D_w) (mu of water
+ * about 0.2 /cm).  weights receives dims[2] values; the filter runs from
+ * (0, 0, -L) to (0, 0, L), L = dims[2] dz / 2, written to start and stop. */
+int dxg_aec_from_wed(const uint64_t dims[3], const double spacing[3], const double* image, int image_is_ct, double* weights,
+                     double start[3], double stop[3]);
+
 /* PencilBeam<false>: position, direction, energy (20..150 keV clamp,
  * beamsettingsmodel.cpp:664), numberOfExposures, numberOfParticlesPerExposure.
  * Uses spectrum 0, no bowtie. */
@@ -80,7 +118,9 @@ typedef struct dxg_dx_beam {
 /* CTSpiralBeam<false> / CTSpiralDualEnergyBeam<false> (beamsettingsmodel.cpp:1150-1837,
  * hdf5wrapper.cpp:548-585).  The gantry turns about start->stop; the table
  * advances pitch * collimation per turn.  Exposure i sits at angle
- * start_angle + i * step_angle.  For dual energy, tube B is offset by
+ * start_angle + i * step_angle; its weight is the AEC weight at its position
+ * times the organ AEC weight at its tube's angle (times the tube's relative
+ * mAs for dual source).  For dual energy, tube B is offset by
  * tube_b_offset_angle and exposures interleave A, B (beamactorcontainer.cpp:135-146),
  * tube A uses spectrum/bowtie 0 with weight relative_mas_a, tube B spectrum/bowtie 1
  * with weight relative_mas_b; fov_b is tube B's field of view.  has_bowtie = 0
@@ -102,6 +142,7 @@ typedef struct dxg_ct_spiral_beam {
     double relative_mas_a;
     double relative_mas_b;
     dxg_aec aec;
+    dxg_organ_aec organ_aec;   /* applied to each tube at its own gantry angle */
 } dxg_ct_spiral_beam;
 
 /* CTSequentialBeam<false> (beamsettingsmodel.cpp:900-1150): number_of_slices

@@ -86,6 +86,11 @@ class AEC(C.Structure):
     _fields_ = [("start", C.c_double * 3), ("stop", C.c_double * 3), ("weights", PD), ("n", u64)]
 
 
+class OrganAEC(C.Structure):
+    _fields_ = [("use", C.c_int32), ("compensate_outside", C.c_int32), ("start_angle", C.c_double),
+                ("stop_angle", C.c_double), ("ramp_angle", C.c_double), ("low_weight", C.c_double)]
+
+
 class PencilParams(C.Structure):
     _fields_ = [("position", C.c_double * 3), ("direction", C.c_double * 3), ("n_exposures", u64),
                 ("particles_per_exposure", u64)]
@@ -102,7 +107,7 @@ class CTSpiralParams(C.Structure):
                 ("sdd_cm", C.c_double), ("fov_cm", C.c_double), ("particles_per_exposure", u64),
                 ("dual_source", C.c_int32), ("has_bowtie", C.c_int32), ("fov_b_cm", C.c_double),
                 ("tube_b_offset_angle", C.c_double), ("relative_mas_a", C.c_double), ("relative_mas_b", C.c_double),
-                ("aec", AEC)]
+                ("aec", AEC), ("organ_aec", OrganAEC)]
 
 
 class CTSequentialParams(C.Structure):
@@ -184,6 +189,9 @@ SIGNATURES = [
     ("dxg_tube_spectrum", C.c_int, [C.POINTER(Tube), PD, PD, u64, C.POINTER(u64)]),
     ("dxg_tube_hvl_mm_al", C.c_int, [C.POINTER(Tube), PD]),
     ("dxg_tube_mean_energy", C.c_int, [C.POINTER(Tube), PD]),
+    ("dxg_organ_aec_weight", C.c_int, [C.POINTER(OrganAEC), C.c_double, PD]),
+    ("dxg_organ_aec_max_weight", C.c_int, [C.POINTER(OrganAEC), PD]),
+    ("dxg_aec_from_wed", C.c_int, [C.POINTER(u64), PD, PD, C.c_int, PD, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("dxg_pencil_exposures", C.c_int, [C.POINTER(PencilParams), C.POINTER(Exposure), u64, C.POINTER(u64)]),
     ("dxg_dx_exposures", C.c_int, [C.POINTER(DXParams), C.POINTER(Exposure), u64, C.POINTER(u64)]),
     ("dxg_ct_spiral_exposures", C.c_int, [C.POINTER(CTSpiralParams), C.POINTER(Exposure), u64, C.POINTER(u64)]),

@@ -114,6 +114,51 @@ struct Aec {
     }
 };
 
+// Organ AEC: weight by gantry angle (see dxg_organ_aec in dxmc_beams.h).
+struct OrganAec {
+    bool on = false;
+    double start = 0, arc = 0, ramp = 0, low = 1, high = 1;
+    bool init(const dxg_organ_aec& f, std::string& err)
+    {
+        if (!f.use)
+            return true;
+        if (!(f.low_weight >= 0) || !std::isfinite(f.low_weight) || !std::isfinite(f.start_angle) || !std::isfinite(f.stop_angle)
+            || !(f.ramp_angle >= 0) || !std::isfinite(f.ramp_angle)) {
+            err = "organ AEC: angles must be finite, ramp and low weight non-negative";
+            return false;
+        }
+        constexpr double two_pi = 2.0 * kPi;
+        on = true;
+        start = f.start_angle;
+        arc = std::fmod(f.stop_angle - f.start_angle, two_pi);
+        if (arc < 0)
+            arc += two_pi;
+        ramp = std::min(f.ramp_angle, 0.5 * (two_pi - arc));
+        low = f.low_weight;
+        high = 1.0;
+        const double rest = two_pi - arc - ramp;   // the turn's weight budget outside the low arc and half the ramps
+        if (f.compensate_outside && rest > 1e-12)
+            high = (two_pi - low * (arc + ramp)) / rest;
+        return true;
+    }
+    double at(double a) const
+    {
+        if (!on)
+            return 1.0;
+        constexpr double two_pi = 2.0 * kPi;
+        double x = std::fmod(a - start, two_pi);
+        if (x < 0)
+            x += two_pi;
+        if (x <= arc)
+            return low;
+        if (ramp > 0 && x < arc + ramp)
+            return low + (high - low) * (x - arc) / ramp;
+        if (ramp > 0 && x > two_pi - ramp)
+            return low + (high - low) * (two_pi - x) / ramp;
+        return high;
+    }
+};
+
 int finish(uint64_t count, uint64_t cap, dxg_exposure* out, uint64_t* n)
 {
     if (n)
@@ -377,6 +422,57 @@ int dxg_dx_exposures(const dxg_dx_beam* b, dxg_exposure* out, uint64_t cap, uint
     return DXG_OK;
 }
 
+int dxg_organ_aec_weight(const dxg_organ_aec* f, double angle, double* weight)
+{
+    if (!f || !weight)
+        return fail(DXG_ERR_ARGUMENT, "organ AEC: null argument");
+    std::string err;
+    OrganAec o;
+    if (!o.init(*f, err))
+        return fail(DXG_ERR_ARGUMENT, err);
+    *weight = o.at(angle);
+    return DXG_OK;
+}
+
+int dxg_organ_aec_max_weight(const dxg_organ_aec* f, double* weight)
+{
+    if (!f || !weight)
+        return fail(DXG_ERR_ARGUMENT, "organ AEC: null argument");
+    std::string err;
+    OrganAec o;
+    if (!o.init(*f, err))
+        return fail(DXG_ERR_ARGUMENT, err);
+    *weight = o.on ? o.high : 1.0;
+    return DXG_OK;
+}
+
+int dxg_aec_from_wed(const uint64_t dims[3], const double spacing[3], const double* image, int image_is_ct, double* weights,
+                     double start[3], double stop[3])
+{
+    if (!dims || !spacing || !image || !weights || !start || !stop)
+        return fail(DXG_ERR_ARGUMENT, "AEC from WED: null argument");
+    if (dims[0] == 0 || dims[1] == 0 || dims[2] == 0 || !(spacing[0] > 0) || !(spacing[1] > 0) || !(spacing[2] > 0))
+        return fail(DXG_ERR_ARGUMENT, "AEC from WED: empty image or non-positive spacing");
+    constexpr double kMuWater = 0.2;   // 1/cm, about that of water at 60-70 keV
+    const uint64_t step = dims[0] * dims[1];
+    const double pixel_cm2 = spacing[0] * spacing[1];
+    for (uint64_t z = 0; z < dims[2]; ++z) {
+        const double* slice = image + z * step;
+        double sum = 0;
+        for (uint64_t i = 0; i < step; ++i)
+            sum += slice[i];
+        const double area = image_is_ct ? (sum / static_cast<double>(step) / 1000.0 + 1.0) * static_cast<double>(step) * pixel_cm2
+                                        : sum * pixel_cm2;
+        const double diameter = 2.0 * std::sqrt(std::max(area, 0.0) / kPi);
+        weights[z] = std::exp(kMuWater * diameter);
+    }
+    const double half = 0.5 * spacing[2] * static_cast<double>(dims[2]);
+    start[0] = start[1] = stop[0] = stop[1] = 0.0;
+    start[2] = -half;
+    stop[2] = half;
+    return DXG_OK;
+}
+
 int dxg_ct_spiral_exposures(const dxg_ct_spiral_beam* b, dxg_exposure* out, uint64_t cap, uint64_t* n)
 {
     if (!b)
@@ -388,6 +484,9 @@ int dxg_ct_spiral_exposures(const dxg_ct_spiral_beam* b, dxg_exposure* out, uint
     std::string err;
     Aec aec;
     if (!aec.init(b->aec, err))
+        return fail(DXG_ERR_ARGUMENT, err);
+    OrganAec organ;
+    if (!organ.init(b->organ_aec, err))
         return fail(DXG_ERR_ARGUMENT, err);
     const V3 s = v3(b->start), e = v3(b->stop);
     const double len = norm(e - s);
@@ -410,11 +509,12 @@ int dxg_ct_spiral_exposures(const dxg_ct_spiral_beam* b, dxg_exposure* out, uint
         const V3 c = s + (static_cast<double>(i) * b->step_angle / (2.0 * kPi) * advance) * axis;
         const double w = aec.at(c);
         if (!b->dual_source) {
-            gantryExposure(out[i], c, axis, e1, e2, a, R, hxa, hy, w, 0, b->has_bowtie ? 0 : -1, b->particles_per_exposure);
+            gantryExposure(out[i], c, axis, e1, e2, a, R, hxa, hy, w * organ.at(a), 0, b->has_bowtie ? 0 : -1, b->particles_per_exposure);
         } else {
-            gantryExposure(out[2 * i], c, axis, e1, e2, a, R, hxa, hy, w * b->relative_mas_a, 0, b->has_bowtie ? 0 : -1,
+            const double ab = a + b->tube_b_offset_angle;
+            gantryExposure(out[2 * i], c, axis, e1, e2, a, R, hxa, hy, w * b->relative_mas_a * organ.at(a), 0, b->has_bowtie ? 0 : -1,
                            b->particles_per_exposure);
-            gantryExposure(out[2 * i + 1], c, axis, e1, e2, a + b->tube_b_offset_angle, R, hxb, hy, w * b->relative_mas_b, 1,
+            gantryExposure(out[2 * i + 1], c, axis, e1, e2, ab, R, hxb, hy, w * b->relative_mas_b * organ.at(ab), 1,
                            b->has_bowtie ? 1 : -1, b->particles_per_exposure);
         }
     }

@@ -255,6 +255,75 @@ class CTAECFilter:
     stop: tuple = (0.0, 0.0, 1.0)
     weights: tuple = ()
 
+    def isEmpty(self) -> bool:
+        return len(self.weights) == 0
+
+    @staticmethod
+    def fromWaterEquivalentDiameter(dims, spacing, image, is_ct: bool = True) -> "CTAECFilter":
+        """DataContainer::calculateAECfilterFromWaterEquivalentDiameter
+        (datacontainer.cpp:42-100): weight exp(0.2 D_w) per z slice of a CT
+        (HU) or density image, from (0, 0, -L) to (0, 0, L)."""
+        d = np.ascontiguousarray(dims, dtype=np.uint64)
+        sp = np.ascontiguousarray(spacing, dtype=np.float64)
+        img = np.ascontiguousarray(image, dtype=np.float64).ravel()
+        if img.size != int(np.prod(d)):
+            raise ValueError("image size does not match dims")
+        w = np.zeros(int(d[2]))
+        start, stop = (C.c_double * 3)(), (C.c_double * 3)()
+        N.check(N.lib().dxg_aec_from_wed(N.ptr(d, N.u64), N.ptr(sp, C.c_double), N.ptr(img, C.c_double), int(bool(is_ct)),
+                                         N.ptr(w, C.c_double), start, stop))
+        return CTAECFilter(tuple(start), tuple(stop), tuple(w))
+
+
+class CTOrganAECFilter:
+    """The organ AEC filter of CT spiral beams (organAECFilter(); settings at
+    beamsettingsmodel.cpp:349-437, stored at hdf5wrapper.cpp:576-580): a low
+    tube-current weight over a gantry-angle arc, linear ramps beside it, and
+    optionally a raised weight elsewhere that keeps the mean over a turn at 1."""
+
+    def __init__(self):
+        self._use = False
+        self._compensate = False
+        self._start = 0.0
+        self._stop = 0.0
+        self._ramp = 0.0
+        self._low = 1.0
+
+    def setUseFilter(self, on: bool) -> None: self._use = bool(on)
+    def useFilter(self) -> bool: return self._use
+    def setCompensateOutside(self, on: bool) -> None: self._compensate = bool(on)
+    def compensateOutside(self) -> bool: return self._compensate
+    def setStartAngle(self, rad: float) -> None: self._start = float(rad)
+    def setStopAngle(self, rad: float) -> None: self._stop = float(rad)
+    def setRampAngle(self, rad: float) -> None: self._ramp = float(rad)
+    def startAngle(self) -> float: return self._start
+    def stopAngle(self) -> float: return self._stop
+    def rampAngle(self) -> float: return self._ramp
+    def setStartAngleDeg(self, deg: float) -> None: self._start = math.radians(deg)
+    def setStopAngleDeg(self, deg: float) -> None: self._stop = math.radians(deg)
+    def setRampAngleDeg(self, deg: float) -> None: self._ramp = math.radians(deg)
+    def startAngleDeg(self) -> float: return math.degrees(self._start)
+    def stopAngleDeg(self) -> float: return math.degrees(self._stop)
+    def rampAngleDeg(self) -> float: return math.degrees(self._ramp)
+    def setLowWeightFactor(self, w: float) -> None: self._low = float(w)
+    def lowWeight(self) -> float: return self._low
+
+    def _struct(self) -> "N.OrganAEC":
+        return N.OrganAEC(int(self._use), int(self._compensate), self._start, self._stop, self._ramp, self._low)
+
+    def maxWeight(self) -> float:
+        v = C.c_double()
+        f = self._struct()
+        N.check(N.lib().dxg_organ_aec_max_weight(C.byref(f), C.byref(v)))
+        return v.value
+
+    def weight(self, angle: float) -> float:
+        """The weight at gantry angle `angle` (rad)."""
+        v = C.c_double()
+        f = self._struct()
+        N.check(N.lib().dxg_organ_aec_weight(C.byref(f), float(angle), C.byref(v)))
+        return v.value
+
 
 class _BeamBase:
     numberOfParticlesPerExposure = 1_000_000
@@ -398,6 +467,10 @@ class CTSpiralBeam(_BeamBase):
         self.fov = 50.0
         self.bowtie: BowtieFilter | None = None
         self.aec: CTAECFilter | None = None
+        self._organ_aec = CTOrganAECFilter()
+
+    def organAECFilter(self) -> CTOrganAECFilter:
+        return self._organ_aec
 
     def setSourceDetectorDistance(self, d): self.sdd = float(d)
     def setCollimation(self, c): self.collimation = float(c)
@@ -420,7 +493,7 @@ class CTSpiralBeam(_BeamBase):
             tuple(self.start), tuple(self.stop), self.start_angle, self.step_angle, self.pitch, self.collimation,
             self.sdd, self.fov, self.numberOfParticlesPerExposure, int(self.dual), int(bool(self.bowtie_list())),
             getattr(self, "fov_b", self.fov), getattr(self, "tube_b_offset", 0.0),
-            getattr(self, "mas_a", 1.0), getattr(self, "mas_b", 1.0), aec)
+            getattr(self, "mas_a", 1.0), getattr(self, "mas_b", 1.0), aec, self._organ_aec._struct())
 
     def bowtie_list(self):
         return [self.bowtie] if self.bowtie is not None else []

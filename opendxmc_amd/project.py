@@ -183,7 +183,6 @@ _VOLUMES = (("densityarray", "density", np.float64), ("ctarray", "ct", np.float6
             ("organarray", "organ", np.uint8))
 _DOSE_VOLUMES = (("dosearray", "dose"), ("dosevariancearray", "dose_variance"), ("doseeventcountarray", "dose_count"))
 
-_ORGAN_AEC_DEFAULT = {"use": False, "compensate": False, "low_weight": 1.0, "angles": (0.0, 0.0, 0.0)}
 
 
 def _deg(rad) -> np.ndarray:
@@ -312,11 +311,12 @@ class HDF5Wrapper:
 
     @staticmethod
     def _organ_aec(beam) -> list[tuple[str, object, type]]:
-        o = dict(_ORGAN_AEC_DEFAULT, **getattr(beam, "organ_aec", {}))
-        return [("use_organ_aec", np.uint64(bool(o["use"])), np.uint64),
-                ("compensate_weight_organ_aec", np.uint64(bool(o["compensate"])), np.uint64),
-                ("low_weight_organ_aec", float(o["low_weight"]), np.float64),
-                ("start_stop_ramp_angle_organ_aec", np.asarray(o["angles"], dtype=np.float64), np.float64)]
+        o = beam.organAECFilter()
+        angles = (o.startAngle(), o.stopAngle(), o.rampAngle())
+        return [("use_organ_aec", np.uint64(o.useFilter()), np.uint64),
+                ("compensate_weight_organ_aec", np.uint64(o.compensateOutside()), np.uint64),
+                ("low_weight_organ_aec", float(o.lowWeight()), np.float64),
+                ("start_stop_ramp_angle_organ_aec", np.asarray(angles, dtype=np.float64), np.float64)]
 
     def saveBeam(self, beam) -> bool:
         """HDF5Wrapper::save(BeamActorContainer) (hdf5wrapper.cpp:461-628, 1052-1068)."""
@@ -421,20 +421,21 @@ class HDF5Wrapper:
                 tube.filtration[z] = float(f)   # addTubeFiltrationMaterial
 
     def _load_organ_aec(self, path: str, beam) -> None:
-        o = dict(_ORGAN_AEC_DEFAULT, **getattr(beam, "organ_aec", {}))
+        o = beam.organAECFilter()
         u = self._get(path, "use_organ_aec", dtype=np.uint64)
         if u is not None:
-            o["use"] = int(u) == 1
+            o.setUseFilter(int(u) == 1)
         a = self._get(path, "start_stop_ramp_angle_organ_aec", 3)
         if a is not None:
-            o["angles"] = tuple(float(x) for x in a)
+            o.setStartAngle(float(a[0]))
+            o.setStopAngle(float(a[1]))
+            o.setRampAngle(float(a[2]))
         c = self._get(path, "compensate_weight_organ_aec", dtype=np.uint64)
         if c is not None:
-            o["compensate"] = int(c) == 1
+            o.setCompensateOutside(int(c) == 1)
         w = self._get(path, "low_weight_organ_aec")
         if w is not None:
-            o["low_weight"] = float(w)
-        beam.organ_aec = o
+            o.setLowWeightFactor(float(w))
 
     def _load_beam(self, kind: str, path: str):
         g = lambda name, n=1, dtype=np.float64: self._get(path, name, n, dtype)   # noqa: E731

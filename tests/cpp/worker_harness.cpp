@@ -148,8 +148,8 @@ void worker(bool deleteAirDose, int nthreads, Data& data, const std::vector<std:
 
 int main(int argc, char** argv)
 {
-    if (argc != 7) {
-        std::cerr << "usage: worker_harness phantom.bin bowtie.txt|- out.bin correction seed ppe\n";
+    if (argc != 7 && argc != 8) {
+        std::cerr << "usage: worker_harness phantom.bin bowtie.txt|- out.bin correction seed ppe [ctdivol_mgy]\n";
         return 2;
     }
     Data data;
@@ -169,6 +169,8 @@ int main(int argc, char** argv)
     beam->setPitch(1.0);
     beam->setStepAngleDeg(15);
     beam->setNumberOfParticlesPerExposure(ppe);
+    if (argc == 8)
+        beam->setCTDIvol(std::strtod(argv[7], nullptr));   // default: 0, uncalibrated
     if (std::string(argv[2]) != "-") {
         std::ifstream bf(argv[2]);
         std::vector<std::pair<double, double>> nodes;

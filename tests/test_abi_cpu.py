@@ -250,3 +250,84 @@ def test_phantoms():
     np.testing.assert_array_equal(a.material, b.material)
     for comp in (phantoms.SOFT_TISSUE, phantoms.ADIPOSE, phantoms.LUNG, phantoms.BONE):
         assert sum(comp.values()) == pytest.approx(1.0, abs=0.002)
+
+
+def organ_aec(start_deg, stop_deg, ramp_deg, low, compensate):
+    f = dx.CTOrganAECFilter()
+    f.setUseFilter(True)
+    f.setStartAngleDeg(start_deg)
+    f.setStopAngleDeg(stop_deg)
+    f.setRampAngleDeg(ramp_deg)
+    f.setLowWeightFactor(low)
+    f.setCompensateOutside(compensate)
+    return f
+
+
+def test_organ_aec_weight_profile():
+    """Low over the arc, linear ramps beside it, high elsewhere; with
+    compensation the mean over a turn is 1 (numerical integral over a fine
+    angle grid), without it the high weight is 1."""
+    f = organ_aec(60.0, 180.0, 20.0, 0.4, True)
+    hi = f.maxWeight()
+    # arc 120 deg, ramp 20 deg: high = (360 - 0.4 * 140) / (360 - 140)
+    assert hi == pytest.approx((360 - 0.4 * 140) / 220, rel=1e-12)
+    for deg, want in ((60, 0.4), (120, 0.4), (180, 0.4), (190, 0.4 + 0.5 * (hi - 0.4)), (200, hi), (300, hi),
+                      (40, hi), (50, 0.4 + 0.5 * (hi - 0.4)), (60 + 360, 0.4), (-200, 0.4)):
+        assert f.weight(math.radians(deg)) == pytest.approx(want, rel=1e-12), deg
+    a = np.linspace(0, 2 * np.pi, 360 * 64, endpoint=False)
+    assert np.mean([f.weight(x) for x in a]) == pytest.approx(1.0, abs=1e-4)
+    g = organ_aec(60.0, 180.0, 20.0, 0.4, False)
+    assert g.maxWeight() == 1.0 and g.weight(math.radians(300)) == 1.0
+    # an arc through 0: from 300 to 40 degrees
+    h = organ_aec(300.0, 40.0, 0.0, 0.5, False)
+    assert h.weight(math.radians(350)) == 0.5 and h.weight(math.radians(10)) == 0.5 and h.weight(math.radians(100)) == 1.0
+    off = dx.CTOrganAECFilter()
+    assert off.weight(1.0) == 1.0 and off.maxWeight() == 1.0
+
+
+def test_organ_aec_applied_to_spiral_exposures():
+    b = dx.CTSpiralBeam((0, 0, -2), (0, 0, 2))
+    b.setCollimation(4.0)
+    b.setStepAngleDeg(5.0)
+    plain = b.exposures()
+    oa = b.organAECFilter()
+    for k, v in dict(setUseFilter=True, setStartAngleDeg=90.0, setStopAngleDeg=180.0, setRampAngleDeg=10.0,
+                     setLowWeightFactor=0.3, setCompensateOutside=True).items():
+        getattr(oa, k)(v)
+    ex = b.exposures()
+    ang = b.start_angle + np.arange(len(ex)) * b.step_angle
+    want = plain["weight"] * np.array([oa.weight(x) for x in ang])
+    np.testing.assert_allclose(ex["weight"], want, rtol=1e-12)
+    assert np.min(ex["weight"]) == pytest.approx(0.3) and np.max(ex["weight"]) == pytest.approx(oa.maxWeight())
+    # dual source: each tube by its own angle
+    d = dx.CTSpiralDualEnergyBeam((0, 0, -2), (0, 0, 2))
+    d.setTubeBoffsetAngleDeg(95.0)
+    for k, v in dict(setUseFilter=True, setStartAngleDeg=90.0, setStopAngleDeg=180.0, setLowWeightFactor=0.3).items():
+        getattr(d.organAECFilter(), k)(v)
+    ex = d.exposures()
+    ang = d.start_angle + np.arange(len(ex) // 2) * d.step_angle
+    wa = np.array([d.organAECFilter().weight(x) for x in ang])
+    wb = np.array([d.organAECFilter().weight(x + math.radians(95.0)) for x in ang])
+    np.testing.assert_allclose(ex["weight"][0::2], wa, rtol=1e-12)
+    np.testing.assert_allclose(ex["weight"][1::2], wb, rtol=1e-12)
+
+
+def test_aec_from_water_equivalent_diameter():
+    """datacontainer.cpp:42-100: A_w = (mean HU / 1000 + 1) x slice area for CT,
+    sum(density) x pixel area for density; D_w = 2 sqrt(A_w / pi); weight
+    exp(0.2 D_w); the filter spans the image's z extent about the origin."""
+    dims, sp = (20, 16, 3), (0.5, 0.5, 2.0)
+    nx, ny, nz = dims
+    ct = np.full((nz, ny, nx), -1000.0)   # air
+    ct[1] = 0.0                             # water slice: A_w = the slice area 10 x 8 cm2
+    ct[2, :8, :] = 1000.0                   # half the slice at +1000 HU: mean 0 HU over the slice... plus air
+    ct[2, 8:, :] = -1000.0
+    f = dx.CTAECFilter.fromWaterEquivalentDiameter(dims, sp, ct, is_ct=True)
+    assert f.start == (0.0, 0.0, -3.0) and f.stop == (0.0, 0.0, 3.0)
+    area = nx * ny * sp[0] * sp[1]
+    np.testing.assert_allclose(f.weights, [1.0, math.exp(0.2 * 2 * math.sqrt(area / math.pi)),
+                                           math.exp(0.2 * 2 * math.sqrt(area / math.pi))], rtol=1e-12)
+    rho = np.zeros((nz, ny, nx))
+    rho[0, :, :4] = 1.0   # a 2 cm x 8 cm block of water
+    g = dx.CTAECFilter.fromWaterEquivalentDiameter(dims, sp, rho, is_ct=False)
+    np.testing.assert_allclose(g.weights, [math.exp(0.2 * 2 * math.sqrt(16.0 / math.pi)), 1.0, 1.0], rtol=1e-12)

@@ -55,14 +55,19 @@ def test_harness_builds_and_reports_failure_without_device(tmp_path):
 
 
 @pytest.mark.gpu
-def test_worker_harness_matches_python_path(tmp_path):
+@pytest.mark.parametrize("ctdivol", [None, 1.0])
+def test_worker_harness_matches_python_path(tmp_path, ctdivol):
+    """worker<C> on the C++ facade and the Python beams through one C-ABI give
+    the same doses, both with default beams (no calibration value: uncalibrated
+    in both facades) and with a CTDIvol of 1 mGy set on both."""
     ph = phantoms.ct_body((40, 40, 24), (0.9, 0.9, 0.9), seed=8)
     write_phantom(tmp_path / "p.bin", ph)
     bow = write_bowtie(tmp_path / "b.txt")
     seed, ppe = 77, 20000
-    print(f"[harness] dims={ph.dims} seed={seed} ppe={ppe}", flush=True)
+    print(f"[harness] dims={ph.dims} seed={seed} ppe={ppe} ctdivol={ctdivol}", flush=True)
+    extra = [] if ctdivol is None else [str(ctdivol)]
     r = subprocess.run([str(harness()), str(tmp_path / "p.bin"), str(tmp_path / "b.txt"), str(tmp_path / "o.bin"),
-                        "1", str(seed), str(ppe)], capture_output=True, text=True, timeout=300)
+                        "1", str(seed), str(ppe), *extra], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     n = ph.size
     raw = (tmp_path / "o.bin").read_bytes()
@@ -85,11 +90,11 @@ def test_worker_harness_matches_python_path(tmp_path):
     beam.setStepAngleDeg(15)
     beam.setBowtieFilter(bow)
     beam.setNumberOfParticlesPerExposure(ppe)
-    # the C++ beam keeps a stored CTDIvol of 1 mGy (32 cm phantom) and worker<C>
-    # runs with useBeamCalibration: calibrate the Python run the same way
-    beam.setCTDIvol(1.0)
-    beam.setCTDIdiameter(32.0)
-    assert dx.Transport(seed=seed)(world, beam)
+    # worker<C> runs with useBeamCalibration (simulationpipeline.cpp:165)
+    if ctdivol is not None:
+        beam.setCTDIvol(ctdivol)
+        beam.setCTDIdiameter(32.0)
+    assert dx.Transport(seed=seed)(world, beam, None, True)
     d, v, c = grid.doseScored()
     air = ph.material == 0
     d[air] = 0.0

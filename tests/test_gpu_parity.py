@@ -185,6 +185,36 @@ def test_beam_families(oracle, family):
     assert v.ok, str(v)
 
 
+def test_spiral_with_wed_and_organ_aec(oracle):
+    """A CT spiral whose tube current follows the water-equivalent diameter
+    per slice (datacontainer.cpp:42-100) and an organ AEC arc with ramps and
+    compensation (beamsettingsmodel.cpp:349-437): the exposure weights differ
+    by a factor of several across the scan, and the transport still matches
+    the oracle."""
+    ph = small_body()
+    hz = 0.5 * ph.dims[2] * ph.spacing[2]
+    beam = dx.CTSpiralBeam((0, 0, -hz), (0, 0, hz), {13: 6.0})
+    beam.setCollimation(4.0)
+    beam.setPitch(1.0)
+    beam.setStepAngleDeg(15.0)
+    beam.setBowtieFilter(read_filters()[DEFAULT_CT_BOWTIE])
+    beam.setAECFilter(dx.CTAECFilter.fromWaterEquivalentDiameter(ph.dims, ph.spacing, ph.density, is_ct=False))
+    o = beam.organAECFilter()
+    o.setUseFilter(True)
+    o.setStartAngleDeg(45.0)
+    o.setStopAngleDeg(135.0)
+    o.setRampAngleDeg(20.0)
+    o.setLowWeightFactor(0.3)
+    o.setCompensateOutside(True)
+    beam.setNumberOfParticlesPerExposure(3_000_000 // beam.numberOfExposures())
+    w = beam.exposures()["weight"]
+    assert w.max() / w.min() > 3.0
+    g, og, hg, ho, gc, oc = run_both(ph, beam, 1, oracle=oracle)
+    v = compare(g, og, hg, ho)
+    print("wed+organ aec", v, flush=True)
+    assert v.ok, str(v)
+
+
 def test_cancel_mid_run():
     ph = phantoms.water_cube(32, 0.2)
     world, grid = make_world(ph, 1)

@@ -171,7 +171,12 @@ def test_beams_round_trip_with_the_reference_quirks(tmp_path):
     spiral.setCTDIvol(8.0)
     spiral.setCTDIdiameter(16.0)
     spiral.setStepAngleDeg(3.0)
-    spiral.organ_aec = {"use": True, "compensate": False, "low_weight": 0.4, "angles": (0.5, 1.5, 0.2)}
+    oaec = spiral.organAECFilter()
+    oaec.setUseFilter(True)
+    oaec.setLowWeightFactor(0.4)
+    oaec.setStartAngle(0.5)
+    oaec.setStopAngle(1.5)
+    oaec.setRampAngle(0.2)
     seq = dx.CTSequentialBeam((1.0, 2.0, 3.0))
     seq.setNumberOfSlices(4)
     seq.setCTDIw(11.0)
@@ -202,7 +207,9 @@ def test_beams_round_trip_with_the_reference_quirks(tmp_path):
         assert s.start == (0.0, 0.0, -5.0) and s.pitch == 1.2 and s.CTDIvol() == 8.0
         assert s.CTDIdiameter() == 32.0               # saved "CTDIdiameter", loaded "CTDIDiameter"
         assert np.isclose(np.degrees(s.step_angle), 3.0) and s.tube.filtration[50] == 0.4
-        assert s.organ_aec == spiral.organ_aec
+        o = s.organAECFilter()
+        assert (o.useFilter(), o.compensateOutside(), o.lowWeight()) == (True, False, 0.4)
+        assert (o.startAngle(), o.stopAngle(), o.rampAngle()) == (0.5, 1.5, 0.2)
     assert q.position == (0.0, 0.0, 0.0)              # saved "start_position", loaded "position"
     assert (q.slices, q.CTDIw(), q.CTDIdiameter()) == (4, 11.0, 32.0)
     assert np.isclose(c.start_angle, np.radians(-45.0)) and c.DAPvalue() == 3.0
