@@ -41,7 +41,6 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 PROFILE = ROOT / "profiles" / "transport_kernel_ct_spiral_1e8.md"
-TIME_TO_2PCT = ROOT / "profiles" / "denoiser_time_to_2pct_ct_spiral_512_dual.md"
 
 
 def measured_traffic_per_history():
@@ -56,30 +55,83 @@ def measured_traffic_per_history():
     return None
 
 
-def committed_time_to_2pct():
-    """Time to 2 % mean relative dose error on the full dual-source CT spiral
-    (512 x 512 x 400), from the committed denoiser study; None when absent."""
-    vals = {}
+def host_cpu() -> dict:
+    """The host CPU: model, logical CPUs, the CPUs this process may run on,
+    and the cgroup CPU quota when one is set; `threads` is what a CPU leg
+    should use (the allowed CPUs, capped by the quota)."""
+    model = "unknown"
     try:
-        for line in TIME_TO_2PCT.read_text().splitlines():
-            if line.startswith("time_to_2pct_"):
-                k, v = line.split(":", 1)
-                vals[k.strip()] = float(v)
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
     except OSError:
-        return None
-    if not {"time_to_2pct_plain_s", "time_to_2pct_denoised_s"} <= vals.keys():
-        return None
+        pass
+    logical = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = logical
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    threads = allowed if quota is None else max(1, min(allowed, int(quota)))
+    return {"model": model, "logical_cpus": logical, "allowed_cpus": allowed, "cgroup_cpu_quota": quota, "threads": threads}
+
+
+def time_to_2pct(ref_histories: float, cpu_seconds: float | None):
+    """Time to 2 % mean relative dose error on the full dual-source CT spiral
+    (512 x 512 x 400 at 0.08 cm), measured in this run: a reference run of
+    ref_histories, plain runs at 1e9, 4e9 and 1.6e10 histories (error a /
+    sqrt(H) fitted, time at the measured rate) and runs of 5e7 .. 1.6e9
+    histories denoised by the shipped network (2 % crossing interpolated on
+    log-log axes; MC time plus the denoiser's features + inference + dose).
+    With cpu_seconds, the oracle's rate on a sample of the same workload gives
+    the CPU time for the plain histories to 2 %."""
+    from opendxmc_amd import denoise as dn
+    from opendxmc_amd import denoise_study as study
+    from opendxmc_amd import phantoms
+
+    def log(msg):
+        print(f"[time-to-2%] {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    ph = phantoms.ct_body((512, 512, 400), (0.08, 0.08, 0.08), seed=1)
+    net = dn.Denoiser.pretrained()
+    ev = study.evaluate(net, ref_histories, (1e9, 4e9, 1.6e10), (5e7, 1e8, 2e8, 4e8, 8e8, 1.6e9), seed=1, ph=ph, log=log)
+    del net
     out = {
-        "workload": "CT spiral dual source 512x512x400 at 0.08 cm, error over body voxels above 20 % of max, "
-                    "vs a 3e11-history reference",
-        "plain_gpu_mc_s": vals["time_to_2pct_plain_s"],
-        "mc_plus_denoiser_s": vals["time_to_2pct_denoised_s"],
-        "speedup_mc_plus_denoiser": vals["time_to_2pct_plain_s"] / vals["time_to_2pct_denoised_s"],
-        "source": str(TIME_TO_2PCT.relative_to(ROOT)),
+        "workload": "CT spiral dual source 512x512x400 at 0.08 cm, seeded body, 2 x 120 kVp 6 mm Al, W1 bowtie, "
+                    "2 x 4 cm collimation, pitch 3.2, 32 cm; error = mean |D - D_ref| / D_ref over body voxels above "
+                    "20 % of the reference maximum",
+        "measured_in_this_run": True,
+        "reference_histories": ev["reference_histories"],
+        "reference_rms_relative_se": ev["reference_rms_relative_se"],
+        "gpu_histories_per_s": ev["histories_per_s"],
+        "plain_points": [(p["histories"], p["error"]) for p in ev["plain"]],
+        "denoised_points": [(p["histories"], p["error"], p["denoise_ms"]) for p in ev["denoised"]],
+        "plain_histories_to_2pct": ev["plain_histories_to_target"],
+        "plain_gpu_mc_s": ev["plain_seconds_to_target"],
+        "mc_plus_denoiser_histories_to_2pct": ev["denoised_histories_to_target"],
+        "mc_plus_denoiser_s": ev["denoised_seconds_to_target"],
+        "speedup_mc_plus_denoiser": ev["speedup"],
     }
-    if "time_to_2pct_cpu_s" in vals:
-        out["cpu_oracle_s"] = vals["time_to_2pct_cpu_s"]
-        out["speedup_plain_gpu_vs_cpu"] = vals["time_to_2pct_cpu_s"] / vals["time_to_2pct_plain_s"]
+    if cpu_seconds:
+        _, beam = ct_dual_workload(1e9)
+        cb = cpu_baseline(ph, beam, seconds_target=cpu_seconds, what="full dual-source CT spiral")
+        t_cpu = ev["plain_histories_to_target"] / cb["value"]
+        out["cpu_oracle_histories_per_s"] = cb["value"]
+        out["cpu_oracle_s"] = t_cpu
+        out["cpu_oracle_sample"] = cb["sample"]
+        out["cpu_note"] = "CPU time = plain histories to 2 % / the oracle's measured rate (extrapolated)"
+        out["speedup_plain_gpu_vs_cpu"] = t_cpu / ev["plain_seconds_to_target"]
+        if ev["denoised_seconds_to_target"]:
+            out["speedup_mc_plus_denoiser_vs_cpu"] = t_cpu / ev["denoised_seconds_to_target"]
+    out["wall_s"] = time.perf_counter() - t0
     return out
 
 
@@ -252,12 +304,14 @@ def pipeline_bench(args) -> int:
     return 0
 
 
-def cpu_baseline(ph, beam, seconds_target: float = 15.0):
-    """Time the oracle on a bounded sample: every exposure, fewer particles."""
+def cpu_baseline(ph, beam, seconds_target: float = 15.0, what: str = "CT spiral"):
+    """Time the oracle on a bounded sample: every exposure, fewer particles,
+    on every CPU this process may use (within the cgroup quota)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
-    threads = min(os.cpu_count() or 1, 16)
+    host = host_cpu()
+    threads = host["threads"]
     ex = beam.exposures()
     spectra = beam.spectra()
     bows = [(b.angle, b.weight) for b in beam.bowties()]
@@ -287,8 +341,10 @@ def cpu_baseline(ph, beam, seconds_target: float = 15.0):
         "unit": "histories/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{c['histories']} histories ({per_ex} per exposure x {len(ex)} exposures) of the same CT "
-                  f"spiral workload, FP64 oracle, {threads} threads, {dt:.1f} s",
+        "host": host,
+        "sample": f"{c['histories']} histories ({per_ex} per exposure x {len(ex)} exposures) of the same {what} "
+                  f"workload, FP64 oracle, {threads} threads on {host['model']} ({host['logical_cpus']} logical CPUs, "
+                  f"{host['allowed_cpus']} allowed, quota {host['cgroup_cpu_quota']}), {dt:.1f} s",
         "steps_per_history": c["steps"] / c["histories"],
         "deposits_per_history": c["deposits"] / c["histories"],
     }
@@ -302,6 +358,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(PIPE_KERNELS), default="ct-spiral-256")
     ap.add_argument("--histories", type=float, default=None, help="histories per step (default per workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-time-to-2pct", action="store_true", help="skip the time-to-2 %% measurement on the full CT spiral")
+    ap.add_argument("--t2-ref-histories", type=float, default=3e11, help="reference histories of the time-to-2 %% leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (histories shard trivially "
@@ -390,11 +448,11 @@ def main() -> int:
     }
     if sum(phase) > 0:   # libraries built with DXG_PHASE_CLOCKS
         result["config"]["phase_cycle_share"] = {k: v / sum(phase) for k, v in zip(("refill", "interact", "step"), phase)}
-    t2 = committed_time_to_2pct()
-    if t2 is not None:
-        result["time_to_2pct"] = t2
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ph, beam)
+    if not args.no_time_to_2pct and args.workload == "ct-spiral-256":
+        del world, grid
+        result["time_to_2pct"] = time_to_2pct(args.t2_ref_histories, None if args.no_cpu_baseline else 10.0)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -160,8 +160,12 @@ def interpolate_crossing(hs, errs, target: float):
     return None
 
 
-def evaluate(net: dn.Denoiser, ref_histories: float, plain_hist, dn_hist, seed: int, target_err: float = 0.02):
-    ph = phantoms.ct_body((512, 512, 400), (0.08, 0.08, 0.08), seed=seed)
+def evaluate(net: dn.Denoiser, ref_histories: float, plain_hist, dn_hist, seed: int, target_err: float = 0.02,
+             ph: phantoms.Phantom | None = None, log=log):
+    """Time to target_err on the full CT spiral; `ph` is the phantom
+    (default: the seeded 512 x 512 x 400 body), `log` the progress printer."""
+    if ph is None:
+        ph = phantoms.ct_body((512, 512, 400), (0.08, 0.08, 0.08), seed=seed)
     world, grid = make_grid(ph)
     z_half = 16.0
 

@@ -1,5 +1,6 @@
-"""bench.py's host logic on the CPU: the one-GPU contract and the committed
-profile figures it copies into its JSON line."""
+"""bench.py's host logic on the CPU: the one-GPU contract, the committed
+counter-profile figures it reads, and the host CPU description of its CPU
+baseline."""
 from __future__ import annotations
 
 import subprocess
@@ -27,12 +28,13 @@ def test_transport_traffic_per_history_is_read_from_the_profile():
     assert t is not None and 50.0 < t < 5000.0
 
 
-def test_time_to_target_figures_are_read_from_the_profile():
-    t = bench.committed_time_to_2pct()
-    assert t is not None
-    assert t["mc_plus_denoiser_s"] < t["plain_gpu_mc_s"] < t["cpu_oracle_s"]
-    assert t["speedup_mc_plus_denoiser"] == pytest.approx(t["plain_gpu_mc_s"] / t["mc_plus_denoiser_s"])
-    assert t["speedup_mc_plus_denoiser"] >= 5.0
+def test_host_cpu_description():
+    h = bench.host_cpu()
+    assert h["logical_cpus"] >= 1 and 1 <= h["allowed_cpus"] <= h["logical_cpus"]
+    assert 1 <= h["threads"] <= h["allowed_cpus"]
+    if h["cgroup_cpu_quota"] is not None:
+        assert h["threads"] <= max(1, int(h["cgroup_cpu_quota"]))
+    assert isinstance(h["model"], str) and h["model"]
 
 
 @pytest.mark.parametrize("kernel", sorted(bench.PIPE_KERNELS.values()))
