@@ -18,6 +18,9 @@ from dataclasses import dataclass
 
 import numpy as np
 
+import opendxmc_amd as dx
+from opendxmc_amd import phantoms
+
 
 @dataclass
 class Verdict:
@@ -64,3 +67,33 @@ def compare(gpu, orc, h_gpu: int, h_orc: int, threshold: float = 0.10, tol_total
     rel = (tg - to) / to
     return Verdict(n, n_fail, float(within.mean()) if n else 1.0, float(tg), float(to), float(rel), float(tz),
                    n_fail <= allowed, abs(rel) <= tol_total)
+
+
+def make_world(ph: phantoms.Phantom, level: int = 1):
+    world = dx.World()
+    grid = world.addItem(dx.AAVoxelGrid(correction=level))
+    grid.setData(ph.dims, ph.density, ph.material, [dx.Material.byWeight(m) for m in ph.materials])
+    grid.setSpacing(ph.spacing)
+    world.build()
+    return world, grid
+
+
+def beam_inputs(beam):
+    ex = beam.exposures()
+    spectra = beam.spectra()
+    bows = [(b.angle, b.weight) for b in beam.bowties()]
+    return ex, spectra, bows
+
+
+def run_both(ph, beam, level, seed=7, oracle=None, threads=16):
+    print(f"[parity] {type(beam).__name__} dims={ph.dims} level={level} histories={beam.numberOfParticles()}", flush=True)
+    world, grid = make_world(ph, level)
+    ok = dx.Transport(seed=seed)(world, beam, None, True)
+    assert ok
+    e, e2, _ = grid.energyScored()
+    hg = grid.histories()
+    counters = grid.counters()
+    ex, spectra, bows = beam_inputs(beam)
+    se, se2, _, oc = oracle.transport(ph.dims, ph.spacing, ph.density, ph.material, ph.materials, ex, spectra, bows,
+                                      level=level, seed=seed + 1000, threads=threads)
+    return (e, e2), (se, se2), hg, oc["histories"], counters, oc

@@ -74,30 +74,33 @@ def test_thin_slab_kerma_per_fluence(oracle, energy):
 
 
 def test_variance_matches_spread_of_seeds():
-    """Per voxel above 10 % of the maximum, the sample variance of 40 seeds'
-    doses against the mean reported variance: (K-1) s^2 / sigma^2 is chi-square
-    with K-1 degrees of freedom, so at most a few voxels may fall outside its
-    central 99.8 %, and the mean ratio s^2 / sigma^2 stays within 10 % of 1."""
+    """Per voxel with at least 1000 events per run (the beam axis and the
+    scatter around it), the sample variance of 40 seeds' doses against the
+    mean reported variance: (K-1) s^2 / sigma^2 is close to chi-square with
+    K-1 degrees of freedom (the deposits' heavy tail widens it a little), so
+    few voxels may fall outside its central 99.8 %, and the mean ratio
+    s^2 / sigma^2 stays within 10 % of 1."""
     ph = phantoms.water_cube(16, 0.4)
     beam = dx.PencilBeam((0.2, 0.2, -10.0), (0, 0, 1), 60.0)
-    beam.setNumberOfParticlesPerExposure(200_000)
+    beam.setNumberOfParticlesPerExposure(1_000_000)
     world, grid = world_of(ph, 1)
     K = 40
-    doses, variances = [], []
+    doses, variances, events = [], [], []
     for seed in range(K):
         grid.clearScores()
         dx.Transport(seed=100 + seed)(world, beam, None, False)
-        d, var, _ = grid.doseScored()
+        d, var, ev = grid.doseScored()
         doses.append(d)
         variances.append(var)
+        events.append(ev)
     doses = np.array(doses)
-    sel = doses.mean(0) > 0.1 * doses.mean(0).max()
+    sel = np.array(events, dtype=np.float64).mean(0) >= 1000
     ratio = doses.var(0, ddof=1)[sel] / np.array(variances).mean(0)[sel]
     q = (K - 1) * ratio
     lo, hi = stats.chi2.ppf([0.001, 0.999], K - 1)
     n_out = int(np.sum((q < lo) | (q > hi)))
-    print(f"{sel.sum()} voxels: mean s^2/sigma^2 {ratio.mean():.3f}, outside the 99.8 % chi-square band {n_out}",
-          flush=True)
-    assert sel.sum() >= 20
+    print(f"{sel.sum()} voxels: mean s^2/sigma^2 {ratio.mean():.3f} (median {np.median(ratio):.3f}), "
+          f"outside the 99.8 % chi-square band {n_out}", flush=True)
+    assert sel.sum() >= 100
     assert abs(ratio.mean() - 1.0) < 0.10
-    assert n_out <= max(1, int(0.03 * sel.sum()))
+    assert n_out <= max(2, int(0.03 * sel.sum()))

@@ -21,40 +21,10 @@ import pytest
 import opendxmc_amd as dx
 from opendxmc_amd import phantoms
 from opendxmc_amd.bowtie import DEFAULT_CT_BOWTIE, read_filters
-from parity import compare
+from parity import compare, make_world, run_both
 from rays import face_rays, random_rays
 
 pytestmark = pytest.mark.gpu
-
-
-def make_world(ph: phantoms.Phantom, level: int = 1):
-    world = dx.World()
-    grid = world.addItem(dx.AAVoxelGrid(correction=level))
-    grid.setData(ph.dims, ph.density, ph.material, [dx.Material.byWeight(m) for m in ph.materials])
-    grid.setSpacing(ph.spacing)
-    world.build()
-    return world, grid
-
-
-def beam_inputs(beam):
-    ex = beam.exposures()
-    spectra = beam.spectra()
-    bows = [(b.angle, b.weight) for b in beam.bowties()]
-    return ex, spectra, bows
-
-
-def run_both(ph, beam, level, seed=7, oracle=None, threads=16):
-    print(f"[parity] {type(beam).__name__} dims={ph.dims} level={level} histories={beam.numberOfParticles()}", flush=True)
-    world, grid = make_world(ph, level)
-    ok = dx.Transport(seed=seed)(world, beam, None, True)
-    assert ok
-    e, e2, _ = grid.energyScored()
-    hg = grid.histories()
-    counters = grid.counters()
-    ex, spectra, bows = beam_inputs(beam)
-    se, se2, _, oc = oracle.transport(ph.dims, ph.spacing, ph.density, ph.material, ph.materials, ex, spectra, bows,
-                                      level=level, seed=seed + 1000, threads=threads)
-    return (e, e2), (se, se2), hg, oc["histories"], counters, oc
 
 
 @pytest.mark.parametrize("dims,spacing", [((13, 9, 7), (0.25, 0.5, 0.3)), ((64, 64, 64), (0.1, 0.1, 0.1)),
