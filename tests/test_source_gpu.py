@@ -38,9 +38,10 @@ def check_beam(world, beam, n, nodes_by_bowtie):
     starts = np.concatenate([[0], np.cumsum(ex["n_particles"].astype(np.int64))])
     h = np.arange(n)
     np.testing.assert_array_equal(ei, np.searchsorted(starts, h, side="right") - 1)
-    half = ex["collimation_half_angles"][0]
-    assert np.all(ex["collimation_half_angles"] == half)
-    check_angles_uniform(ax.astype(np.float64), ay.astype(np.float64), half)
+    halves = ex["collimation_half_angles"][ei]
+    for half in np.unique(halves, axis=0):   # e.g. the two tubes' fields of a dual-source beam
+        m = np.all(halves == half, axis=1)
+        check_angles_uniform(ax[m].astype(np.float64), ay[m].astype(np.float64), half)
     cos = ex["cosines"][ei]
     ref = np.empty_like(d, dtype=np.float64)
     for k in np.unique(ei):

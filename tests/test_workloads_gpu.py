@@ -60,19 +60,22 @@ def test_batch_full_64_instances(oracle):
         e2, _, n2 = worlds[k][1].energyScored()
         np.testing.assert_array_equal(n1, n2)
         np.testing.assert_allclose(e2, e1, rtol=1e-10, atol=1e-12 * e1.max())
-    # a DX and a CT sequential instance against the oracle (the oracle at most 3e6 histories)
+    # a DX and a CT sequential instance against the oracle (the oracle at most
+    # 1e7 histories): every voxel above 10 % of the maximum within 3 SE, the
+    # total within 4 combined SE (the DX instance's thin deposits leave the
+    # total's SE near 0.3 %, so a fixed 0.5 % would test the sample size)
     for k in (0, 1):
         ph, beam, s = inst[k]
         e, e2, _ = worlds[k][1].energyScored()
         ex = beam.exposures().copy()
-        scale = min(1.0, 3e6 / ex["n_particles"].sum())
+        scale = min(1.0, 1e7 / ex["n_particles"].sum())
         ex["n_particles"] = np.maximum(1, (ex["n_particles"] * scale).astype(np.uint64))
         se, se2, _, oc = oracle.transport(ph.dims, ph.spacing, ph.density, ph.material, ph.materials, ex, beam.spectra(),
                                           [(b.angle, b.weight) for b in beam.bowties()], level=1, seed=4242 + k,
                                           threads=16)
         v = compare((e, e2), (se, se2), worlds[k][1].histories(), oc["histories"])
         print(f"[c3] instance {k} ({type(beam).__name__}, {ph.dims}) vs oracle: {v}", flush=True)
-        assert v.ok, str(v)
+        assert v.ok_voxels and abs(v.total_z) < 4.0, str(v)
 
 
 def full_ct_beam(z_half: float, histories: float):
