@@ -154,6 +154,21 @@ uint64_t dxg_grid_size(const dxg_world* w);
 int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress,
                       int apply_calibration, uint64_t seed);
 
+/* ---- dxg_transport_run without allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Same arguments, scores, counters and
+ * results as dxg_transport_run; it allocates and frees no device memory and
+ * creates no HIP events (counters.kernel_ms is then the host's clock around
+ * each launch and its wait).  The history pool, the event counters and the
+ * beam's tables live in a fixed 1 MiB scratch array per device, which the
+ * library holds until the process ends: the beam's exposure records (80 B
+ * each), their first histories (8 B each) and its spectrum/bowtie tables must
+ * fit (about 12,000 exposures with 1 keV spectra), else DXG_ERR_ARGUMENT.
+ * Every call on one GPU shares that GPU's scratch, so calls that overlap on
+ * one GPU (from several threads) race: serialise them. */
+int dxg_transport_run_static(dxg_world* w, const dxg_beam* beam, dxg_progress* progress,
+                             int apply_calibration, uint64_t seed);
+
 /* Zero the energy scores and the history count. */
 int dxg_world_clear_scores(dxg_world* w);
 

@@ -174,6 +174,7 @@ SIGNATURES = [
     ("dxg_world_build", C.c_int, [C.c_void_p]),
     ("dxg_grid_size", u64, [C.c_void_p]),
     ("dxg_transport_run", C.c_int, [C.c_void_p, C.POINTER(Beam), C.c_void_p, C.c_int, u64]),
+    ("dxg_transport_run_static", C.c_int, [C.c_void_p, C.POINTER(Beam), C.c_void_p, C.c_int, u64]),
     ("dxg_world_clear_scores", C.c_int, [C.c_void_p]),
     ("dxg_read_energy_scores", C.c_int, [C.c_void_p, PD, PD, C.POINTER(u64)]),
     ("dxg_read_dose_scores", C.c_int, [C.c_void_p, PD, PD, C.POINTER(u64)]),

@@ -172,6 +172,12 @@ struct WorldView {
 };
 int worldView(const dxg_world* w, WorldView& out);
 
+// The allocation-free run's scratch array (transport.hip): its size, and its
+// address on `device`, which must be the current device (hipGetSymbolAddress
+// on the first call per device; a hipError_t).
+size_t staticScratchBytes();
+int staticScratch(int device, void** out);
+
 // Run preparation shared by dxg_transport_run and batched runs: validates the
 // beam, uploads its tables to the world and fills p (all but the history
 // range); total receives the beam's history count.

@@ -39,6 +39,8 @@
 //   (level 1: x^2 drawn from F^2(x), accepted with (1+cos^2)/2).
 #include "device.hpp"
 
+#include <mutex>
+
 namespace dxg {
 
 namespace {
@@ -1144,5 +1146,44 @@ void launchCopy(const void* src, void* dst, uint64_t bytes, hipStream_t s)
 }
 
 int transportBlockSize() { return kBlock; }
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free run (dxg_transport_run_static): one array per
+// device, held until the process ends.  Host code never passes `scratch`
+// itself (in host code the name is a stand-in, not a device address): it asks
+// staticScratch() for the array's address on the current device, taken once
+// per device with hipGetSymbolAddress.
+constexpr size_t kStaticScratchBytes = size_t(1) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};   // scratch's address per device, once taken
+}  // namespace
+
+size_t staticScratchBytes() { return kStaticScratchBytes; }
+
+int staticScratch(int device, void** out)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess)
+            return e;
+        if (cur != device)
+            return hipErrorInvalidDevice;   // the device must be current
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
 
 }  // namespace dxg

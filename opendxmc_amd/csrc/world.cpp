@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -495,21 +496,23 @@ int dxg_world_clear_scores(dxg_world* w)
 
 namespace dxg {
 
-// Validates a beam, uploads its exposure and spectrum/bowtie tables to the
-// world's device buffers and fills the launch parameters (everything but the
-// history range).  total receives the beam's history count.
-int prepareRun(dxg_world* w, const dxg_beam* beam, uint64_t seed, DevParams& p, uint64_t& total)
+// The beam's device tables built on the host: exposure records, their first
+// beam history, the spectrum/bowtie blob staged into LDS, the history count.
+struct HostBeam {
+    std::vector<dxg::DevExposure> ex;
+    std::vector<unsigned long long> ex_start;
+    std::vector<uint32_t> blob;
+    dxg::BeamBlob bb {};
+    uint64_t total = 0;
+};
+
+// Validates a beam and builds its device tables on the host.
+int buildHostBeam(const dxg_beam* beam, HostBeam& hb)
 {
-    if (!w || !beam)
-        return fail(DXG_ERR_ARGUMENT, "dxg_transport_run: null argument");
-    if (!w->built)
-        return fail(DXG_ERR_STATE, "dxg_transport_run: World::build() has not run");
     if (beam->n_exposures == 0 || !beam->exposures)
         return fail(DXG_ERR_ARGUMENT, "dxg_transport_run: beam has no exposures");
     if (beam->n_spectra == 0 || !beam->spectra)
         return fail(DXG_ERR_ARGUMENT, "dxg_transport_run: beam has no spectrum");
-    HIP_TRY(hipSetDevice(w->device));
-
     // ---- spectra -> alias tables
     std::vector<float> se, sp;
     std::vector<int> sa, soff, sn;
@@ -571,8 +574,11 @@ int prepareRun(dxg_world* w, const dxg_beam* beam, uint64_t seed, DevParams& p, 
         bw.push_back(1.f);
     }
     // ---- exposures
-    std::vector<dxg::DevExposure> ex(beam->n_exposures);
-    std::vector<unsigned long long> ex_start(beam->n_exposures);
+    std::vector<dxg::DevExposure>& ex = hb.ex;
+    std::vector<unsigned long long>& ex_start = hb.ex_start;
+    ex.assign(beam->n_exposures, dxg::DevExposure {});
+    ex_start.assign(beam->n_exposures, 0);
+    uint64_t& total = hb.total;
     total = 0;
     for (uint64_t i = 0; i < beam->n_exposures; ++i) {
         const dxg_exposure& e = beam->exposures[i];
@@ -606,10 +612,8 @@ int prepareRun(dxg_world* w, const dxg_beam* beam, uint64_t seed, DevParams& p, 
     }
     if (total == 0)
         return fail(DXG_ERR_ARGUMENT, "beam has zero particles");
-    HIP_TRY(w->d_ex.upload(ex, w->stream));
-    HIP_TRY(w->d_ex_start.upload(ex_start, w->stream));
     // ---- the small per-beam tables travel as one blob staged into LDS
-    dxg::BeamBlob bb {};
+    dxg::BeamBlob& bb = hb.bb;
     bb.n_spec = static_cast<int>(soff.size());
     bb.n_bow = static_cast<int>(boff.size());
     bb.n_bins = static_cast<int>(se.size());
@@ -624,7 +628,8 @@ int prepareRun(dxg_world* w, const dxg_beam* beam, uint64_t seed, DevParams& p, 
     bb.o_angle = bb.o_alias + bb.n_bins;
     bb.o_weight = bb.o_angle + bb.n_nodes;
     bb.words = bb.o_weight + bb.n_nodes;
-    std::vector<uint32_t> blob(static_cast<size_t>(bb.words));
+    std::vector<uint32_t>& blob = hb.blob;
+    blob.assign(static_cast<size_t>(bb.words), 0u);
     auto fbits = [](float f) {
         uint32_t u;
         std::memcpy(&u, &f, 4);
@@ -647,18 +652,22 @@ int prepareRun(dxg_world* w, const dxg_beam* beam, uint64_t seed, DevParams& p, 
         blob[static_cast<size_t>(bb.o_angle + i)] = fbits(ba[static_cast<size_t>(i)]);
         blob[static_cast<size_t>(bb.o_weight + i)] = fbits(bw[static_cast<size_t>(i)]);
     }
-    HIP_TRY(w->d_blob.upload(blob, w->stream));
+    return DXG_OK;
+}
 
+// Fills the launch parameters of a run of `beam` on world w (everything but
+// the history range and the device addresses of the beam tables, the pool and
+// the counters) and plans the LDS use.
+int runParams(dxg_world* w, const dxg_beam* beam, const HostBeam& hb, uint64_t seed, DevParams& p)
+{
+    const dxg::BeamBlob& bb = hb.bb;
     fillParams(w, p);
-    p.exposures = w->d_ex.ptr;
-    p.ex_start = w->d_ex_start.ptr;
     p.n_ex = static_cast<int>(beam->n_exposures);
     p.ppe = beam->exposures[0].n_particles;
     for (uint64_t i = 1; i < beam->n_exposures; ++i)
         if (beam->exposures[i].n_particles != p.ppe)
             p.ppe = 0;
     p.inv_ppe = p.ppe ? 1.0 / static_cast<double>(p.ppe) : 0.0;
-    p.blob = w->d_blob.ptr;
     p.bb = bb;
     p.seed = seed;
     p.rng_offset = w->histories;
@@ -699,6 +708,31 @@ int prepareRun(dxg_world* w, const dxg_beam* beam, uint64_t seed, DevParams& p, 
     return DXG_OK;
 }
 
+// Validates a beam, uploads its exposure and spectrum/bowtie tables to the
+// world's device buffers and fills the launch parameters (everything but the
+// history range).  total receives the beam's history count.
+int prepareRun(dxg_world* w, const dxg_beam* beam, uint64_t seed, DevParams& p, uint64_t& total)
+{
+    if (!w || !beam)
+        return fail(DXG_ERR_ARGUMENT, "dxg_transport_run: null argument");
+    if (!w->built)
+        return fail(DXG_ERR_STATE, "dxg_transport_run: World::build() has not run");
+    HostBeam hb;
+    if (int rc = buildHostBeam(beam, hb); rc != DXG_OK)
+        return rc;
+    HIP_TRY(hipSetDevice(w->device));
+    HIP_TRY(w->d_ex.upload(hb.ex, w->stream));
+    HIP_TRY(w->d_ex_start.upload(hb.ex_start, w->stream));
+    HIP_TRY(w->d_blob.upload(hb.blob, w->stream));
+    if (int rc = runParams(w, beam, hb, seed, p); rc != DXG_OK)
+        return rc;
+    p.exposures = w->d_ex.ptr;
+    p.ex_start = w->d_ex_start.ptr;
+    p.blob = w->d_blob.ptr;
+    total = hb.total;
+    return DXG_OK;
+}
+
 void addHistories(dxg_world* w, uint64_t n, const dxg_counters& c)
 {
     w->histories += n;
@@ -707,18 +741,12 @@ void addHistories(dxg_world* w, uint64_t n, const dxg_counters& c)
 
 int worldDevice(const dxg_world* w) { return w->device; }
 
-}  // namespace dxg
-
-extern "C" {
-
-int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress, int apply_calibration, uint64_t seed)
+// The launches of one run: the persistent transport kernel over the history
+// range in equal chunks (progress and cancel act between them), event
+// counters summed into the world.  With `events` the kernel time comes from
+// HIP events around each launch, else from the host clock.
+int runChunks(dxg_world* w, DevParams& p, uint64_t total, dxg_progress* progress, bool events)
 {
-    dxg::DevParams p;
-    uint64_t total = 0;
-    if (int rc = dxg::prepareRun(w, beam, seed, p, total); rc != DXG_OK)
-        return rc;
-    if (apply_calibration && beam->calibration > 0.0)
-        p.dose_scale = static_cast<float>(beam->calibration);
     const size_t lds_bytes = dxg::transportLdsBytes(p);
 
     // ---- launch geometry: fill every CU to the occupancy limit
@@ -748,8 +776,10 @@ int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress
         progress->setMessage("Simulating " + std::to_string(total) + " histories");
     }
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    HIP_TRY(hipEventCreate(&ev0));
-    HIP_TRY(hipEventCreate(&ev1));
+    if (events) {
+        HIP_TRY(hipEventCreate(&ev0));
+        HIP_TRY(hipEventCreate(&ev1));
+    }
     dxg_counters acc {};
     int status = DXG_OK;
     for (uint64_t h0 = 0; h0 < total; h0 += chunk) {
@@ -759,16 +789,22 @@ int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress
         }
         p.h_begin = h0;
         p.h_end = std::min(total, h0 + chunk);
-        dxg::launchReset(w->d_pool.ptr, w->d_counters.ptr, w->stream);
-        HIP_TRY(hipEventRecord(ev0, w->stream));
+        dxg::launchReset(p.pool, p.counters, w->stream);
+        const auto t0 = std::chrono::steady_clock::now();
+        if (events)
+            HIP_TRY(hipEventRecord(ev0, w->stream));
         dxg::launchTransport(p, blocks, block, w->stream);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(ev1, w->stream));
+        if (events)
+            HIP_TRY(hipEventRecord(ev1, w->stream));
         dxg::DevCounters c {};
-        HIP_TRY(hipMemcpyAsync(&c, w->d_counters.ptr, sizeof c, hipMemcpyDeviceToHost, w->stream));
+        HIP_TRY(hipMemcpyAsync(&c, p.counters, sizeof c, hipMemcpyDeviceToHost, w->stream));
         HIP_TRY(hipStreamSynchronize(w->stream));
         float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+        if (events)
+            HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+        else   // no events without allocation: the host's clock around launch and wait
+            ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
         acc.kernel_ms += ms;
         acc.kernel_launches += 1;
         acc.histories += c.histories;
@@ -786,12 +822,73 @@ int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress
         if (progress)
             progress->n.store(p.h_end);
     }
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
+    if (events) {
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+    }
     w->counters = acc;
     if (progress && status == DXG_OK)
         progress->setMessage("Simulation finished");
     return status;
+}
+
+
+}  // namespace dxg
+
+extern "C" {
+
+int dxg_transport_run(dxg_world* w, const dxg_beam* beam, dxg_progress* progress, int apply_calibration, uint64_t seed)
+{
+    dxg::DevParams p;
+    uint64_t total = 0;
+    if (int rc = dxg::prepareRun(w, beam, seed, p, total); rc != DXG_OK)
+        return rc;
+    if (apply_calibration && beam->calibration > 0.0)
+        p.dose_scale = static_cast<float>(beam->calibration);
+    return dxg::runChunks(w, p, total, progress, true);
+}
+
+int dxg_transport_run_static(dxg_world* w, const dxg_beam* beam, dxg_progress* progress, int apply_calibration, uint64_t seed)
+{
+    if (!w || !beam)
+        return fail(DXG_ERR_ARGUMENT, "dxg_transport_run_static: null argument");
+    if (!w->built)
+        return fail(DXG_ERR_STATE, "dxg_transport_run_static: World::build() has not run");
+    dxg::HostBeam hb;
+    if (int rc = dxg::buildHostBeam(beam, hb); rc != DXG_OK)
+        return rc;
+    // scratch layout: [0, 64) pool counter, [64, 256) event counters, then the
+    // beam blob, the exposures' first histories and the exposure records, each
+    // at a 256-byte boundary
+    auto up256 = [](size_t b) { return (b + 255) / 256 * 256; };
+    static_assert(sizeof(dxg::DevCounters) <= 192, "counters fit their scratch slot");
+    const size_t off_blob = 256;
+    const size_t off_start = off_blob + up256(hb.blob.size() * sizeof(uint32_t));
+    const size_t off_ex = off_start + up256(hb.ex_start.size() * sizeof(unsigned long long));
+    const size_t need = off_ex + hb.ex.size() * sizeof(dxg::DevExposure);
+    if (need > dxg::staticScratchBytes())
+        return fail(DXG_ERR_ARGUMENT, "dxg_transport_run_static: the beam's tables need " + std::to_string(need)
+                                          + " bytes of scratch, more than its " + std::to_string(dxg::staticScratchBytes())
+                                          + " (about 12,000 exposures with small spectra); use dxg_transport_run");
+    HIP_TRY(hipSetDevice(w->device));
+    void* base = nullptr;
+    HIP_TRY(static_cast<hipError_t>(dxg::staticScratch(w->device, &base)));
+    char* sc = static_cast<char*>(base);
+    HIP_TRY(hipMemcpyAsync(sc + off_blob, hb.blob.data(), hb.blob.size() * sizeof(uint32_t), hipMemcpyHostToDevice, w->stream));
+    HIP_TRY(hipMemcpyAsync(sc + off_start, hb.ex_start.data(), hb.ex_start.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                           w->stream));
+    HIP_TRY(hipMemcpyAsync(sc + off_ex, hb.ex.data(), hb.ex.size() * sizeof(dxg::DevExposure), hipMemcpyHostToDevice, w->stream));
+    dxg::DevParams p;
+    if (int rc = dxg::runParams(w, beam, hb, seed, p); rc != DXG_OK)
+        return rc;
+    p.pool = reinterpret_cast<unsigned long long*>(sc);
+    p.counters = reinterpret_cast<dxg::DevCounters*>(sc + 64);
+    p.blob = reinterpret_cast<const uint32_t*>(sc + off_blob);
+    p.ex_start = reinterpret_cast<const unsigned long long*>(sc + off_start);
+    p.exposures = reinterpret_cast<const dxg::DevExposure*>(sc + off_ex);
+    if (apply_calibration && beam->calibration > 0.0)
+        p.dose_scale = static_cast<float>(beam->calibration);
+    return dxg::runChunks(w, p, hb.total, progress, false);
 }
 
 int dxg_read_energy_scores(const dxg_world* w, double* sum_e, double* sum_e2, uint64_t* n_events)
