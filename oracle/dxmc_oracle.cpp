@@ -1160,4 +1160,101 @@ int oracle_transport(const uint64_t dims[3], const double spacing[3], const doub
     return 0;
 }
 
+// CT beam calibration (the CT beams' CTDIw / CTDIvol and CTDIdiameter,
+// beamsettingsmodel.cpp:1054-1071, 1276-1295, applied with useBeamCalibration,
+// simulationpipeline.cpp:165; DXMClib's own chain is absent, so this restates
+// the build's definition): one axial rotation (`rotation`, all exposures with
+// one particle count) in a PMMA cylinder (1.19 g/cm3, H/C/O by mass,
+// `diameter_cm` across, 15 cm long) in dry air (0.001205 g/cm3) on a 0.25 cm
+// grid whose voxels count by their centres; five 100 mm x 1 cm chambers of
+// PMMA, one on the axis and four 1 cm below the surface at 0, 90, 180, 270
+// degrees.  Chamber dose = energy / mass; CTDI100 = dose x 10 cm / collimation;
+// CTDIw = centre / 3 + 2/3 x mean of the four; factor = ctdi_w / CTDIw x
+// (rotation particles per exposure / run particles per exposure).  Outputs:
+// ctdi100[5] and their standard errors se100[5] (mGy per the rotation's
+// histories, each chamber's energy as a sum over histories), CTDIw and the
+// factor.
+int oracle_ctdi(const oracle_exposure* rotation, uint64_t n_exposures, const oracle_spectrum* spectra, uint64_t n_spectra,
+    const oracle_bowtie* bowties, uint64_t n_bowties, double ctdi_w, double diameter_cm, double collimation_cm,
+    uint64_t run_particles_per_exposure, uint64_t seed, int n_threads, double* ctdi100, double* se100, double* ctdiw_sim,
+    double* factor)
+{
+    if (!rotation || n_exposures == 0 || !ctdi100 || !se100 || !ctdiw_sim || !factor || !(collimation_cm > 0)
+        || run_particles_per_exposure == 0) {
+        g_err = "oracle_ctdi: bad argument";
+        return -1;
+    }
+    constexpr double h = 0.25, length = 15.0, chamber_half = 5.0, chamber_r = 0.5, depth = 1.0;
+    constexpr double rho_pmma = 1.19, rho_air = 0.001205;
+    const double R = 0.5 * diameter_cm;
+    const uint64_t nxy = 2 * static_cast<uint64_t>(std::ceil((R + 1.0) / h));
+    const uint64_t nz = 2 * static_cast<uint64_t>(std::ceil((0.5 * length + 0.5) / h));
+    const uint64_t dims[3] = { nxy, nxy, nz };
+    const double spacing[3] = { h, h, h };
+    const size_t n = nxy * nxy * nz;
+    std::vector<double> density(n, rho_air);
+    std::vector<uint8_t> material(n, 0);
+    std::vector<int> chamber(n, -1);
+    const double px[5] = { 0.0, R - depth, 0.0, -(R - depth), 0.0 };
+    const double py[5] = { 0.0, 0.0, R - depth, 0.0, -(R - depth) };
+    for (uint64_t k = 0; k < nz; ++k)
+        for (uint64_t j = 0; j < nxy; ++j)
+            for (uint64_t i = 0; i < nxy; ++i) {
+                const double x = (static_cast<double>(i) + 0.5) * h - 0.5 * static_cast<double>(nxy) * h;
+                const double y = (static_cast<double>(j) + 0.5) * h - 0.5 * static_cast<double>(nxy) * h;
+                const double z = (static_cast<double>(k) + 0.5) * h - 0.5 * static_cast<double>(nz) * h;
+                const size_t v = i + nxy * (j + nxy * k);
+                if (x * x + y * y > R * R || std::fabs(z) > 0.5 * length)
+                    continue;
+                material[v] = 1;
+                density[v] = rho_pmma;
+                if (std::fabs(z) > chamber_half)
+                    continue;
+                for (int c = 0; c < 5; ++c)
+                    if ((x - px[c]) * (x - px[c]) + (y - py[c]) * (y - py[c]) <= chamber_r * chamber_r)
+                        chamber[v] = c;
+            }
+    const int air_z[4] = { 6, 7, 8, 18 };
+    const double air_w[4] = { 0.000124, 0.755268, 0.231781, 0.012827 };
+    const int pmma_z[3] = { 1, 6, 8 };
+    const double pmma_w[3] = { 0.080538, 0.599848, 0.319614 };
+    const oracle_material mats[2] = { { air_z, air_w, 4 }, { pmma_z, pmma_w, 3 } };
+    std::vector<double> e(n, 0.0), e2(n, 0.0);
+    std::vector<uint64_t> ev(n, 0);
+    oracle_counters cnt {};
+    int rc = oracle_transport(dims, spacing, density.data(), material.data(), mats, 2, rotation, n_exposures, spectra, n_spectra,
+        bowties, n_bowties, 1, seed, 0, n_threads, e.data(), e2.data(), ev.data(), &cnt);
+    if (rc != 0)
+        return rc;
+    double energy[5] = {}, energy2[5] = {}, mass_kg[5] = {};
+    const double voxel_kg = h * h * h * rho_pmma / 1000.0;
+    for (size_t v = 0; v < n; ++v)
+        if (chamber[v] >= 0) {
+            energy[chamber[v]] += e[v];
+            energy2[chamber[v]] += e2[v];
+            mass_kg[chamber[v]] += voxel_kg;
+        }
+    const double H = static_cast<double>(cnt.histories);
+    constexpr double joule_per_kev = 1.602176634e-16;
+    for (int c = 0; c < 5; ++c) {
+        if (!(mass_kg[c] > 0)) {
+            g_err = "oracle_ctdi: phantom too small for its chambers";
+            return -2;
+        }
+        const double to_ctdi = joule_per_kev / mass_kg[c] * 1000.0 * (2.0 * chamber_half) / collimation_cm;
+        ctdi100[c] = energy[c] * to_ctdi;
+        // variance of the chamber's energy as a sum of per-deposit scores
+        const double var = H > 1 ? std::max(0.0, (energy2[c] - energy[c] * energy[c] / H) * H / (H - 1.0)) : 0.0;
+        se100[c] = std::sqrt(var) * to_ctdi;
+    }
+    const double sim = ctdi100[0] / 3.0 + (2.0 / 3.0) * 0.25 * (ctdi100[1] + ctdi100[2] + ctdi100[3] + ctdi100[4]);
+    if (!(sim > 0)) {
+        g_err = "oracle_ctdi: no dose reached the chambers";
+        return -3;
+    }
+    *ctdiw_sim = sim;
+    *factor = ctdi_w / sim * static_cast<double>(rotation[0].n_particles) / static_cast<double>(run_particles_per_exposure);
+    return 0;
+}
+
 }  // extern "C"

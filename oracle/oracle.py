@@ -61,6 +61,8 @@ def lib() -> C.CDLL:
         h.oracle_traverse_f32.argtypes = [C.POINTER(u64), C.POINTER(C.c_float), C.POINTER(C.c_uint8), u64,
                                           C.POINTER(C.c_float), C.POINTER(C.c_float), u64, C.POINTER(C.c_float),
                                           C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+        h.oracle_ctdi.argtypes = [C.c_void_p, u64, C.POINTER(Spectrum), u64, C.POINTER(Bowtie), u64, C.c_double, C.c_double,
+                                  C.c_double, u64, u64, C.c_int, PD, PD, PD, PD]
         h.oracle_dose_scores.argtypes = [u64, PD, PD, u64, PD, PD, PD, PD]
         h.oracle_sample_source.argtypes = [C.c_void_p, C.POINTER(Spectrum), C.POINTER(Bowtie), u64, u64, PD]
         h.oracle_transport.argtypes = [C.POINTER(u64), PD, PD, C.POINTER(C.c_uint8), C.POINTER(Material), u64,
@@ -170,6 +172,27 @@ def dose_scores(density, spacing, histories: int, sum_e, sum_e2) -> tuple[np.nda
                                 e2.ctypes.data_as(PD), dose.ctypes.data_as(PD), var.ctypes.data_as(PD)) != 0:
         raise RuntimeError(_err())
     return dose, var
+
+
+def ctdi(rotation: np.ndarray, spectra, bowties, ctdi_w: float, diameter_cm: float, collimation_cm: float,
+         run_particles_per_exposure: int, seed: int = 1, threads: int = 8) -> dict:
+    """The CTDI calibration chain on the CPU: per-chamber CTDI100 and their SE,
+    CTDIw of the rotation and the calibration factor."""
+    k = _Keep()
+    ex = k.keep(np.ascontiguousarray(rotation))
+    sp = [Spectrum(k.keep(np.ascontiguousarray(e, dtype=np.float64)).ctypes.data_as(PD),
+                   k.keep(np.ascontiguousarray(w, dtype=np.float64)).ctypes.data_as(PD), len(e)) for e, w in spectra]
+    sp_arr = k.keep((Spectrum * max(1, len(sp)))(*sp))
+    bw = [Bowtie(k.keep(np.ascontiguousarray(a, dtype=np.float64)).ctypes.data_as(PD),
+                 k.keep(np.ascontiguousarray(w, dtype=np.float64)).ctypes.data_as(PD), len(a)) for a, w in bowties]
+    bw_arr = k.keep((Bowtie * max(1, len(bw)))(*bw))
+    c100, se = np.zeros(5), np.zeros(5)
+    w_sim, f = C.c_double(), C.c_double()
+    if lib().oracle_ctdi(ex.ctypes.data, len(ex), sp_arr, len(sp), bw_arr, len(bw), ctdi_w, diameter_cm, collimation_cm,
+                         int(run_particles_per_exposure), seed, threads, c100.ctypes.data_as(PD), se.ctypes.data_as(PD),
+                         C.byref(w_sim), C.byref(f)) != 0:
+        raise RuntimeError(_err())
+    return {"ctdi100": c100, "se100": se, "ctdiw": w_sim.value, "factor": f.value}
 
 
 def rng_stream(seed: int, history: int, n: int) -> np.ndarray:

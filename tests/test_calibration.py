@@ -188,3 +188,77 @@ def test_dap_calibrated_dose_is_independent_of_particle_count():
         d = grid.doseScored()[0]
         totals.append(float((d * ph.density).sum()))
     assert totals[1] == pytest.approx(totals[0], rel=0.01)
+
+
+def ctdi_rotation(diameter_cm: float, bowtie: bool, ppe: int, step_deg: float = 10.0):
+    """One axial rotation of a 120 kV CT beam (9 mm Al, 4 cm collimation) as
+    the CTDI calibration simulates it."""
+    from opendxmc_amd.engine import _Rotation
+    beam = dx.CTSequentialBeam((0, 0, 0))
+    beam.setCollimation(4.0)
+    beam.setStepAngleDeg(step_deg)
+    if bowtie:
+        beam.setBowtieFilter(read_filters()[DEFAULT_CT_BOWTIE])
+    beam.setCTDIdiameter(diameter_cm)
+    rot = _Rotation(beam)
+    ex = rot.exposures().copy()
+    ex["n_particles"] = ppe
+    return beam, ex, rot.spectra(), [(b.angle, b.weight) for b in rot.bowties()]
+
+
+@pytest.mark.parametrize("diameter,bowtie,lo,hi", [(32.0, True, 0.35, 0.65), (16.0, False, 0.75, 1.15)])
+def test_oracle_ctdi_centre_to_periphery(oracle, diameter, bowtie, lo, hi):
+    """The oracle's CTDI phantom and chambers against what CT dosimetry
+    publishes for them: at 120 kV the centre chamber's CTDI100 is about half
+    the peripheral mean in the 32 cm body phantom (body bowtie) and close to
+    it in the 16 cm head phantom (here without a bowtie, which lowers the
+    centre a little) (e.g. AAPM Report 96, 2008; ImPACT CTDI tables).  A
+    chamber at the surface instead of 1 cm below it, or a radius taken for a
+    diameter, moves the ratio outside these bands."""
+    _, ex, spectra, bows = ctdi_rotation(diameter, bowtie, 200_000)
+    r = oracle.ctdi(ex, spectra, bows, 1.0, diameter, 4.0, 200_000, seed=3, threads=8)
+    c = r["ctdi100"]
+    ratio = c[0] / np.mean(c[1:])
+    print(f"{diameter} cm: CTDI100 {np.round(c / c.max(), 3)}, centre / periphery {ratio:.3f}, rel SE {r['se100'] / c}")
+    assert np.all(r["se100"] < 0.04 * c)
+    assert lo <= ratio <= hi
+    # the four peripheral chambers see the same dose (a full rotation)
+    assert np.std(c[1:]) / np.mean(c[1:]) < 0.05
+    assert r["ctdiw"] == pytest.approx(c[0] / 3 + 2 / 3 * np.mean(c[1:]), rel=1e-12)
+    assert r["factor"] == pytest.approx(1.0 / r["ctdiw"], rel=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("diameter,bowtie", [(32.0, True), (16.0, False)])
+def test_ctdi_calibration_matches_oracle(oracle, diameter, bowtie):
+    """dxg_calibration_ctdi's simulated CTDIw against the oracle's restatement
+    of the same chain, same rotation, within 4 combined standard errors; the
+    factor is CTDIw(stated) / CTDIw(simulated) x the particle-count ratio."""
+    from opendxmc_amd.engine import _Keep, beam_struct
+    import ctypes as C
+
+    ppe = 100_000
+    beam, ex, spectra, bows = ctdi_rotation(diameter, bowtie, ppe)
+
+    class Rot:
+        def exposures(self):
+            return ex
+
+        def spectra(self):
+            return spectra
+
+        def bowties(self):
+            return [dx.BowtieFilter(list(zip(a, w))) for a, w in bows]
+
+    k = _Keep()
+    b = beam_struct(Rot(), k, calibrated=False)
+    f, sim = C.c_double(), C.c_double()
+    N.check(N.lib().dxg_calibration_ctdi(C.byref(b), 10.0, diameter, 4.0, 50_000, 12345, C.byref(f), C.byref(sim)))
+    r = oracle.ctdi(ex, spectra, bows, 10.0, diameter, 4.0, 50_000, seed=99, threads=16)
+    s = r["se100"]
+    se_w = math.sqrt((s[0] / 3) ** 2 + sum((2 / 3 * 0.25 * x) ** 2 for x in s[1:]))
+    z = (sim.value - r["ctdiw"]) / (math.sqrt(2.0) * se_w)
+    print(f"{diameter} cm: CTDIw gpu {sim.value:.6e} oracle {r['ctdiw']:.6e} (rel SE {se_w / r['ctdiw']:.4f}) z {z:+.2f}",
+          flush=True)
+    assert abs(z) < 4.0
+    assert f.value == pytest.approx(10.0 / sim.value * ppe / 50_000, rel=1e-12)
