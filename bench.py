@@ -175,7 +175,20 @@ def ct_dual_workload(histories: int):
     return ph, beam
 
 
+def pencil_workload(histories: int):
+    """Plain MC, small: a 60 keV pencil beam along +z (through a voxel centre,
+    x = y = 0.05 cm) into a 64^3 water cube at 0.1 cm."""
+    import opendxmc_amd as dx
+    from opendxmc_amd import phantoms
+
+    ph = phantoms.water_cube(64, 0.1)
+    beam = dx.PencilBeam((0.05, 0.05, -10.0), (0.0, 0.0, 1.0), 60.0)
+    beam.setNumberOfParticlesPerExposure(int(histories))
+    return ph, beam
+
+
 WORKLOADS = {
+    "pencil-64": (pencil_workload, 1e6, "Plain MC small, 60 keV pencil beam into a 64^3 water cube at 0.1 cm"),
     "ct-spiral-256": (ct_workload, 1e8, "CT spiral plain MC, 256x256x200 at 0.15 cm seeded body phantom, 120 kVp 6 mm Al, "
                                         "Siemens Definition Flash W1 120kV bowtie, 4 cm collimation, pitch 1, 30 cm"),
     "ct-spiral-512-dual": (ct_dual_workload, 1e9, "CT spiral dual source plain MC, 512x512x400 at 0.08 cm seeded body "
@@ -387,6 +400,9 @@ def main() -> int:
         transport.seed = 1000 + i
         transport(world, beam, None, True)
         print(f"warmup {i} done", file=sys.stderr, flush=True)
+    c = grid.counters()   # the launch settings the warm-up runs chose
+    launch = {"refill_threshold": c["refill_threshold"], "interact_threshold": c["interact_threshold"],
+              "outer_threshold": c["outer_threshold"], "batch": c["batch"], "autotuned": bool(c["autotuned"])}
     steps = bytes_alg = kernel_ms = launches = hist = deposits = crossings = 0
     phase = [0, 0, 0]
     t0 = time.perf_counter()
@@ -431,6 +447,7 @@ def main() -> int:
             "deposits_per_history": deposits / max(hist, 1),
             "cell_crossings_per_history": crossings / max(hist, 1),
             "kernel_launches_per_step": launches / max(args.steps, 1),
+            "launch_settings": launch,
         },
         "roofline": {
             "bound": "hbm",
@@ -449,7 +466,7 @@ def main() -> int:
     if sum(phase) > 0:   # libraries built with DXG_PHASE_CLOCKS
         result["config"]["phase_cycle_share"] = {k: v / sum(phase) for k, v in zip(("refill", "interact", "step"), phase)}
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(ph, beam)
+        result["cpu_baseline"] = cpu_baseline(ph, beam, what=description)
     if not args.no_time_to_2pct and args.workload == "ct-spiral-256":
         del world, grid
         result["time_to_2pct"] = time_to_2pct(args.t2_ref_histories, None if args.no_cpu_baseline else 10.0)

@@ -109,6 +109,17 @@ typedef struct dxg_counters {
     uint64_t cell_crossings;   /* macro-cell boundary crossings (majorant changes, no voxel read) */
     uint64_t phase_cycles[3];  /* wavefront clock cycles spent refilling, interacting and stepping
                                   (libraries built with DXG_PHASE_CLOCKS; zero otherwise) */
+    /* the launch settings the run ended with: dead / parked / outside lanes
+     * that start a refill / interaction / outer-flight phase, histories a
+     * wavefront claims per pool atomic, and 1 when they were chosen by the
+     * run's own timing (dxg_transport_run tunes them on the first run of a
+     * beam layout on a built world and keeps them; DXG_AUTOTUNE=0 or any
+     * explicit DXG_REFILL / DXG_INTERACT / DXG_OUTERT / DXG_BATCH turns that off) */
+    int32_t refill_threshold;
+    int32_t interact_threshold;
+    int32_t outer_threshold;
+    int32_t autotuned;
+    uint64_t batch;
 } dxg_counters;
 
 const char* dxg_last_error(void);

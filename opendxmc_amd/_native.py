@@ -69,6 +69,8 @@ class Counters(C.Structure):
         ("emitted_energy", C.c_double), ("escaped_energy", C.c_double),
         ("deposited_energy", C.c_double), ("kernel_ms", C.c_double), ("kernel_launches", u64),
         ("uncollided", u64), ("cell_crossings", u64), ("phase_cycles", u64 * 3),
+        ("refill_threshold", C.c_int32), ("interact_threshold", C.c_int32), ("outer_threshold", C.c_int32),
+        ("autotuned", C.c_int32), ("batch", u64),
     ]
 
     def as_dict(self) -> dict:

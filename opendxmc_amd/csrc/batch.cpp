@@ -306,6 +306,8 @@ int dxg_batch_run(dxg_batch* b, double* kernel_ms)
         acc.kernel_ms = ms;          // the shared launch
         acc.kernel_launches = 1;
         dxg::addHistories(b->inst[static_cast<size_t>(i)].world, b->inst[static_cast<size_t>(i)].histories, acc);
+        if (int rc = dxg::foldTally(b->inst[static_cast<size_t>(i)].world); rc != DXG_OK)
+            return rc;
     }
     if (kernel_ms)
         *kernel_ms = ms;

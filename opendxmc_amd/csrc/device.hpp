@@ -112,8 +112,11 @@ struct DevParams {
     float cell_qscale;
     int cell_lds;
 
-    // tally: one record of kTallyRecord doubles per voxel
+    // tally: one record of kTallyRecord doubles per voxel, in tally_replicas
+    // copies tally_stride doubles apart; work-group b adds to copy b % replicas
     double* tally;
+    unsigned long long tally_stride;
+    int tally_replicas;
     float dose_scale;   // beam calibration applied to every deposit (1: none)
 
     // beam
@@ -144,6 +147,8 @@ void launchTransport(const DevParams& p, int blocks, int threads, hipStream_t s)
 void launchTraverse(const DevParams& p, uint64_t n_rays, const float* pos, const float* dir, uint64_t n_steps,
                     const float* steps, int64_t* out_index, int32_t* out_material, hipStream_t s);
 void launchReset(unsigned long long* pool, DevCounters* counters, hipStream_t s);
+// Adds tally copies 1..replicas-1 into copy 0 (in that order) and zeroes them.
+void launchTallyFold(double* tally, unsigned long long stride, int replicas, hipStream_t s);
 // Source particles of beam histories [0, n): 8 floats each (ax, ay, direction,
 // E, w, exposure index), from the transport kernels' sampling code.
 void launchSource(const DevParams& p, uint64_t n, float* out, hipStream_t s);
@@ -182,6 +187,9 @@ int staticScratch(int device, void** out);
 // beam, uploads its tables to the world and fills p (all but the history
 // range); total receives the beam's history count.
 int prepareRun(dxg_world* w, const dxg_beam* beam, uint64_t seed, DevParams& p, uint64_t& total);
+// Folds the world's tally copies into one after a run (no-op with one copy);
+// enqueued on the world's stream.
+int foldTally(dxg_world* w);
 // After a run outside dxg_transport_run: account n histories and set the counters.
 void addHistories(dxg_world* w, uint64_t n, const dxg_counters& c);
 int worldDevice(const dxg_world* w);

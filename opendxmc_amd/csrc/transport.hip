@@ -252,7 +252,7 @@ __device__ inline void rotate(float& dx, float& dy, float& dz, float cost, float
 // request; so a record update is spread over three consecutive lanes of one
 // instruction (one value each, all in the record's aligned 32 bytes) -- one
 // request per update where three separate atomics made three.
-__device__ inline void tallyFlush(const DevParams& P, int deps, int vi, float e1, float e2)
+__device__ inline void tallyFlush(const DevParams& P, double* tally, int deps, int vi, float e1, float e2)
 {
     const unsigned long long m = __ballot(deps > 0);
     const int k = __popcll(m);
@@ -276,7 +276,7 @@ __device__ inline void tallyFlush(const DevParams& P, int deps, int vi, float e1
         const double d2 = static_cast<double>(__int_as_float(__builtin_amdgcn_ds_bpermute(src * 4, __float_as_int(e2)))) * scale;
         if (slot < 3 * k) {
             const double v = c == 0 ? d1 + d2 : (c == 1 ? d1 * d1 + d2 * d2 : static_cast<double>(n));
-            atomicAdd(P.tally + static_cast<size_t>(idx) * kTallyRecord + c, v);
+            atomicAdd(tally + static_cast<size_t>(idx) * kTallyRecord + c, v);
         }
     }
 }
@@ -458,6 +458,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
     };
     Rng rng { 1, 2, 3, 4 };
     bool collided = false;
+    double* const tally = P.tally + (P.tally_replicas > 1 ? static_cast<size_t>(blockIdx.x % P.tally_replicas) * P.tally_stride : 0);
 
     // wavefront-uniform event counters (popcounts of ballots, kept in SGPRs)
     unsigned long long n_steps = 0, n_real = 0, n_dep = 0, n_hist = 0, n_cross = 0;
@@ -695,7 +696,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     rotate(dx, dy, dz, cost, kTwoPi * rngU(rng));
                 }
             }
-            tallyFlush(P, deps, vi, e1, e2);
+            tallyFlush(P, tally, deps, vi, e1, e2);
             if (deps > 0)
                 atomicAdd(&e_wave[2], e_scored);
             n_real += n_int;
@@ -1025,6 +1026,20 @@ __global__ void source_kernel(const DevParams P, uint64_t n, float* __restrict__
     o[7] = static_cast<float>(ei);
 }
 
+__global__ void tally_fold_kernel(double* __restrict__ t, unsigned long long stride, int replicas)
+{
+    for (unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < stride;
+         i += static_cast<unsigned long long>(gridDim.x) * blockDim.x) {
+        double s = t[i];
+        for (int r = 1; r < replicas; ++r) {
+            double* q = t + static_cast<size_t>(r) * stride + i;
+            s += *q;
+            *q = 0.0;
+        }
+        t[i] = s;
+    }
+}
+
 __global__ void reset_kernel(unsigned long long* pool, DevCounters* c)
 {
     if (threadIdx.x == 0) {
@@ -1132,6 +1147,15 @@ void launchSource(const DevParams& p, uint64_t n, float* out, hipStream_t s)
 {
     const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
     hipLaunchKernelGGL(source_kernel, dim3(blocks), dim3(256), sizeof(uint32_t) * static_cast<size_t>(p.bb.words), s, p, n, out);
+}
+
+void launchTallyFold(double* tally, unsigned long long stride, int replicas, hipStream_t s)
+{
+    if (replicas <= 1)
+        return;
+    const unsigned long long want = (stride + 255) / 256;
+    const unsigned blocks = static_cast<unsigned>(std::min<unsigned long long>(want, 8192));
+    hipLaunchKernelGGL(tally_fold_kernel, dim3(blocks), dim3(256), 0, s, tally, stride, replicas);
 }
 
 void launchReset(unsigned long long* pool, DevCounters* counters, hipStream_t s)

@@ -13,8 +13,10 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace {
@@ -104,7 +106,8 @@ struct dxg_world {
     DevBuf<float> d_majorant, d_s, d_rfwd, d_rinv;
     DevBuf<float4> d_ia_shell;
     DevBuf<float> d_ia_inv, d_fl_cum, d_fl_line;
-    DevBuf<double> d_tally;   // one record of dxg::kTallyRecord doubles per voxel
+    DevBuf<double> d_tally;   // one record of dxg::kTallyRecord doubles per voxel, `replicas` copies
+    int replicas = 1;
     size_t tallyDoubles() const { return dxg::kTallyRecord * static_cast<size_t>(dims[0] * dims[1] * dims[2]); }
     DevBuf<float> d_cells;   // macro-cell majorant ratios
     DevBuf<uint8_t> d_cells_q;   // the same, as bytes of cell_qscale (padded to 4)
@@ -221,6 +224,8 @@ void fillParams(const dxg_world* w, dxg::DevParams& p)
     p.fl_cum = w->d_fl_cum.ptr;
     p.fl_line = w->d_fl_line.ptr;
     p.tally = w->d_tally.ptr;
+    p.tally_stride = w->tallyDoubles();
+    p.tally_replicas = w->replicas;
     p.pool = w->d_pool.ptr;
     p.counters = w->d_counters.ptr;
     if (w->cell > 0) {
@@ -465,8 +470,19 @@ int dxg_world_build(dxg_world* w)
             HIP_TRY(w->d_cells_q.upload(q, w->stream));
         }
     }
-    HIP_TRY(w->d_tally.alloc(w->tallyDoubles()));
-    HIP_TRY(hipMemsetAsync(w->d_tally.ptr, 0, w->tallyDoubles() * sizeof(double), w->stream));
+    // Copies of the tally for small grids: the FP64 atomics of one voxel record
+    // are applied one after another where the memory holds it, so a narrow
+    // beam on a small grid (a pencil beam's axis voxels take most first
+    // interactions) waits on a few records; work-groups spread over the copies
+    // and a fold after each run sums them.  Up to 16 copies within 128 MiB.
+    {
+        const size_t bytes = w->tallyDoubles() * sizeof(double);
+        w->replicas = static_cast<int>(std::clamp<size_t>((size_t(128) << 20) / std::max<size_t>(bytes, 1), 1, 16));
+        if (const char* env = std::getenv("DXG_TALLY_REPLICAS"))
+            w->replicas = std::clamp(std::atoi(env), 1, 64);
+    }
+    HIP_TRY(w->d_tally.alloc(w->tallyDoubles() * static_cast<size_t>(w->replicas)));
+    HIP_TRY(hipMemsetAsync(w->d_tally.ptr, 0, w->tallyDoubles() * static_cast<size_t>(w->replicas) * sizeof(double), w->stream));
     HIP_TRY(w->d_pool.alloc(1));
     HIP_TRY(w->d_counters.alloc(1));
     HIP_TRY(hipStreamSynchronize(w->stream));
@@ -486,7 +502,7 @@ int dxg_world_clear_scores(dxg_world* w)
 {
     if (!w || !w->built)
         return fail(DXG_ERR_STATE, "dxg_world_clear_scores: world not built");
-    HIP_TRY(hipMemsetAsync(w->d_tally.ptr, 0, w->tallyDoubles() * sizeof(double), w->stream));
+    HIP_TRY(hipMemsetAsync(w->d_tally.ptr, 0, w->tallyDoubles() * static_cast<size_t>(w->replicas) * sizeof(double), w->stream));
     HIP_TRY(hipStreamSynchronize(w->stream));
     w->histories = 0;
     return DXG_OK;
@@ -741,10 +757,69 @@ void addHistories(dxg_world* w, uint64_t n, const dxg_counters& c)
 
 int worldDevice(const dxg_world* w) { return w->device; }
 
+int foldTally(dxg_world* w)
+{
+    HIP_TRY(hipSetDevice(w->device));
+    dxg::launchTallyFold(w->d_tally.ptr, w->tallyDoubles(), w->replicas, w->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    return DXG_OK;
+}
+
+// Launch settings chosen by the run's own timing, kept per world build and
+// beam layout: dead lanes that start a refill, parked lanes that start an
+// interaction phase, lanes outside the box that start an outer-flight phase,
+// and histories a wavefront claims per pool atomic.  Events, tallies and
+// scores do not depend on them (per-history random streams), only time does.
+struct Tuned {
+    int refill, interact, outer;
+    unsigned long long batch;
+};
+constexpr Tuned kTuneCandidates[] = {
+    { 16, 24, 16, 256 }, { 12, 20, 12, 256 }, { 24, 32, 16, 256 }, { 16, 32, 24, 256 }, { 20, 24, 16, 512 }, { 12, 24, 8, 128 },
+};
+constexpr int kTuneRounds = 2;   // every candidate timed twice, interleaved; the faster launch counts
+
+struct TuneKey {
+    const dxg_world* w;
+    uint64_t generation;
+    int level, n_ex, lds_lut;
+    bool cells, uniform_ppe;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(w, generation, level, n_ex, lds_lut, cells, uniform_ppe)
+               < std::tie(o.w, o.generation, o.level, o.n_ex, o.lds_lut, o.cells, o.uniform_ppe);
+    }
+};
+std::mutex g_tune_mutex;
+std::map<TuneKey, Tuned> g_tuned;
+
+bool autotuneEnabled()
+{
+    if (const char* env = std::getenv("DXG_AUTOTUNE"); env && std::atoi(env) == 0)
+        return false;
+    for (const char* name : { "DXG_REFILL", "DXG_INTERACT", "DXG_OUTERT", "DXG_BATCH" })
+        if (std::getenv(name))
+            return false;
+    return true;
+}
+
+void applyTuned(DevParams& p, const Tuned& t)
+{
+    p.refill_threshold = t.refill;
+    p.interact_threshold = t.interact;
+    p.outer_threshold = t.outer;
+    p.batch = t.batch;
+}
+
 // The launches of one run: the persistent transport kernel over the history
 // range in equal chunks (progress and cancel act between them), event
 // counters summed into the world.  With `events` the kernel time comes from
-// HIP events around each launch, else from the host clock.
+// HIP events around each launch, else from the host clock.  With `events` and
+// autotuning on, the first run of a beam layout on a world build with enough
+// histories starts with short launches of each candidate setting
+// (kTuneCandidates, twice each, interleaved); the fastest per history is kept
+// for the rest of the run and for later runs of that layout.
 int runChunks(dxg_world* w, DevParams& p, uint64_t total, dxg_progress* progress, bool events)
 {
     const size_t lds_bytes = dxg::transportLdsBytes(p);
@@ -758,16 +833,6 @@ int runChunks(dxg_world* w, DevParams& p, uint64_t total, dxg_progress* progress
     if (const char* env = std::getenv("DXG_BLOCKS"))
         blocks = std::max(1, std::atoi(env));
     const uint64_t lanes = static_cast<uint64_t>(blocks) * block;
-    // One launch covers up to 256 histories per lane: the drain at the end of a
-    // launch is paid rarely, and progress/cancel still act every few tens of ms.
-    uint64_t chunk = std::max<uint64_t>(lanes * 256, 1ull << 20);
-    if (const char* env = std::getenv("DXG_CHUNK"))
-        chunk = std::max<uint64_t>(1, std::strtoull(env, nullptr, 10));
-    // equal launches: no short last launch that is mostly drain
-    if (total > chunk) {
-        const uint64_t n_launch = (total + chunk - 1) / chunk;
-        chunk = (total + n_launch - 1) / n_launch;
-    }
 
     if (progress) {
         progress->stop.store(false);
@@ -782,13 +847,10 @@ int runChunks(dxg_world* w, DevParams& p, uint64_t total, dxg_progress* progress
     }
     dxg_counters acc {};
     int status = DXG_OK;
-    for (uint64_t h0 = 0; h0 < total; h0 += chunk) {
-        if (progress && progress->stop.load()) {
-            status = fail(DXG_ERR_CANCELLED, "transport cancelled by the progress stop flag");
-            break;
-        }
-        p.h_begin = h0;
-        p.h_end = std::min(total, h0 + chunk);
+    // one launch over beam histories [h_begin, h_end); ms receives its time
+    auto launchRange = [&](uint64_t h_begin, uint64_t h_end, float& ms) -> int {
+        p.h_begin = h_begin;
+        p.h_end = h_end;
         dxg::launchReset(p.pool, p.counters, w->stream);
         const auto t0 = std::chrono::steady_clock::now();
         if (events)
@@ -800,7 +862,7 @@ int runChunks(dxg_world* w, DevParams& p, uint64_t total, dxg_progress* progress
         dxg::DevCounters c {};
         HIP_TRY(hipMemcpyAsync(&c, p.counters, sizeof c, hipMemcpyDeviceToHost, w->stream));
         HIP_TRY(hipStreamSynchronize(w->stream));
-        float ms = 0;
+        ms = 0;
         if (events)
             HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
         else   // no events without allocation: the host's clock around launch and wait
@@ -818,20 +880,95 @@ int runChunks(dxg_world* w, DevParams& p, uint64_t total, dxg_progress* progress
         acc.cell_crossings += c.crossings;
         for (int k = 0; k < 3; ++k)
             acc.phase_cycles[k] += c.phase_cycles[k];
-        w->histories += p.h_end - p.h_begin;
+        w->histories += h_end - h_begin;
         if (progress)
-            progress->n.store(p.h_end);
+            progress->n.store(h_end);
+        return DXG_OK;
+    };
+
+    // ---- autotuned launch settings
+    uint64_t h0 = 0;
+    int autotuned = 0;
+    if (events && autotuneEnabled()) {
+        const TuneKey key { w, w->generation, p.level, p.n_ex, p.lds_lut, p.cell_ratio != nullptr, p.ppe != 0 };
+        bool found = false;
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            if (auto it = g_tuned.find(key); it != g_tuned.end()) {
+                applyTuned(p, it->second);
+                found = true;
+            }
+        }
+        autotuned = found ? 1 : 0;
+        constexpr int n_cand = static_cast<int>(sizeof kTuneCandidates / sizeof kTuneCandidates[0]);
+        const uint64_t sub = std::max<uint64_t>(lanes * 16, 1ull << 20);
+        if (!found && total >= sub * (kTuneRounds * n_cand + 2)) {
+            float best[n_cand];
+            for (int c = 0; c < n_cand; ++c)
+                best[c] = 3.0e38f;
+            for (int r = 0; r < kTuneRounds && status == DXG_OK; ++r)
+                for (int c = 0; c < n_cand; ++c) {
+                    if (progress && progress->stop.load()) {
+                        status = fail(DXG_ERR_CANCELLED, "transport cancelled by the progress stop flag");
+                        break;
+                    }
+                    applyTuned(p, kTuneCandidates[c]);
+                    float ms = 0;
+                    if (int rc = launchRange(h0, h0 + sub, ms); rc != DXG_OK)
+                        return rc;
+                    best[c] = std::min(best[c], ms);
+                    h0 += sub;
+                }
+            int pick = 0;
+            for (int c = 1; c < n_cand; ++c)
+                if (best[c] < best[pick])
+                    pick = c;
+            applyTuned(p, kTuneCandidates[pick]);
+            if (status == DXG_OK) {
+                std::lock_guard<std::mutex> lock(g_tune_mutex);
+                g_tuned[key] = kTuneCandidates[pick];
+                autotuned = 1;
+            }
+        }
+    }
+
+    // ---- the rest of the run in equal launches of up to 256 histories per
+    // lane: the drain at the end of a launch is paid rarely, and progress /
+    // cancel still act every few tens of ms
+    const uint64_t rest = total - h0;
+    uint64_t chunk = std::max<uint64_t>(lanes * 256, 1ull << 20);
+    if (const char* env = std::getenv("DXG_CHUNK"))
+        chunk = std::max<uint64_t>(1, std::strtoull(env, nullptr, 10));
+    if (rest > chunk) {   // equal launches: no short last launch that is mostly drain
+        const uint64_t n_launch = (rest + chunk - 1) / chunk;
+        chunk = (rest + n_launch - 1) / n_launch;
+    }
+    for (; status == DXG_OK && h0 < total; h0 += chunk) {
+        if (progress && progress->stop.load()) {
+            status = fail(DXG_ERR_CANCELLED, "transport cancelled by the progress stop flag");
+            break;
+        }
+        float ms = 0;
+        if (int rc = launchRange(h0, std::min(total, h0 + chunk), ms); rc != DXG_OK)
+            return rc;
     }
     if (events) {
         (void)hipEventDestroy(ev0);
         (void)hipEventDestroy(ev1);
     }
+    dxg::launchTallyFold(p.tally, p.tally_stride, p.tally_replicas, w->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    acc.refill_threshold = p.refill_threshold;
+    acc.interact_threshold = p.interact_threshold;
+    acc.outer_threshold = p.outer_threshold;
+    acc.batch = p.batch;
+    acc.autotuned = autotuned;
     w->counters = acc;
     if (progress && status == DXG_OK)
         progress->setMessage("Simulation finished");
     return status;
 }
-
 
 }  // namespace dxg
 

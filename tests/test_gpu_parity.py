@@ -361,3 +361,72 @@ def test_results_do_not_depend_on_the_phase_schedule():
         np.testing.assert_array_equal(n, n0)
         np.testing.assert_allclose(e, e0, rtol=1e-9, atol=1e-12 * e0.max())
         np.testing.assert_allclose(s2, s0, rtol=1e-9, atol=1e-12 * s0.max())
+
+
+def test_autotuned_launch_settings_leave_results_unchanged():
+    """The first run of a beam layout on a world build times each candidate
+    launch setting on short launches and keeps the fastest; the events of the
+    run are those of a run with the default settings (DXG_AUTOTUNE=0), and the
+    next run reuses the kept settings without timing again."""
+    ph = small_body()
+    beam = dx.CTSpiralBeam((0, 0, -12), (0, 0, 12), {13: 6.0})
+    beam.setCollimation(4.0)
+    beam.setStepAngleDeg(10.0)
+    beam.setNumberOfParticlesPerExposure(120_000_000 // beam.numberOfExposures())
+    counts = {}
+    for mode in ("tuned", "default"):
+        world, grid = make_world(ph, 1)
+        if mode == "default":
+            os.environ["DXG_AUTOTUNE"] = "0"
+        try:
+            assert dx.Transport(seed=5)(world, beam, None, False)
+            c = grid.counters()
+            print(f"[autotune] {mode}: launches {c['kernel_launches']}, refill {c['refill_threshold']}, interact "
+                  f"{c['interact_threshold']}, outer {c['outer_threshold']}, batch {c['batch']}, tuned {c['autotuned']}",
+                  flush=True)
+            counts[mode] = {k: c[k] for k in ("histories", "steps", "real_events", "deposits", "uncollided")}
+            if mode == "tuned":
+                assert c["autotuned"] == 1 and c["kernel_launches"] > 12
+                first = (c["refill_threshold"], c["interact_threshold"], c["outer_threshold"], c["batch"])
+                assert dx.Transport(seed=6)(world, beam, None, False)
+                c2 = grid.counters()
+                assert c2["autotuned"] == 1 and c2["kernel_launches"] < c["kernel_launches"]
+                assert (c2["refill_threshold"], c2["interact_threshold"], c2["outer_threshold"], c2["batch"]) == first
+            else:
+                assert c["autotuned"] == 0 and (c["refill_threshold"], c["interact_threshold"]) == (16, 24)
+        finally:
+            os.environ.pop("DXG_AUTOTUNE", None)
+    assert counts["tuned"] == counts["default"]
+
+
+def test_air_fill_medium_effect_on_the_pencil_cube():
+    """The world here ends at the grid (vacuum beyond); DXMClib's World fills
+    the space around its items with air (its padding is not in the reference
+    tree).  Its effect on the 64^3 water pencil-beam case: the same cube inside
+    10 cm of air on every side (air voxels, 0.001205 g/cm3; the source inside
+    the air at z = -10 cm) against the bare cube: every voxel above 10 % of the
+    maximum agrees within 3 SE and the total dose within 0.5 %."""
+    n, pad, h = 64, 100, 0.1
+    N = n + 2 * pad
+    dens = np.full((N, N, N), 0.001205)
+    mat = np.zeros((N, N, N), np.uint8)
+    sl = slice(pad, pad + n)
+    dens[sl, sl, sl] = 1.0
+    mat[sl, sl, sl] = 1
+    padded = phantoms.Phantom((N, N, N), (h, h, h), dens.ravel(), mat.ravel(), [phantoms.AIR, phantoms.WATER],
+                              ["air", "water"])
+    bare = phantoms.water_cube(n, h)
+    beam = dx.PencilBeam((0.05, 0.05, -10.0), (0, 0, 1), 60.0)
+    beam.setNumberOfParticlesPerExposure(20_000_000)
+    res = {}
+    for name, ph in (("bare", bare), ("air", padded)):
+        world, grid = make_world(ph, 1)
+        assert dx.Transport(seed=71)(world, beam, None, False)
+        e, e2, _ = grid.energyScored()
+        if name == "air":
+            e = e.reshape(N, N, N)[sl, sl, sl].ravel()
+            e2 = e2.reshape(N, N, N)[sl, sl, sl].ravel()
+        res[name] = (e, e2, grid.histories())
+    v = compare(res["air"][:2], res["bare"][:2], res["air"][2], res["bare"][2])
+    print(f"[fill medium] cube in 10 cm of air vs bare cube: {v}", flush=True)
+    assert v.ok, str(v)
