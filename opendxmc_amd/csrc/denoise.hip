@@ -152,8 +152,12 @@ __device__ inline u32x4 dpp4(u32x4 old, u32x4 src)
 #define DXG_CONV_WAVES 4
 #endif
 constexpr int kConvWaves = DXG_CONV_WAVES;
+// Input rows a wavefront loads ahead of the row its MFMAs use: 1..3 (4 no
+// longer fits two wavefronts per SIMD), chosen per process (DXG_CONV_AHEAD;
+// default kConvAheadDefault).
+constexpr int kConvAheadDefault = 2;
 
-template <bool SHIFT>
+template <bool SHIFT, int kConvAhead>
 __global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
 {
     __shared__ u32x4 wl[kTaps * 32 * 4];   // 55,296 B
@@ -203,34 +207,40 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
             // neighbouring tile's registers; only tile 0 at dx = -1 and tile 3
             // at dx = +1 take their voxel from the next lane of the same 16-lane
             // row (one DPP move per register), with the halo voxel x0 - 1 /
-            // x0 + 64 in the lane that falls off the row.  The next row's loads
-            // are issued before this row's 24 MFMAs (two buffers, alternating).
+            // x0 + 64 in the lane that falls off the row.
+            // Rows are loaded kConvAhead rows ahead of their MFMAs (a ring of
+            // kConvAhead + 1 buffers).  Every row issues the same six loads
+            // (four operand chunks and the two halo voxels) whatever the
+            // position: rows outside the volume and halo voxels beyond its x
+            // ends read a clamped in-range address and are dropped afterwards
+            // (wave-uniform tests).  The fixed load count per row lets the wait
+            // before a row's MFMAs cover that row only, with the later rows'
+            // loads still in flight.
             const u32x4 zero = { 0, 0, 0, 0 };
-            u32x4 buf[2][4], lbuf[2], rbuf[2];
-            bool have[2] = { false, false };
+            constexpr int kRing = kConvAhead + 1;
+            u32x4 buf[kRing][4], lbuf[kRing], rbuf[kRing];
+            bool have[kRing];
+            const int xl = max(x0 - 1, 0), xr = min(x0 + 64, a.nx - 1);
             auto loadRow = [&](int q, int b) {
                 const int zz = z + q / 3 - 1, yy = y + q % 3 - 1;
                 have[b] = static_cast<unsigned>(zz) < static_cast<unsigned>(a.nz) && static_cast<unsigned>(yy) < static_cast<unsigned>(a.ny);
-                if (!have[b])
-                    return;
-                const size_t rowb = (static_cast<size_t>(n * a.nz + zz) * a.ny + yy) * a.nx;
+                const int zc = min(max(zz, 0), a.nz - 1), yc = min(max(yy, 0), a.ny - 1);
+                const size_t rowb = (static_cast<size_t>(n * a.nz + zc) * a.ny + yc) * a.nx;
                 const uint16_t* base = a.x + (rowb + x0 + 4 * lr) * kC + 8 * lq;
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                     buf[b][j] = load16(base + j * kC);
-                lbuf[b] = zero;
-                rbuf[b] = zero;
-                if (lr == 0 && x0 > 0)
-                    lbuf[b] = load16(a.x + (rowb + x0 - 1) * kC + 8 * lq);
-                if (lr == 15 && x0 + 64 < a.nx)
-                    rbuf[b] = load16(a.x + (rowb + x0 + 64) * kC + 8 * lq);
+                lbuf[b] = load16(a.x + (rowb + xl) * kC + 8 * lq);
+                rbuf[b] = load16(a.x + (rowb + xr) * kC + 8 * lq);
             };
-            loadRow(0, 0);
+#pragma unroll
+            for (int q = 0; q < kConvAhead; ++q)
+                loadRow(q, q);
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
-                const int b = q & 1;
-                if (q < 8)
-                    loadRow(q + 1, b ^ 1);
+                const int b = q % kRing;
+                if (q + kConvAhead < 9)
+                    loadRow(q + kConvAhead, (q + kConvAhead) % kRing);
                 if (!have[b])
                     continue;
                 const u32x4(&c)[4] = buf[b];
@@ -242,9 +252,9 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
                         if (dx == 0)
                             bop[j] = asOperand(c[j]);
                         else if (dx < 0)
-                            bop[j] = asOperand(j > 0 ? c[j - 1] : dpp4<kRowShr1>(lbuf[b], c[3]));   // lane n <- lane n-1
+                            bop[j] = asOperand(j > 0 ? c[j - 1] : dpp4<kRowShr1>(x0 > 0 ? lbuf[b] : zero, c[3]));   // lane n <- lane n-1
                         else
-                            bop[j] = asOperand(j < 3 ? c[j + 1] : dpp4<kRowShl1>(rbuf[b], c[0]));   // lane n <- lane n+1
+                            bop[j] = asOperand(j < 3 ? c[j + 1] : dpp4<kRowShl1>(x0 + 64 < a.nx ? rbuf[b] : zero, c[0]));   // lane n <- lane n+1
                     }
                     const int tap = q * 3 + dx + 1;
                     bf16x8 aop[2];
@@ -716,6 +726,16 @@ int roundUp8(int v) { return (v + 7) & ~7; }
 
 // Activation operands for the x-neighbour taps: moved across lanes (default)
 // or loaded again per tap (DXG_CONV_SHIFT=0).
+int convAhead()
+{
+    static const int ahead = [] {
+        const char* e = std::getenv("DXG_CONV_AHEAD");
+        const int v = e ? std::atoi(e) : kConvAheadDefault;
+        return v >= 1 && v <= 3 ? v : kConvAheadDefault;
+    }();
+    return ahead;
+}
+
 bool convShift()
 {
     static const bool on = [] {
@@ -756,10 +776,20 @@ hipError_t launchConv(ConvArgs a, const Dims& d, hipStream_t s)
     const int wg_tiles = (a.wave_tiles + kConvWaves - 1) / kConvWaves;
     const int grid = roundUp8(std::min(wg_tiles, 2 * computeUnits()));
     a.chunk = (wg_tiles + grid - 1) / grid;
-    if (convShift())
-        hipLaunchKernelGGL(conv3d_kernel<true>, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
-    else
-        hipLaunchKernelGGL(conv3d_kernel<false>, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
+    if (!convShift()) {
+        hipLaunchKernelGGL((conv3d_kernel<false, 1>), dim3(grid), dim3(64 * kConvWaves), 0, s, a);
+        return hipGetLastError();
+    }
+    switch (convAhead()) {
+    case 1:
+        hipLaunchKernelGGL((conv3d_kernel<true, 1>), dim3(grid), dim3(64 * kConvWaves), 0, s, a);
+        break;
+    case 2:
+        hipLaunchKernelGGL((conv3d_kernel<true, 2>), dim3(grid), dim3(64 * kConvWaves), 0, s, a);
+        break;
+    default:
+        hipLaunchKernelGGL((conv3d_kernel<true, 3>), dim3(grid), dim3(64 * kConvWaves), 0, s, a);
+    }
     return hipGetLastError();
 }
 
