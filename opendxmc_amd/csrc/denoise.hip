@@ -152,12 +152,13 @@ __device__ inline u32x4 dpp4(u32x4 old, u32x4 src)
 #define DXG_CONV_WAVES 4
 #endif
 constexpr int kConvWaves = DXG_CONV_WAVES;
-// Input rows a wavefront loads ahead of the row its MFMAs use: 1..3 (4 no
-// longer fits two wavefronts per SIMD), chosen per process (DXG_CONV_AHEAD;
-// default kConvAheadDefault).
-constexpr int kConvAheadDefault = 2;
+// One wavefront tile: 64 voxels of one x row.
+struct Tile {
+    int x0, y, z, n;
+    bool valid;
+};
 
-template <bool SHIFT, int kConvAhead>
+template <bool SHIFT>
 __global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
 {
     __shared__ u32x4 wl[kTaps * 32 * 4];   // 55,296 B
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
     }
     __syncthreads();
 
-    const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lq = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, lr = lane & 15, lq = lane >> 4;
     float bias[2][4];
     for (int m = 0; m < 2; ++m)
         for (int i = 0; i < 4; ++i)
@@ -182,65 +183,95 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
     const int begin = logical * a.chunk;
     const int end = min(begin + a.chunk, wg_tiles);
     const int nxs = a.nx >> 6;
-
-    for (int wt = begin; wt < end; ++wt) {
+    auto tileAt = [&](int wt) {
+        Tile tc { 0, 0, 0, 0, false };
         const int k = wt * kConvWaves + wave;
-        if (k >= a.wave_tiles)
-            break;
-        const int t = a.tiles ? a.tiles[k] : k;
-        int r = t / nxs;
-        const int x0 = (t - r * nxs) * 64;
-        const int y = r % a.ny;
-        r /= a.ny;
-        const int z = r % a.nz;
-        const int n = r / a.nz;
+        if (wt < end && k < a.wave_tiles) {
+            const int t = a.tiles ? a.tiles[k] : k;
+            int r = t / nxs;
+            tc.x0 = (t - r * nxs) * 64;
+            tc.y = r % a.ny;
+            r /= a.ny;
+            tc.z = r % a.nz;
+            tc.n = r / a.nz;
+            tc.valid = true;
+        }
+        return tc;
+    };
+
+    // Input rows (SHIFT): the nine (dz, dy) rows of a tile in turn, each
+    // loaded once.  Tile j, lane n holds voxel x0 + 4n + j (the four tiles
+    // interleave), so the x-neighbour operands of tiles 1-3 / 0-2 are simply
+    // the neighbouring tile's registers; only tile 0 at dx = -1 and tile 3 at
+    // dx = +1 take their voxel from the next lane of the same 16-lane row (one
+    // DPP move per register), with the halo voxel x0 - 1 / x0 + 64 in the lane
+    // that falls off the row.
+    // Rows stream through a ring of three buffers two rows ahead of the
+    // MFMAs, across tiles: the next tile's first two rows load during this
+    // tile's last two, so a tile does not start by waiting for its first row.
+    // Nine rows per tile keep row q of every tile in buffer q % 3.  Every row
+    // issues the same six loads (four operand chunks and the two halo voxels)
+    // whatever the position: rows outside the volume, halo voxels beyond its x
+    // ends and rows of a missing next tile read a clamped in-range address and
+    // are dropped afterwards (wave-uniform tests).  The fixed count lets the
+    // wait before a row's MFMAs cover that row only.
+    constexpr int kRing = 3;
+    const u32x4 zero = { 0, 0, 0, 0 };
+    u32x4 buf[kRing][4], lbuf[kRing], rbuf[kRing];
+    bool have[kRing] = { false, false, false };
+    auto loadRow = [&](const Tile& tc, int q, int b) {
+        const int zz = tc.z + q / 3 - 1, yy = tc.y + q % 3 - 1;
+        have[b] = tc.valid && static_cast<unsigned>(zz) < static_cast<unsigned>(a.nz) && static_cast<unsigned>(yy) < static_cast<unsigned>(a.ny);
+        const int zc = min(max(zz, 0), a.nz - 1), yc = min(max(yy, 0), a.ny - 1);
+        const size_t rowb = (static_cast<size_t>(tc.n * a.nz + zc) * a.ny + yc) * a.nx;
+        const uint16_t* base = a.x + (rowb + tc.x0 + 4 * lr) * kC + 8 * lq;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            buf[b][j] = load16(base + j * kC);
+        lbuf[b] = load16(a.x + (rowb + max(tc.x0 - 1, 0)) * kC + 8 * lq);
+        rbuf[b] = load16(a.x + (rowb + min(tc.x0 + 64, a.nx - 1)) * kC + 8 * lq);
+    };
+
+    Tile cur = tileAt(begin);
+    if (SHIFT && cur.valid) {
+        loadRow(cur, 0, 0);
+        loadRow(cur, 1, 1);
+    }
+    for (int wt = begin; cur.valid; ++wt) {
+        const Tile nxt = tileAt(wt + 1);
+        const int x0 = cur.x0, y = cur.y, z = cur.z, n = cur.n;
 
         f32x4 acc[2][4];
         for (int m = 0; m < 2; ++m)
             for (int j = 0; j < 4; ++j)
                 acc[m][j] = f32x4 { 0, 0, 0, 0 };
 
+        // The epilogue's residual (or gate) values, loaded before the MFMAs
+        // so that their latency hides behind them; loaded after them, each
+        // store's possible alias with the next load serialised the waits.
+        // The residual may alias the output: each tile reads and writes only
+        // its own voxels.
+        const size_t orow = (static_cast<size_t>(n * a.nz + z) * a.ny + y) * a.nx;
+        u32x2 side[4][2];
+        const uint16_t* side_src = a.res ? a.res : a.gate;
+        if (side_src) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const size_t v = orow + x0 + (SHIFT ? 4 * lr + j : 16 * j + lr);
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+                    side[j][m] = *reinterpret_cast<const u32x2*>(side_src + v * kC + 16 * m + 4 * lq);
+            }
+        }
+
         if (SHIFT) {
-            // The nine (dz, dy) input rows in turn, each loaded once.  Tile j,
-            // lane n holds voxel x0 + 4n + j (the four tiles interleave), so the
-            // x-neighbour operands of tiles 1-3 / 0-2 are simply the
-            // neighbouring tile's registers; only tile 0 at dx = -1 and tile 3
-            // at dx = +1 take their voxel from the next lane of the same 16-lane
-            // row (one DPP move per register), with the halo voxel x0 - 1 /
-            // x0 + 64 in the lane that falls off the row.
-            // Rows are loaded kConvAhead rows ahead of their MFMAs (a ring of
-            // kConvAhead + 1 buffers).  Every row issues the same six loads
-            // (four operand chunks and the two halo voxels) whatever the
-            // position: rows outside the volume and halo voxels beyond its x
-            // ends read a clamped in-range address and are dropped afterwards
-            // (wave-uniform tests).  The fixed load count per row lets the wait
-            // before a row's MFMAs cover that row only, with the later rows'
-            // loads still in flight.
-            const u32x4 zero = { 0, 0, 0, 0 };
-            constexpr int kRing = kConvAhead + 1;
-            u32x4 buf[kRing][4], lbuf[kRing], rbuf[kRing];
-            bool have[kRing];
-            const int xl = max(x0 - 1, 0), xr = min(x0 + 64, a.nx - 1);
-            auto loadRow = [&](int q, int b) {
-                const int zz = z + q / 3 - 1, yy = y + q % 3 - 1;
-                have[b] = static_cast<unsigned>(zz) < static_cast<unsigned>(a.nz) && static_cast<unsigned>(yy) < static_cast<unsigned>(a.ny);
-                const int zc = min(max(zz, 0), a.nz - 1), yc = min(max(yy, 0), a.ny - 1);
-                const size_t rowb = (static_cast<size_t>(n * a.nz + zc) * a.ny + yc) * a.nx;
-                const uint16_t* base = a.x + (rowb + x0 + 4 * lr) * kC + 8 * lq;
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    buf[b][j] = load16(base + j * kC);
-                lbuf[b] = load16(a.x + (rowb + xl) * kC + 8 * lq);
-                rbuf[b] = load16(a.x + (rowb + xr) * kC + 8 * lq);
-            };
-#pragma unroll
-            for (int q = 0; q < kConvAhead; ++q)
-                loadRow(q, q);
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
                 const int b = q % kRing;
-                if (q + kConvAhead < 9)
-                    loadRow(q + kConvAhead, (q + kConvAhead) % kRing);
+                if (q + 2 < 9)
+                    loadRow(cur, q + 2, (q + 2) % kRing);
+                else
+                    loadRow(nxt, q + 2 - 9, (q + 2) % kRing);
                 if (!have[b])
                     continue;
                 const u32x4(&c)[4] = buf[b];
@@ -304,7 +335,6 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
             }
         }
 
-        const size_t orow = (static_cast<size_t>(n * a.nz + z) * a.ny + y) * a.nx;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const size_t v = orow + x0 + (SHIFT ? 4 * lr + j : 16 * j + lr);
@@ -315,7 +345,7 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
                 for (int i = 0; i < 4; ++i)
                     o[i] = acc[m][j][i] + bias[m][i];
                 if (a.res) {
-                    const u32x2 rv = *reinterpret_cast<const u32x2*>(a.res + v * kC + co0);
+                    const u32x2 rv = side[j][m];
                     o[0] += fromBf16(rv[0] & 0xffffu);
                     o[1] += fromBf16(rv[0] >> 16);
                     o[2] += fromBf16(rv[1] & 0xffffu);
@@ -325,7 +355,7 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
                     for (int i = 0; i < 4; ++i)
                         o[i] = fmaxf(o[i], 0.0f);
                 if (a.gate) {
-                    const u32x2 gv = *reinterpret_cast<const u32x2*>(a.gate + v * kC + co0);
+                    const u32x2 gv = a.res ? *reinterpret_cast<const u32x2*>(a.gate + v * kC + co0) : side[j][m];
                     const uint32_t h[4] = { gv[0] & 0xffffu, gv[0] >> 16, gv[1] & 0xffffu, gv[1] >> 16 };
                     for (int i = 0; i < 4; ++i)
                         if (h[i] == 0 || (h[i] & 0x8000u))
@@ -341,8 +371,10 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
                 }
             }
         }
+        cur = nxt;
     }
 }
+
 
 struct WgradArgs {
     const uint16_t* x;       // layer input
@@ -726,16 +758,6 @@ int roundUp8(int v) { return (v + 7) & ~7; }
 
 // Activation operands for the x-neighbour taps: moved across lanes (default)
 // or loaded again per tap (DXG_CONV_SHIFT=0).
-int convAhead()
-{
-    static const int ahead = [] {
-        const char* e = std::getenv("DXG_CONV_AHEAD");
-        const int v = e ? std::atoi(e) : kConvAheadDefault;
-        return v >= 1 && v <= 3 ? v : kConvAheadDefault;
-    }();
-    return ahead;
-}
-
 bool convShift()
 {
     static const bool on = [] {
@@ -776,20 +798,10 @@ hipError_t launchConv(ConvArgs a, const Dims& d, hipStream_t s)
     const int wg_tiles = (a.wave_tiles + kConvWaves - 1) / kConvWaves;
     const int grid = roundUp8(std::min(wg_tiles, 2 * computeUnits()));
     a.chunk = (wg_tiles + grid - 1) / grid;
-    if (!convShift()) {
-        hipLaunchKernelGGL((conv3d_kernel<false, 1>), dim3(grid), dim3(64 * kConvWaves), 0, s, a);
-        return hipGetLastError();
-    }
-    switch (convAhead()) {
-    case 1:
-        hipLaunchKernelGGL((conv3d_kernel<true, 1>), dim3(grid), dim3(64 * kConvWaves), 0, s, a);
-        break;
-    case 2:
-        hipLaunchKernelGGL((conv3d_kernel<true, 2>), dim3(grid), dim3(64 * kConvWaves), 0, s, a);
-        break;
-    default:
-        hipLaunchKernelGGL((conv3d_kernel<true, 3>), dim3(grid), dim3(64 * kConvWaves), 0, s, a);
-    }
+    if (convShift())
+        hipLaunchKernelGGL(conv3d_kernel<true>, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
+    else
+        hipLaunchKernelGGL(conv3d_kernel<false>, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
     return hipGetLastError();
 }
 
