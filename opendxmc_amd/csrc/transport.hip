@@ -552,6 +552,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 const float c_inc = __fmaf_rn(et, b.y - a.y, a.y);
                 const float c_tot = __fmaf_rn(et, b.z - a.z, a.z);
                 const float r = rngU(rng) * c_tot;
+                float scatter_cost = 2.0f;   // cos of the scattering angle; 2: the photon does not scatter
                 if (r < c_pe) {
                     float e_out = 0.0f;   // carried away by a K fluorescence photon (level 2)
                     if (LEVEL == 2) {
@@ -657,7 +658,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     deps = 1;
                     e_scored = de;
                     E = enew;
-                    rotate(dx, dy, dz, cost, kTwoPi * rngU(rng));
+                    scatter_cost = cost;
                     if (E < kCutoff) {
                         const float rest = w * E;
                         e2 = rest;
@@ -693,8 +694,13 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                                 break;
                         }
                     }
-                    rotate(dx, dy, dz, cost, kTwoPi * rngU(rng));
+                    scatter_cost = cost;
                 }
+                // Compton and Rayleigh lanes turn together (the azimuth is the
+                // last draw of both processes, so the random streams are
+                // consumed in the same order as with a turn in each branch)
+                if (scatter_cost <= 1.0f)
+                    rotate(dx, dy, dz, scatter_cost, kTwoPi * rngU(rng));
             }
             tallyFlush(P, tally, deps, vi, e1, e2);
             if (deps > 0)
