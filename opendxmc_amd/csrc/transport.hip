@@ -1,10 +1,13 @@
 // HIP/CDNA4 kernels of the voxel photon-transport engine.
 //
 // transport_kernel is a persistent history-pool kernel.  Every lane carries
-// one photon in registers (FP32 state) and is in one of four states:
+// one photon in registers (FP32 state) and is in one of these states:
 //   DEAD      needs a new history,
 //   TRACK     Woodcock tracking,
 //   INTERACT  parked at a real collision, waiting for its interaction,
+//   COMPTON / RAYLEIGH  parked with that process chosen, its rejection
+//             sampling not yet accepted,
+//   OUTER     waiting for an exact flight through the outer medium,
 //   DONE      the history pool is exhausted.
 // Each loop iteration a wavefront runs exactly ONE phase, chosen from ballots
 // of the lane states, with the lanes of the other states masked off:
@@ -12,8 +15,10 @@
 //              they take consecutive ids from the wavefront's private batch of
 //              pool histories (one global atomic per `batch` histories) and
 //              sample their source particle together;
-//   * interact once interact_threshold lanes are parked (or no lane tracks):
-//              photoelectric / Compton / Rayleigh for all parked lanes at once;
+//   * interact once interact_threshold lanes are parked (or no lane tracks
+//              and fewer than outer_threshold wait for outer flights):
+//              photoelectric / Compton / Rayleigh for all parked lanes at
+//              once, one rejection-sampling trial per lane and phase;
 //   * outer    once outer_threshold lanes wait outside the box of the
 //              non-outer voxels (or no lane tracks): exact flights through the
 //              uniform outer medium, to a collision, the box or the grid's edge;
@@ -72,7 +77,9 @@ constexpr int kIaShellsd = 5;    // impulse-approximation shells per material
 constexpr int kIaInvNd = 2048;   // unit Compton profile inverse-CDF nodes
 constexpr int kFlMaxd = 12;      // fluorescing elements per material
 
-enum LaneState : int { kDead = 0, kTrack = 1, kInteract = 2, kDone = 3, kOuter = 4 };
+// kCompton / kRayleigh: parked with that process chosen, its rejection
+// sampling still to accept (parked like kInteract)
+enum LaneState : int { kDead = 0, kTrack = 1, kInteract = 2, kDone = 3, kOuter = 4, kCompton = 5, kRayleigh = 6 };
 
 // The hardware's approximations (v_log_f32, v_exp_f32, v_rcp_f32, v_sqrt_f32;
 // about 1 ulp) for sampling.  The library calls expand to ~10-20 instructions
@@ -478,7 +485,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
         t_iter = __builtin_amdgcn_s_memtime();
 #endif
         const unsigned long long m_dead = __ballot(state == kDead);
-        const unsigned long long m_int = __ballot(state == kInteract);
+        const unsigned long long m_int = __ballot(state == kInteract || state >= kCompton);
         const unsigned long long m_track = __ballot(state == kTrack);
         const unsigned long long m_outer = __ballot(state == kOuter);
         const int n_dead = __popcll(m_dead), n_int = __popcll(m_int), n_track = __popcll(m_track), n_outer = __popcll(m_outer);
@@ -537,23 +544,40 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
         }
 
         // -------------------------------------------------------- interact
-        if (n_int > 0 && (n_int >= P.interact_threshold || n_track == 0)) {
+        // One attempt per parked lane and phase: a lane whose Compton or
+        // Rayleigh rejection sampling rejects stays parked with its process
+        // chosen (kCompton / kRayleigh) and tries again in the next phase, so
+        // the phase runs each lane's trial once instead of looping until the
+        // wavefront's slowest lane accepts.  The trials draw the same numbers
+        // in the same order as a loop would.
+        // (with no lane tracking, waiting outer flights go first once enough
+        // wait: lanes parked for another rejection trial would otherwise run
+        // the phase for a few lanes again and again)
+        if (n_int > 0 && (n_int >= P.interact_threshold || (n_track == 0 && n_outer < P.outer_threshold))) {
             int deps = 0;
             float e1 = 0.0f, e2 = 0.0f;   // this phase's deposits at vi (tallyFlush)
             double e_scored = 0.0;
-            if (state == kInteract) {
-                collided = true;
-                state = outer ? kOuter : kTrack;   // the photon stays where it is
-                tau = -1.0f;   // the photon stays in its macro cell; its next flight is drawn anew
+            const bool fresh = state == kInteract;
+            n_real += __popcll(__ballot(fresh));
+            if (fresh || state >= kCompton) {
                 const int m = mat;
-                const float4 a = P.mu_parts[m * kLutNd + ej];
-                const float4 b = P.mu_parts[m * kLutNd + ej + 1];
-                const float c_pe = __fmaf_rn(et, b.x - a.x, a.x);
-                const float c_inc = __fmaf_rn(et, b.y - a.y, a.y);
-                const float c_tot = __fmaf_rn(et, b.z - a.z, a.z);
-                const float r = rngU(rng) * c_tot;
+                int proc;   // 0 photoelectric, 1 Compton, 2 Rayleigh
+                if (fresh) {
+                    collided = true;
+                    tau = -1.0f;   // the photon stays in its macro cell; its next flight is drawn anew
+                    const float4 a = P.mu_parts[m * kLutNd + ej];
+                    const float4 b = P.mu_parts[m * kLutNd + ej + 1];
+                    const float c_pe = __fmaf_rn(et, b.x - a.x, a.x);
+                    const float c_inc = __fmaf_rn(et, b.y - a.y, a.y);
+                    const float c_tot = __fmaf_rn(et, b.z - a.z, a.z);
+                    const float r = rngU(rng) * c_tot;
+                    proc = r < c_pe ? 0 : (r < c_inc ? 1 : 2);
+                } else {
+                    proc = state == kCompton ? 1 : 2;
+                }
+                state = outer ? kOuter : kTrack;   // the photon stays where it is
                 float scatter_cost = 2.0f;   // cos of the scattering angle; 2: the photon does not scatter
-                if (r < c_pe) {
+                if (proc == 0) {
                     float e_out = 0.0f;   // carried away by a K fluorescence photon (level 2)
                     if (LEVEL == 2) {
                         const float* cum = P.fl_cum + (static_cast<size_t>(m) * kLutNd + ej) * kFlMaxd;
@@ -590,31 +614,30 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     } else {
                         state = kDead;
                     }
-                } else if (r < c_inc) {
-                    // Compton: Klein-Nishina (Butcher & Messel), level 1 rejects on S(x)/Z,
-                    // level 2 adds the impulse approximation: a shell drawn by electron
-                    // fraction, p_z from its Compton profile, the Doppler-shifted E'
+                } else if (proc == 1) {
+                    // Compton, one trial: Klein-Nishina (Butcher & Messel), level 1
+                    // rejects on S(x)/Z, level 2 adds the impulse approximation: a
+                    // shell drawn by electron fraction, p_z from its Compton profile,
+                    // the Doppler-shifted E'
                     const float k = E * kInvMeC2;
                     const float e0 = hwRcp(1.0f + 2.0f * k);
                     const float a1 = hwLog(1.0f + 2.0f * k);
                     const float a2 = 0.5f * (1.0f - e0 * e0);
-                    float cost, enew;
-                    for (;;) {
-                        float eps;
-                        for (;;) {
-                            const float r1 = rngU(rng), r2 = rngU(rng), r3 = rngU(rng);
-                            if (r1 * (a1 + a2) < a1)
-                                eps = hwExp(-a1 * r2);
-                            else
-                                eps = hwSqrt(e0 * e0 + (1.0f - e0 * e0) * r2);
-                            const float t = (1.0f - eps) * hwRcp(k * eps);
-                            const float sin2 = t * (2.0f - t);
-                            const float g = 1.0f - eps * sin2 / (1.0f + eps * eps);
-                            if (r3 > g)
-                                continue;
-                            cost = 1.0f - t;
-                            if (LEVEL == 0)
-                                break;
+                    float cost = 1.0f, enew = E;
+                    const float r1 = rngU(rng), r2 = rngU(rng), r3 = rngU(rng);
+                    float eps;
+                    if (r1 * (a1 + a2) < a1)
+                        eps = hwExp(-a1 * r2);
+                    else
+                        eps = hwSqrt(e0 * e0 + (1.0f - e0 * e0) * r2);
+                    const float t = (1.0f - eps) * hwRcp(k * eps);
+                    const float sin2 = t * (2.0f - t);
+                    const float g = 1.0f - eps * sin2 / (1.0f + eps * eps);
+                    bool ok = !(r3 > g);
+                    if (ok) {
+                        cost = 1.0f - t;
+                        enew = E * eps;
+                        if (LEVEL > 0) {
                             const float x = E * kInvHc * hwSqrt(fmaxf(0.0f, 0.5f * (1.0f - cost)));
                             const float fx = x * (static_cast<float>(kSfNd - 1) / kXMaxf);
                             float sv = 1.0f;
@@ -622,55 +645,58 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                                 const int kk = static_cast<int>(fx);
                                 sv = lerpTab(P.s_table + m * kSfNd, kk, fx - static_cast<float>(kk));
                             }
-                            if (rngU(rng) <= sv)
-                                break;
+                            ok = rngU(rng) <= sv;
                         }
-                        if (LEVEL < 2) {
-                            enew = E * eps;
-                            break;
-                        }
+                    }
+                    if (LEVEL == 2 && ok) {
                         const float us = rngU(rng);
                         const float4* sh = P.ia_shell + m * kIaShellsd;
                         float4 s = sh[0];
                         for (int q = 1; q < kIaShellsd && us > s.x; ++q)
                             s = sh[q];
-                        if (E <= s.y)
-                            continue;   // the shell cannot be ionised: draw again
-                        const float fy = rngU(rng) * static_cast<float>(kIaInvNd - 1);
-                        const int kk = min(static_cast<int>(fy), kIaInvNd - 2);
-                        const float pz = lerpTab(P.ia_inv, kk, fy - static_cast<float>(kk)) * s.z;
-                        // E'/mc^2 from p_z |q| = E E'(1-cos)/mc^2 - (E - E') (in m_e c units):
-                        // b = k (h + p_z sqrt(Q)) / (K^2 - p_z^2), with the discriminant
-                        // written as p_z^2 Q so that FP32 does not cancel it away
-                        const float K = 1.0f + k * (1.0f - cost);
-                        const float p2 = pz * pz;
-                        const float h = K - p2 * cost;
-                        const float den = K * K - p2;
-                        const float Q = (K - cost) * (K - cost) + (1.0f - cost * cost) * (1.0f - p2);
-                        if (!(den > 0.0f))
-                            continue;
-                        enew = k * (h + pz * hwSqrt(Q)) * hwRcp(den) * (1.0f / kInvMeC2);
-                        if (enew > 0.0f && E - enew >= s.y)
-                            break;
+                        ok = E > s.y;   // else the shell cannot be ionised: another trial
+                        if (ok) {
+                            const float fy = rngU(rng) * static_cast<float>(kIaInvNd - 1);
+                            const int kk = min(static_cast<int>(fy), kIaInvNd - 2);
+                            const float pz = lerpTab(P.ia_inv, kk, fy - static_cast<float>(kk)) * s.z;
+                            // E'/mc^2 from p_z |q| = E E'(1-cos)/mc^2 - (E - E') (in m_e c units):
+                            // b = k (h + p_z sqrt(Q)) / (K^2 - p_z^2), with the discriminant
+                            // written as p_z^2 Q so that FP32 does not cancel it away
+                            const float K = 1.0f + k * (1.0f - cost);
+                            const float p2 = pz * pz;
+                            const float h = K - p2 * cost;
+                            const float den = K * K - p2;
+                            const float Q = (K - cost) * (K - cost) + (1.0f - cost * cost) * (1.0f - p2);
+                            ok = den > 0.0f;
+                            if (ok) {
+                                enew = k * (h + pz * hwSqrt(Q)) * hwRcp(den) * (1.0f / kInvMeC2);
+                                ok = enew > 0.0f && E - enew >= s.y;
+                            }
+                        }
                     }
-                    const float de = w * (E - enew);
-                    e1 = de;
-                    deps = 1;
-                    e_scored = de;
-                    E = enew;
-                    scatter_cost = cost;
-                    if (E < kCutoff) {
-                        const float rest = w * E;
-                        e2 = rest;
-                        deps = 2;
-                        e_scored += rest;
-                        state = kDead;
+                    if (!ok) {
+                        state = kCompton;   // parked: the next phase draws the next trial
                     } else {
-                        energyBin<CELLS>(E, s_maj, ej, et, maj);
+                        const float de = w * (E - enew);
+                        e1 = de;
+                        deps = 1;
+                        e_scored = de;
+                        E = enew;
+                        scatter_cost = cost;
+                        if (E < kCutoff) {
+                            const float rest = w * E;
+                            e2 = rest;
+                            deps = 2;
+                            e_scored += rest;
+                            state = kDead;
+                        } else {
+                            energyBin<CELLS>(E, s_maj, ej, et, maj);
+                        }
                     }
                 } else {
-                    // Rayleigh
+                    // Rayleigh, one trial (level 0 always accepts)
                     float cost;
+                    bool ok = true;
                     if (LEVEL == 0) {
                         const float q = 4.0f - 8.0f * rngU(rng);
                         const float aa = -0.5f * q + hwSqrt(0.25f * q * q + 1.0f);
@@ -685,27 +711,25 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                             const int kk = static_cast<int>(fx);
                             amax = lerpTab(P.ray_fwd + m * kSfNd, kk, fx - static_cast<float>(kk));
                         }
-                        for (;;) {
-                            const float fy = rngU(rng) * amax * static_cast<float>(kRayInvNd - 1);
-                            const int kk = min(static_cast<int>(fy), kRayInvNd - 2);
-                            const float u = fminf(umax, lerpTab(P.ray_inv + m * kRayInvNd, kk, fy - static_cast<float>(kk)) * (kXMaxf * kXMaxf));
-                            cost = 1.0f - 2.0f * u / umax;
-                            if (2.0f * rngU(rng) <= 1.0f + cost * cost)
-                                break;
-                        }
+                        const float fy = rngU(rng) * amax * static_cast<float>(kRayInvNd - 1);
+                        const int kk = min(static_cast<int>(fy), kRayInvNd - 2);
+                        const float u = fminf(umax, lerpTab(P.ray_inv + m * kRayInvNd, kk, fy - static_cast<float>(kk)) * (kXMaxf * kXMaxf));
+                        cost = 1.0f - 2.0f * u / umax;
+                        ok = 2.0f * rngU(rng) <= 1.0f + cost * cost;
                     }
-                    scatter_cost = cost;
+                    if (ok)
+                        scatter_cost = cost;
+                    else
+                        state = kRayleigh;   // parked: the next phase draws the next trial
                 }
                 // Compton and Rayleigh lanes turn together (the azimuth is the
-                // last draw of both processes, so the random streams are
-                // consumed in the same order as with a turn in each branch)
+                // last draw of both processes)
                 if (scatter_cost <= 1.0f)
                     rotate(dx, dy, dz, scatter_cost, kTwoPi * rngU(rng));
             }
             tallyFlush(P, tally, deps, vi, e1, e2);
             if (deps > 0)
                 atomicAdd(&e_wave[2], e_scored);
-            n_real += n_int;
             n_dep += __popcll(__ballot(deps > 0)) + __popcll(__ballot(deps > 1));
             DXG_PHASE_END(1);
             continue;
