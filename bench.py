@@ -404,7 +404,9 @@ def main() -> int:
     launch = {"refill_threshold": c["refill_threshold"], "interact_threshold": c["interact_threshold"],
               "outer_threshold": c["outer_threshold"], "batch": c["batch"], "autotuned": bool(c["autotuned"])}
     steps = bytes_alg = kernel_ms = launches = hist = deposits = crossings = 0
-    phase = [0, 0, 0]
+    phase = [0, 0, 0, 0]
+    phase_lanes = [0, 0, 0, 0]
+    phase_iters = [0, 0, 0, 0]
     t0 = time.perf_counter()
     for i in range(args.steps):
         transport.seed = i + 1
@@ -417,6 +419,8 @@ def main() -> int:
         launches += c["kernel_launches"]
         hist += c["histories"]
         phase = [a + b for a, b in zip(phase, c["phase_cycles"])]
+        phase_lanes = [a + b for a, b in zip(phase_lanes, c["phase_lanes"])]
+        phase_iters = [a + b for a, b in zip(phase_iters, c["phase_iters"])]
         print(f"step {i}: {c['histories']} histories, kernel {c['kernel_ms']:.1f} ms", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     bytes_alg = 5.0 * steps + 24.0 * deposits
@@ -464,7 +468,10 @@ def main() -> int:
         },
     }
     if sum(phase) > 0:   # libraries built with DXG_PHASE_CLOCKS
-        result["config"]["phase_cycle_share"] = {k: v / sum(phase) for k, v in zip(("refill", "interact", "step"), phase)}
+        names = ("refill", "interact", "outer", "step")
+        result["config"]["phase_cycle_share"] = {k: v / sum(phase) for k, v in zip(names, phase)}
+        result["config"]["phase_lane_use"] = {k: ln / (64.0 * it) if it else 0.0 for k, ln, it in zip(names, phase_lanes, phase_iters)}
+        result["config"]["phase_iters_per_history"] = {k: it / max(hist, 1) for k, it in zip(names, phase_iters)}
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ph, beam, what=description)
     if not args.no_time_to_2pct and args.workload == "ct-spiral-256":

@@ -107,8 +107,13 @@ typedef struct dxg_counters {
     uint64_t kernel_launches;
     uint64_t uncollided;       /* histories leaving the grid with no real interaction */
     uint64_t cell_crossings;   /* macro-cell boundary crossings (majorant changes, no voxel read) */
-    uint64_t phase_cycles[3];  /* wavefront clock cycles spent refilling, interacting and stepping
-                                  (libraries built with DXG_PHASE_CLOCKS; zero otherwise) */
+    /* per phase (refill, interact, outer-medium flights, step): wavefront
+     * clock cycles, lanes doing the phase's work summed over the phase's
+     * iterations, and iterations (libraries built with DXG_PHASE_CLOCKS;
+     * zero otherwise) */
+    uint64_t phase_cycles[4];
+    uint64_t phase_lanes[4];
+    uint64_t phase_iters[4];
     /* the launch settings the run ended with: dead / parked / outside lanes
      * that start a refill / interaction / outer-flight phase, histories a
      * wavefront claims per pool atomic, and 1 when they were chosen by the

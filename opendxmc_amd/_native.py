@@ -68,14 +68,16 @@ class Counters(C.Structure):
         ("histories", u64), ("steps", u64), ("real_events", u64), ("deposits", u64),
         ("emitted_energy", C.c_double), ("escaped_energy", C.c_double),
         ("deposited_energy", C.c_double), ("kernel_ms", C.c_double), ("kernel_launches", u64),
-        ("uncollided", u64), ("cell_crossings", u64), ("phase_cycles", u64 * 3),
+        ("uncollided", u64), ("cell_crossings", u64), ("phase_cycles", u64 * 4),
+        ("phase_lanes", u64 * 4), ("phase_iters", u64 * 4),
         ("refill_threshold", C.c_int32), ("interact_threshold", C.c_int32), ("outer_threshold", C.c_int32),
         ("autotuned", C.c_int32), ("batch", u64),
     ]
 
     def as_dict(self) -> dict:
         d = {name: getattr(self, name) for name, _ in self._fields_}
-        d["phase_cycles"] = list(self.phase_cycles)
+        for name in ("phase_cycles", "phase_lanes", "phase_iters"):
+            d[name] = list(getattr(self, name))
         return d
 
 

@@ -301,8 +301,11 @@ int dxg_batch_run(dxg_batch* b, double* kernel_ms)
         acc.deposited_energy = c.deposited;
         acc.uncollided = c.uncollided;
         acc.cell_crossings = c.crossings;
-        for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < 4; ++k) {
             acc.phase_cycles[k] = c.phase_cycles[k];
+            acc.phase_lanes[k] = c.phase_lanes[k];
+            acc.phase_iters[k] = c.phase_iters[k];
+        }
         acc.kernel_ms = ms;          // the shared launch
         acc.kernel_launches = 1;
         dxg::addHistories(b->inst[static_cast<size_t>(i)].world, b->inst[static_cast<size_t>(i)].histories, acc);

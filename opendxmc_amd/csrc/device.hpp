@@ -34,7 +34,10 @@ struct DevCounters {
     double deposited;
     unsigned long long uncollided;   // histories that left the grid with no real interaction
     unsigned long long crossings;    // macro-cell boundary crossings (no voxel read)
-    unsigned long long phase_cycles[3];   // wavefront clock cycles in refill, interact, step (DXG_PHASE_CLOCKS builds)
+    // per phase (refill, interact, outer flights, step), DXG_PHASE_CLOCKS builds only:
+    unsigned long long phase_cycles[4];   // wavefront clock cycles
+    unsigned long long phase_lanes[4];    // lanes doing the phase's work, summed over iterations
+    unsigned long long phase_iters[4];    // iterations
 };
 
 // Packed voxel word: the FP32 density with its 8 low mantissa bits replaced

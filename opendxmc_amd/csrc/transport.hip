@@ -411,6 +411,36 @@ __device__ inline int sampleSource(const DevParams& P, const uint32_t* s_blob, u
     return ei;
 }
 
+// Photons of a work-group sampled at their source and carried to the grid's
+// box (sampleSource, then boxEntry) by one wavefront for 64 histories at
+// once, with every lane busy, and taken by the dead lanes of any wavefront of
+// the group: a refill then costs its lanes three LDS reads instead of the
+// sampling, run for the few lanes that happen to be dead.
+//   head counts entries taken, tail entries published; both only grow.  A
+//   wavefront samples only while it holds busy and the ring is empty, so it
+//   never overwrites an entry that is published and not yet taken.  A taker
+//   reads its entries, then claims them with a compare-and-swap on head; when
+//   that fails (another wavefront took entries first, after which the ring
+//   may have been written again) it reads again.  done is set, before busy
+//   clears, by the sampler that finds the instance's pool used up.
+constexpr int kRing = 64;
+struct SourceQueue {
+    unsigned head, tail, busy, done;
+    unsigned long long cur, end;   // pool histories claimed by the group and not yet sampled (sampler only)
+    float4 ring[3][kRing];         // px py pz dx | dy dz E w | RNG state after sampling
+};
+
+__device__ inline void queueReset(SourceQueue& Q)
+{
+    Q.head = Q.tail = Q.busy = Q.done = 0;
+    Q.cur = Q.end = 0;
+}
+
+__device__ inline unsigned ldsAcquire(const unsigned* a)
+{
+    return __builtin_amdgcn_readfirstlane(__hip_atomic_load(a, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+
 }  // namespace
 
 // Stages one instance's tables into LDS: the majorant, the attenuation table
@@ -442,7 +472,7 @@ __device__ inline const float* stageTables(const DevParams& P, uint32_t* lds)
 // then adds the wavefront's counters and energy sums to P.counters.
 template <bool LDS_LUT, int LEVEL, bool CELLS>
 __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_maj, const float* mu_tab, const uint32_t* s_blob,
-                                             double* e_wave)
+                                             double* e_wave, SourceQueue& Q)
 {
     const int lane = static_cast<int>(__lane_id());
     float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 1, E = 0, w = 0;
@@ -469,15 +499,19 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
 
     // wavefront-uniform event counters (popcounts of ballots, kept in SGPRs)
     unsigned long long n_steps = 0, n_real = 0, n_dep = 0, n_hist = 0, n_cross = 0;
-    // wavefront-private batch of pool histories [cur, end), claimed P.batch at a time
-    unsigned long long cur = 0, end = 0;
 #ifdef DXG_PHASE_CLOCKS
-    // wavefront clock cycles per phase (refill, interact, step): diagnostics builds only
-    unsigned long long phase_cycles[3] = { 0, 0, 0 };
+    // per phase (refill, interact, outer flights, step): wavefront clock
+    // cycles, working lanes and iterations; diagnostics builds only
+    unsigned long long phase_cycles[4] = { 0, 0, 0, 0 }, phase_lanes[4] = { 0, 0, 0, 0 }, phase_iters[4] = { 0, 0, 0, 0 };
     unsigned long long t_iter = 0;
-#define DXG_PHASE_END(k) phase_cycles[k] += __builtin_amdgcn_s_memtime() - t_iter
+#define DXG_PHASE_END(k, lanes)                                        \
+    do {                                                               \
+        phase_cycles[k] += __builtin_amdgcn_s_memtime() - t_iter;      \
+        phase_lanes[k] += static_cast<unsigned long long>(lanes);      \
+        phase_iters[k] += 1;                                           \
+    } while (0)
 #else
-#define DXG_PHASE_END(k)
+#define DXG_PHASE_END(k, lanes)
 #endif
 
     for (;;) {
@@ -494,37 +528,45 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
 
         // ------------------------------------------------------------ refill
         if (n_dead > 0 && (n_dead >= P.refill_threshold || (n_track == 0 && n_int == 0 && n_outer == 0))) {
-            const unsigned long long rank = __popcll(m_dead & ((1ull << lane) - 1ull));
-            const unsigned long long avail = end - cur;
-            unsigned long long mine;
-            if (avail >= static_cast<unsigned long long>(n_dead)) {
-                mine = cur + rank;
-                cur += n_dead;
-            } else {
-                // the rest of this batch first, then a fresh batch from the pool
-                const int leader = __ffsll(static_cast<long long>(m_dead)) - 1;
-                unsigned long long base = 0;
-                if (lane == leader)
-                    base = atomicAdd(P.pool, static_cast<unsigned long long>(P.batch));
-                base = __shfl(base, leader, 64);
-                mine = rank < avail ? cur + rank : base + (rank - avail);
-                cur = base + (n_dead - avail);
-                end = base + P.batch;
-            }
-            bool started = false;
-            if (state == kDead) {
-                const unsigned long long h = P.h_begin + mine;
-                if (h >= P.h_end) {
-                    state = kDone;
-                } else {
-                    started = true;
-                    collided = false;
-                    rngSeed(rng, P.seed, h + P.rng_offset);
-                    float ax, ay;
-                    (void)sampleSource(P, s_blob, h, rng, px, py, pz, dx, dy, dz, E, w, ax, ay);
-                    const double we = static_cast<double>(w) * static_cast<double>(E);
-                    atomicAdd(&e_wave[0], we);
-                    if (boxEntry(P, px, py, pz, dx, dy, dz)) {
+            const int rank = __popcll(m_dead & ((1ull << lane) - 1ull));
+            const bool dead = state == kDead;
+            // take up to n_dead photons from the group's queue
+            int k = 0;
+            for (;;) {
+                const unsigned head = ldsAcquire(&Q.head);
+                const unsigned tail = ldsAcquire(&Q.tail);
+                k = min(n_dead, static_cast<int>(tail - head));
+                if (k == 0)
+                    break;
+                const bool take = dead && rank < k;
+                float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a, c = a;
+                if (take) {
+                    const int slot = static_cast<int>((head + static_cast<unsigned>(rank)) & (kRing - 1));
+                    a = Q.ring[0][slot];
+                    b = Q.ring[1][slot];
+                    c = Q.ring[2][slot];
+                }
+                int ok = 0;
+                if (lane == 0) {
+                    unsigned expected = head;
+                    ok = __hip_atomic_compare_exchange_strong(&Q.head, &expected, head + static_cast<unsigned>(k), __ATOMIC_ACQ_REL,
+                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                if (__builtin_amdgcn_readfirstlane(ok)) {
+                    if (take) {
+                        px = a.x;
+                        py = a.y;
+                        pz = a.z;
+                        dx = a.w;
+                        dy = b.x;
+                        dz = b.y;
+                        E = b.z;
+                        w = b.w;
+                        rng.s0 = __float_as_uint(c.x);
+                        rng.s1 = __float_as_uint(c.y);
+                        rng.s2 = __float_as_uint(c.z);
+                        rng.s3 = __float_as_uint(c.w);
+                        collided = false;
                         energyBin<CELLS>(E, s_maj, ej, et, maj);
                         outer = !CELLS && P.outer_on && !inBox(P.outer_lo, P.outer_hi, px, py, pz);
                         state = outer ? kOuter : kTrack;
@@ -533,14 +575,88 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                             cellc = cellCoord(px, P.bmin[0], P.cell_inv[0], P.cnx, dx)
                                     | (cellCoord(py, P.bmin[1], P.cell_inv[1], P.cny, dy) << 10)
                                     | (cellCoord(pz, P.bmin[2], P.cell_inv[2], P.cnz, dz) << 20);
-                    } else {
-                        atomicAdd(&e_wave[1], we);
                     }
+                    break;
                 }
             }
-            n_hist += __popcll(__ballot(started));
-            DXG_PHASE_END(0);
-            continue;
+            bool sampled = false;
+            if (k < n_dead) {
+                // the queue ran short: sample the next 64 histories unless
+                // another wavefront is at it or the pool is used up
+                int got = 0;
+                if (lane == 0) {
+                    unsigned expected = 0;
+                    got = __hip_atomic_load(&Q.done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == 0
+                          && __hip_atomic_compare_exchange_strong(&Q.busy, &expected, 1u, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                if (__builtin_amdgcn_readfirstlane(got)) {
+                    const unsigned tail = ldsAcquire(&Q.tail);
+                    if (ldsAcquire(&Q.head) == tail) {   // every published entry taken: the ring may be written
+                        sampled = true;
+                        int n = 0;
+                        unsigned long long first = 0;
+                        if (lane == 0) {
+                            const unsigned long long total = P.h_end - P.h_begin;
+                            unsigned long long qc = Q.cur, qe = Q.end;
+                            if (qc >= qe) {
+                                const unsigned long long base = atomicAdd(P.pool, static_cast<unsigned long long>(P.batch));
+                                qc = base;
+                                qe = min(base + static_cast<unsigned long long>(P.batch), total);
+                            }
+                            n = qe > qc ? static_cast<int>(min(qe - qc, 64ull)) : 0;
+                            first = qc;
+                            Q.cur = qc + static_cast<unsigned long long>(n);
+                            Q.end = qe;
+                            if (n == 0)
+                                __hip_atomic_store(&Q.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                        n = __builtin_amdgcn_readfirstlane(n);
+                        first = (static_cast<unsigned long long>(__builtin_amdgcn_readfirstlane(static_cast<unsigned>(first >> 32))) << 32)
+                                | __builtin_amdgcn_readfirstlane(static_cast<unsigned>(first));
+                        bool hit = false;
+                        float sx = 0, sy = 0, sz = 0, sdx = 0, sdy = 0, sdz = 1, sE = 0, sw = 0;
+                        Rng sr { 1, 2, 3, 4 };
+                        if (lane < n) {
+                            const unsigned long long h = P.h_begin + first + static_cast<unsigned long long>(lane);
+                            rngSeed(sr, P.seed, h + P.rng_offset);
+                            float ax, ay;
+                            (void)sampleSource(P, s_blob, h, sr, sx, sy, sz, sdx, sdy, sdz, sE, sw, ax, ay);
+                            const double we = static_cast<double>(sw) * static_cast<double>(sE);
+                            atomicAdd(&e_wave[0], we);
+                            hit = boxEntry(P, sx, sy, sz, sdx, sdy, sdz);
+                            if (!hit)
+                                atomicAdd(&e_wave[1], we);
+                        }
+                        const unsigned long long m_hit = __ballot(hit);
+                        if (hit) {
+                            const int slot = static_cast<int>((tail + static_cast<unsigned>(__popcll(m_hit & ((1ull << lane) - 1ull)))) & (kRing - 1));
+                            Q.ring[0][slot] = make_float4(sx, sy, sz, sdx);
+                            Q.ring[1][slot] = make_float4(sdy, sdz, sE, sw);
+                            Q.ring[2][slot] = make_float4(__uint_as_float(sr.s0), __uint_as_float(sr.s1), __uint_as_float(sr.s2), __uint_as_float(sr.s3));
+                        }
+                        n_hist += static_cast<unsigned long long>(n);
+                        if (lane == 0)   // publish, after the ring writes
+                            __hip_atomic_fetch_add(&Q.tail, static_cast<unsigned>(__popcll(m_hit)), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    if (lane == 0)
+                        __hip_atomic_store(&Q.busy, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                // the pool used up and every entry taken: the lanes still dead are done
+                if (ldsAcquire(&Q.done) != 0 && ldsAcquire(&Q.head) == ldsAcquire(&Q.tail) && dead && rank >= k)
+                    state = kDone;
+            }
+            DXG_PHASE_END(0, n_dead);
+            if (k > 0 || sampled)
+                continue;
+            if (n_track + n_int + n_outer == 0) {   // nothing to do but wait for another wavefront's sampling
+                __builtin_amdgcn_s_sleep(2);
+                continue;
+            }
+            // nothing taken: the other phases run in this iteration
+#ifdef DXG_PHASE_CLOCKS
+            t_iter = __builtin_amdgcn_s_memtime();
+#endif
         }
 
         // -------------------------------------------------------- interact
@@ -731,7 +847,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
             if (deps > 0)
                 atomicAdd(&e_wave[2], e_scored);
             n_dep += __popcll(__ballot(deps > 0)) + __popcll(__ballot(deps > 1));
-            DXG_PHASE_END(1);
+            DXG_PHASE_END(1, n_int);
             continue;
         }
 
@@ -815,7 +931,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
             n_steps += __popcll(__ballot(tentative));
             for (int b = 0; b < 4; ++b)   // crossed <= kMaxCross < 16: sum it bit by bit
                 n_cross += static_cast<unsigned long long>(__popcll(__ballot((crossed >> b) & 1))) << b;
-            DXG_PHASE_END(2);
+            DXG_PHASE_END(3, n_track);
             continue;
         }
         // ----------------------------------------------- outer-medium flights
@@ -865,7 +981,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     state = kDead;
                 }
             }
-            DXG_PHASE_END(2);
+            DXG_PHASE_END(2, n_outer);
             continue;
         }
 
@@ -894,7 +1010,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
             }
         }
         n_steps += n_track;
-        DXG_PHASE_END(2);
+        DXG_PHASE_END(3, n_track);
     }
 #undef DXG_PHASE_END
 
@@ -909,8 +1025,11 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
         atomicAdd(&P.counters->uncollided, n_unc);
         atomicAdd(&P.counters->crossings, n_cross);
 #ifdef DXG_PHASE_CLOCKS
-        for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < 4; ++k) {
             atomicAdd(&P.counters->phase_cycles[k], phase_cycles[k]);
+            atomicAdd(&P.counters->phase_lanes[k], phase_lanes[k]);
+            atomicAdd(&P.counters->phase_iters[k], phase_iters[k]);
+        }
 #endif
         atomicAdd(&P.counters->emitted, e_emit);
         atomicAdd(&P.counters->escaped, e_esc);
@@ -925,14 +1044,17 @@ __global__ void __launch_bounds__(kBlock) DXG_TRANSPORT_OCCUPANCY transport_kern
     const float* mu_tab = stageTables<LDS_LUT>(P, lds);
     // per-wavefront FP64 sums in LDS: energy emitted, escaped, deposited; uncollided exits
     __shared__ double s_energy[kBlock / 64][4];
+    __shared__ SourceQueue s_queue;
     const int lane = static_cast<int>(__lane_id());
     const int wave = static_cast<int>(threadIdx.x) >> 6;
     if (lane < 4)
         s_energy[wave][lane] = 0.0;
+    if (threadIdx.x == 0)
+        queueReset(s_queue);
     __syncthreads();
     const float* s_maj = reinterpret_cast<const float*>(lds);
     const uint32_t* s_blob = lds + kLutNd + (LDS_LUT ? P.n_mat * kLutNd : 0);
-    runHistories<LDS_LUT, LEVEL, CELLS>(P, s_maj, mu_tab, s_blob, s_energy[wave]);
+    runHistories<LDS_LUT, LEVEL, CELLS>(P, s_maj, mu_tab, s_blob, s_energy[wave], s_queue);
 }
 
 // Many instances (world + beam) in one persistent launch.  A block starts on
@@ -946,6 +1068,7 @@ __global__ void __launch_bounds__(kBlock) DXG_TRANSPORT_OCCUPANCY transport_batc
 {
     extern __shared__ uint32_t lds[];
     __shared__ double s_energy[kBlock / 64][4];
+    __shared__ SourceQueue s_queue;
     __shared__ int s_next;
     const int lane = static_cast<int>(__lane_id());
     const int wave = static_cast<int>(threadIdx.x) >> 6;
@@ -956,10 +1079,12 @@ __global__ void __launch_bounds__(kBlock) DXG_TRANSPORT_OCCUPANCY transport_batc
         const float* mu_tab = stageTables<LDS_LUT>(P, lds);
         if (lane < 4)
             s_energy[wave][lane] = 0.0;
+        if (threadIdx.x == 0)
+            queueReset(s_queue);
         __syncthreads();
         const float* s_maj = reinterpret_cast<const float*>(lds);
         const uint32_t* s_blob = lds + kLutNd + (LDS_LUT ? P.n_mat * kLutNd : 0);
-        runHistories<LDS_LUT, LEVEL, CELLS>(P, s_maj, mu_tab, s_blob, s_energy[wave]);
+        runHistories<LDS_LUT, LEVEL, CELLS>(P, s_maj, mu_tab, s_blob, s_energy[wave], s_queue);
         __syncthreads();   // every wavefront is done with this instance's tables
         if (threadIdx.x == 0) {
             int next = -1;
@@ -995,8 +1120,8 @@ __global__ void batch_reset_kernel(const DevParams* __restrict__ params, int n_i
     c->deposited = 0;
     c->uncollided = 0;
     c->crossings = 0;
-    for (int k = 0; k < 3; ++k)
-        c->phase_cycles[k] = 0;
+    for (int k = 0; k < 4; ++k)
+        c->phase_cycles[k] = c->phase_lanes[k] = c->phase_iters[k] = 0;
 }
 
 // The traversal of the transport kernels replayed on given rays: the slab
@@ -1083,8 +1208,8 @@ __global__ void reset_kernel(unsigned long long* pool, DevCounters* c)
         c->deposited = 0;
         c->uncollided = 0;
         c->crossings = 0;
-        for (int k = 0; k < 3; ++k)
-            c->phase_cycles[k] = 0;
+        for (int k = 0; k < 4; ++k)
+            c->phase_cycles[k] = c->phase_lanes[k] = c->phase_iters[k] = 0;
     }
 }
 

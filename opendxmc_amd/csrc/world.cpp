@@ -549,6 +549,21 @@ int buildHostBeam(const dxg_beam* beam, HostBeam& hb)
         }
         if (!(tot > 0))
             return fail(DXG_ERR_ARGUMENT, "spectrum weights sum to zero");
+        // a spectrum equal to an earlier one (both tubes of a dual-source beam
+        // at one voltage) shares its table: the tables sit in LDS, where every
+        // kilobyte counts against work-groups per CU
+        uint64_t same = s;
+        for (uint64_t j = 0; j < s && same == s; ++j) {
+            const dxg_spectrum& o = beam->spectra[j];
+            if (o.n == spec.n && std::equal(spec.energy_kev, spec.energy_kev + spec.n, o.energy_kev)
+                && std::equal(spec.weight, spec.weight + spec.n, o.weight))
+                same = j;
+        }
+        if (same != s) {
+            soff.push_back(soff[same]);
+            sn.push_back(sn[same]);
+            continue;
+        }
         std::vector<float> prob;
         std::vector<int> alias;
         aliasTable(wv, prob, alias);
@@ -776,7 +791,7 @@ struct Tuned {
     unsigned long long batch;
 };
 constexpr Tuned kTuneCandidates[] = {
-    { 16, 24, 16, 256 }, { 12, 20, 12, 256 }, { 24, 32, 16, 256 }, { 16, 32, 24, 256 }, { 20, 24, 16, 512 }, { 12, 24, 8, 128 },
+    { 16, 24, 16, 256 }, { 12, 20, 12, 256 }, { 16, 32, 24, 256 }, { 20, 24, 16, 512 }, { 12, 24, 8, 128 },
 };
 constexpr int kTuneRounds = 2;   // every candidate timed twice, interleaved; the faster launch counts
 
@@ -878,8 +893,11 @@ int runChunks(dxg_world* w, DevParams& p, uint64_t total, dxg_progress* progress
         acc.deposited_energy += c.deposited;
         acc.uncollided += c.uncollided;
         acc.cell_crossings += c.crossings;
-        for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < 4; ++k) {
             acc.phase_cycles[k] += c.phase_cycles[k];
+            acc.phase_lanes[k] += c.phase_lanes[k];
+            acc.phase_iters[k] += c.phase_iters[k];
+        }
         w->histories += h_end - h_begin;
         if (progress)
             progress->n.store(h_end);

@@ -386,7 +386,8 @@ def test_autotuned_launch_settings_leave_results_unchanged():
                   flush=True)
             counts[mode] = {k: c[k] for k in ("histories", "steps", "real_events", "deposits", "uncollided")}
             if mode == "tuned":
-                assert c["autotuned"] == 1 and c["kernel_launches"] > 12
+                # five candidate settings, each timed twice, then the rest of the run
+                assert c["autotuned"] == 1 and c["kernel_launches"] > 10
                 first = (c["refill_threshold"], c["interact_threshold"], c["outer_threshold"], c["batch"])
                 assert dx.Transport(seed=6)(world, beam, None, False)
                 c2 = grid.counters()
