@@ -411,11 +411,51 @@ __device__ inline int sampleSource(const DevParams& P, const uint32_t* s_blob, u
     return ei;
 }
 
+// Outcome of a flight through the outer medium (outerFlight).
+enum FlightEnd { kFlightCollide = 0, kFlightBox = 1, kFlightEscape = 2 };
+
+// A photon in the outer medium (outside the box of the non-outer voxels,
+// where every voxel holds outer_word): its flight is sampled exactly with
+// that medium's attenuation and ends at a real collision in the outer medium
+// (vi receives the voxel), at the box's face (the photon continues there with
+// Woodcock steps: flights are memoryless, so the rest is drawn anew) or at the
+// grid's edge.  One random number.
+__device__ inline FlightEnd outerFlight(const DevParams& P, const float* mu_tab, int ej, float et, Rng& rng, float& px, float& py,
+                                        float& pz, float dx, float dy, float dz, int& vi)
+{
+    const int ma = wordMaterial(P.outer_word);
+    const float mu_a = lerpTab(mu_tab + ma * kLutNd, ej, et) * wordDensity(P.outer_word);
+    const float s = -hwLog(rngU(rng)) * hwRcp(mu_a);
+    const float ix = hwRcp(dx), iy = hwRcp(dy), iz = hwRcp(dz);
+    float t0 = 0.0f, t1 = __builtin_inff();
+    clipSlab(px, dx, ix, P.outer_lo[0], P.outer_hi[0], t0, t1);
+    clipSlab(py, dy, iy, P.outer_lo[1], P.outer_hi[1], t0, t1);
+    clipSlab(pz, dz, iz, P.outer_lo[2], P.outer_hi[2], t0, t1);
+    // an empty box (a uniform grid: lo above hi) is never entered
+    const float t_box = t0 <= t1 && P.outer_lo[0] <= P.outer_hi[0] ? t0 : __builtin_inff();
+    float t_edge = 0.0f, t_far = __builtin_inff();   // the grid: where the ray leaves it
+    clipSlab(px, dx, ix, P.bmin[0], P.bmax[0], t_edge, t_far);
+    clipSlab(py, dy, iy, P.bmin[1], P.bmax[1], t_edge, t_far);
+    clipSlab(pz, dz, iz, P.bmin[2], P.bmax[2], t_edge, t_far);
+    if (s < fminf(t_box, t_far)) {
+        advance(px, py, pz, s, dx, dy, dz);
+        vi = voxelIndex(P, px, py, pz);
+        return vi < 0 ? kFlightEscape : kFlightCollide;   // vi < 0: rounding at the grid's face
+    }
+    if (t_box < t_far) {
+        advance(px, py, pz, t_box, dx, dy, dz);
+        return kFlightBox;
+    }
+    return kFlightEscape;
+}
+
 // Photons of a work-group sampled at their source and carried to the grid's
-// box (sampleSource, then boxEntry) by one wavefront for 64 histories at
-// once, with every lane busy, and taken by the dead lanes of any wavefront of
-// the group: a refill then costs its lanes three LDS reads instead of the
-// sampling, run for the few lanes that happen to be dead.
+// box (sampleSource, then boxEntry, then with global tracking the first
+// flight through the outer medium) by one wavefront for 64 histories at once,
+// with every lane busy, and taken by the dead lanes of any wavefront of the
+// group: a refill then costs its lanes three LDS reads instead of the
+// sampling, run for the few lanes that happen to be dead.  The draws are
+// those the taking lane would have made, in the same order.
 //   head counts entries taken, tail entries published; both only grow.  A
 //   wavefront samples only while it holds busy and the ring is empty, so it
 //   never overwrites an entry that is published and not yet taken.  A taker
@@ -561,15 +601,22 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         dy = b.x;
                         dz = b.y;
                         E = b.z;
-                        w = b.w;
+                        w = fabsf(b.w);
                         rng.s0 = __float_as_uint(c.x);
                         rng.s1 = __float_as_uint(c.y);
                         rng.s2 = __float_as_uint(c.z);
                         rng.s3 = __float_as_uint(c.w);
                         collided = false;
                         energyBin<CELLS>(E, s_maj, ej, et, maj);
-                        outer = !CELLS && P.outer_on && !inBox(P.outer_lo, P.outer_hi, px, py, pz);
-                        state = outer ? kOuter : kTrack;
+                        // at the box's face or entered straight into it, else at a
+                        // collision in the outer medium (negative weight)
+                        outer = b.w < 0.0f;
+                        state = kTrack;
+                        if (outer) {
+                            vi = voxelIndex(P, px, py, pz);
+                            mat = wordMaterial(P.outer_word);
+                            state = kInteract;
+                        }
                         tau = -1.0f;
                         if (CELLS)
                             cellc = cellCoord(px, P.bmin[0], P.cell_inv[0], P.cnx, dx)
@@ -625,6 +672,21 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                             const double we = static_cast<double>(sw) * static_cast<double>(sE);
                             atomicAdd(&e_wave[0], we);
                             hit = boxEntry(P, sx, sy, sz, sdx, sdy, sdz);
+                            if (hit && !CELLS && P.outer_on && !inBox(P.outer_lo, P.outer_hi, sx, sy, sz)) {
+                                // the first flight through the outer medium, here with
+                                // every lane busy: the photon is queued at the box's face
+                                // or at a collision in the outer medium (negative weight)
+                                int sej, svi;
+                                float set, smaj;
+                                energyBin<CELLS>(sE, s_maj, sej, set, smaj);
+                                const FlightEnd fe = outerFlight(P, mu_tab, sej, set, sr, sx, sy, sz, sdx, sdy, sdz, svi);
+                                if (fe == kFlightEscape) {
+                                    hit = false;
+                                    atomicAdd(&e_wave[3], 1.0);   // uncollided exit
+                                } else if (fe == kFlightCollide) {
+                                    sw = -sw;
+                                }
+                            }
                             if (!hit)
                                 atomicAdd(&e_wave[1], we);
                         }
@@ -941,40 +1003,14 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
         // few lanes beside the Woodcock steps of the rest.
         if (n_outer > 0 && (n_outer >= P.outer_threshold || n_track == 0)) {
             if (state == kOuter) {
-                // In the outer medium (every voxel outside the box holds outer_word):
-                // the flight is sampled exactly with that medium's attenuation and
-                // ends at a real collision, at the box's face (the photon continues
-                // there with Woodcock steps: flights are memoryless, so the rest is
-                // drawn anew) or at the grid's edge.
-                const int ma = wordMaterial(P.outer_word);
-                const float mu_a = lerpTab(mu_tab + ma * kLutNd, ej, et) * wordDensity(P.outer_word);
-                const float s = -hwLog(rngU(rng)) * hwRcp(mu_a);
-                const float ix = hwRcp(dx), iy = hwRcp(dy), iz = hwRcp(dz);
-                float t0 = 0.0f, t1 = __builtin_inff();
-                clipSlab(px, dx, ix, P.outer_lo[0], P.outer_hi[0], t0, t1);
-                clipSlab(py, dy, iy, P.outer_lo[1], P.outer_hi[1], t0, t1);
-                clipSlab(pz, dz, iz, P.outer_lo[2], P.outer_hi[2], t0, t1);
-                // an empty box (a uniform grid: lo above hi) is never entered
-                const float t_box = t0 <= t1 && P.outer_lo[0] <= P.outer_hi[0] ? t0 : __builtin_inff();
-                float t_edge = 0.0f, t_far = __builtin_inff();   // the grid: where the ray leaves it
-                clipSlab(px, dx, ix, P.bmin[0], P.bmax[0], t_edge, t_far);
-                clipSlab(py, dy, iy, P.bmin[1], P.bmax[1], t_edge, t_far);
-                clipSlab(pz, dz, iz, P.bmin[2], P.bmax[2], t_edge, t_far);
-                bool escape = false;
-                if (s < fminf(t_box, t_far)) {
-                    advance(px, py, pz, s, dx, dy, dz);
-                    vi = voxelIndex(P, px, py, pz);
-                    escape = vi < 0;   // rounding at the grid's face
-                    mat = ma;
+                const FlightEnd end = outerFlight(P, mu_tab, ej, et, rng, px, py, pz, dx, dy, dz, vi);
+                if (end == kFlightCollide) {
+                    mat = wordMaterial(P.outer_word);
                     state = kInteract;
-                } else if (t_box < t_far) {
-                    advance(px, py, pz, t_box, dx, dy, dz);
+                } else if (end == kFlightBox) {
                     outer = false;
                     state = kTrack;
                 } else {
-                    escape = true;
-                }
-                if (escape) {
                     atomicAdd(&e_wave[1], static_cast<double>(w) * static_cast<double>(E));
                     if (!collided)
                         atomicAdd(&e_wave[3], 1.0);   // uncollided exit
