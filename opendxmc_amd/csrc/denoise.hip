@@ -84,9 +84,13 @@ __host__ __device__ inline float fromBf16(uint32_t h)
     return f;
 }
 
+// Two floats to BF16 (round to nearest even) in one v_cvt_pk_bf16_f32; the
+// same rounding as toBf16 (a NaN comes out quiet).
 __device__ inline uint32_t pack2(float lo, float hi)
 {
-    return static_cast<uint32_t>(toBf16(lo)) | (static_cast<uint32_t>(toBf16(hi)) << 16);
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const bf16x2 v = { static_cast<__bf16>(lo), static_cast<__bf16>(hi) };
+    return __builtin_bit_cast(uint32_t, v);
 }
 
 __device__ inline bf16x8 asOperand(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
