@@ -35,6 +35,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <random>
 #include <string>
 #include <vector>
@@ -124,9 +125,9 @@ struct ConvArgs {
     int y32_channels;        // 1 or 32
     int relu;
     int nb, nz, ny, nx;
-    int wave_tiles;          // tiles to compute: nb*nz*ny*(nx/64), or the length of `tiles`
+    int wave_tiles;          // tiles (or row pairs) to compute: nb*nz*ny*(nx/64), or the length of `tiles`
     int chunk;               // work-group tiles (4 wave tiles) per work-group
-    const int* tiles;        // null: every tile; else the (ascending) tile indices to compute
+    const int* tiles;        // null: every tile; else the (ascending) tile indices to compute (pair entries: conv3d_pair_kernel)
 };
 
 // Weight fragment (tap, out channel co, 16-B chunk q) lives at chunk
@@ -156,6 +157,11 @@ __device__ inline u32x4 dpp4(u32x4 old, u32x4 src)
 #define DXG_CONV_WAVES 4
 #endif
 constexpr int kConvWaves = DXG_CONV_WAVES;
+// Output rows per wavefront tile of conv3d_rows_kernel.
+#ifndef DXG_CONV_ROWS
+#define DXG_CONV_ROWS 2
+#endif
+constexpr int kConvRows = DXG_CONV_ROWS;
 // One wavefront tile: 64 voxels of one x row.
 struct Tile {
     int x0, y, z, n;
@@ -379,6 +385,212 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
     }
 }
 
+
+// Row-pair convolution: a wavefront computes output rows y and y + 1 (64
+// voxels each, same x range) together.  Their 3x3 input-row neighbourhoods
+// overlap in six of nine rows per pair of rows: the pair streams the twelve
+// rows (dz, y - 1 .. y + 2) once and applies each to both outputs (rows y and
+// y + 1 feed six taps), so it loads 12 rows where two single-row tiles load
+// 18; the input loads, not the MFMAs or the weight reads, bound the
+// single-row kernel.  Same operand layout and ring as conv3d_kernel<true>;
+// twelve rows per pair keep row q of every pair in buffer q % 3.
+// Entries of a.tiles (when given) are pair * 4 + mask: bit 0 stores row y,
+// bit 1 row y + 1 (a pair index counts (n, z, y / 2, x / 64) like a tile
+// index with ceil(ny / 2) rows); without a list every pair is computed and
+// row y + 1 is stored when it exists.
+template <int R>
+__global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu(2))) void conv3d_rows_kernel(ConvArgs a)
+{
+    __shared__ u32x4 wl[kTaps * 32 * 4];   // 55,296 B
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kTaps * 32 * 4; i += 64 * kConvWaves) {
+        const int row = i >> 2;
+        wl[wSlot(row >> 5, row & 31, i & 3)] = reinterpret_cast<const u32x4*>(a.w)[i];
+    }
+    __syncthreads();
+
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, lr = lane & 15, lq = lane >> 4;
+    float bias[2][4];
+    for (int m = 0; m < 2; ++m)
+        for (int i = 0; i < 4; ++i)
+            bias[m][i] = a.bias ? a.bias[16 * m + 4 * lq + i] : 0.0f;
+
+    // XCD-aware order: consecutive work-group ids alternate between the 8
+    // XCDs, so map them to contiguous slabs of pairs per XCD.
+    const int g = gridDim.x;
+    const int logical = (blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
+    const int wg_tiles = (a.wave_tiles + kConvWaves - 1) / kConvWaves;
+    const int begin = logical * a.chunk;
+    const int end = min(begin + a.chunk, wg_tiles);
+    const int nxs = a.nx >> 6, nyp = (a.ny + R - 1) / R;
+    auto tileAt = [&](int wt, int& mask) {
+        Tile tc { 0, 0, 0, 0, false };
+        mask = 0;
+        const int k = wt * kConvWaves + wave;
+        if (wt < end && k < a.wave_tiles) {
+            int t = k;
+            if (a.tiles) {
+                t = a.tiles[k] >> R;
+                mask = a.tiles[k] & ((1 << R) - 1);
+            }
+            int r = t / nxs;
+            tc.x0 = (t - r * nxs) * 64;
+            tc.y = R * (r % nyp);
+            r /= nyp;
+            tc.z = r % a.nz;
+            tc.n = r / a.nz;
+            tc.valid = true;
+            if (!a.tiles)
+                mask = (1 << min(R, a.ny - tc.y)) - 1;
+        }
+        return tc;
+    };
+
+    constexpr int kRing = 3;
+    constexpr int kRows = 3 * (R + 2);
+    const u32x4 zero = { 0, 0, 0, 0 };
+    u32x4 buf[kRing][4], lbuf[kRing], rbuf[kRing];
+    bool have[kRing] = { false, false, false };
+    auto loadRow = [&](const Tile& tc, int q, int b) {
+        const int zz = tc.z + q / (R + 2) - 1, yy = tc.y + q % (R + 2) - 1;
+        have[b] = tc.valid && static_cast<unsigned>(zz) < static_cast<unsigned>(a.nz) && static_cast<unsigned>(yy) < static_cast<unsigned>(a.ny);
+        const int zc = min(max(zz, 0), a.nz - 1), yc = min(max(yy, 0), a.ny - 1);
+        const size_t rowb = (static_cast<size_t>(tc.n * a.nz + zc) * a.ny + yc) * a.nx;
+        const uint16_t* base = a.x + (rowb + tc.x0 + 4 * lr) * kC + 8 * lq;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            buf[b][j] = load16(base + j * kC);
+        lbuf[b] = load16(a.x + (rowb + max(tc.x0 - 1, 0)) * kC + 8 * lq);
+        rbuf[b] = load16(a.x + (rowb + min(tc.x0 + 64, a.nx - 1)) * kC + 8 * lq);
+    };
+
+    int cmask = 0;
+    Tile cur = tileAt(begin, cmask);
+    if (cur.valid) {
+        loadRow(cur, 0, 0);
+        loadRow(cur, 1, 1);
+    }
+    for (int wt = begin; cur.valid; ++wt) {
+        int nmask = 0;
+        const Tile nxt = tileAt(wt + 1, nmask);
+        const int x0 = cur.x0, y = cur.y, z = cur.z, n = cur.n;
+
+        f32x4 acc[R][2][4];   // [output row][channel half][voxel group]
+        for (int o = 0; o < R; ++o)
+            for (int m = 0; m < 2; ++m)
+                for (int j = 0; j < 4; ++j)
+                    acc[o][m][j] = f32x4 { 0, 0, 0, 0 };
+
+        // the epilogue's residual (or gate) values of both rows, loaded before the MFMAs
+        const size_t orow0 = (static_cast<size_t>(n * a.nz + z) * a.ny + y) * a.nx;
+        // (up to two rows: more would not leave registers for the ring)
+        constexpr bool kPrefetchSide = R <= 2;
+        u32x2 side[kPrefetchSide ? R : 1][4][2];
+        const uint16_t* side_src = a.res ? a.res : a.gate;
+        if (kPrefetchSide && side_src) {
+#pragma unroll
+            for (int o = 0; o < R; ++o) {
+                if (!((cmask >> o) & 1))
+                    continue;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const size_t v = orow0 + static_cast<size_t>(o) * a.nx + x0 + 4 * lr + j;
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+                        side[o][j][m] = *reinterpret_cast<const u32x2*>(side_src + v * kC + 16 * m + 4 * lq);
+                }
+            }
+        }
+
+#pragma unroll
+        for (int q = 0; q < kRows; ++q) {
+            const int b = q % kRing;
+            if (q + 2 < kRows)
+                loadRow(cur, q + 2, (q + 2) % kRing);
+            else
+                loadRow(nxt, q + 2 - kRows, (q + 2) % kRing);
+            if (!have[b])
+                continue;
+            const int dz = q / (R + 2), rr = q % (R + 2);   // input row y + rr - 1 of plane z + dz - 1
+            const u32x4(&c)[4] = buf[b];
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                bf16x8 bop[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (dx == 0)
+                        bop[j] = asOperand(c[j]);
+                    else if (dx < 0)
+                        bop[j] = asOperand(j > 0 ? c[j - 1] : dpp4<kRowShr1>(x0 > 0 ? lbuf[b] : zero, c[3]));   // lane n <- lane n-1
+                    else
+                        bop[j] = asOperand(j < 3 ? c[j + 1] : dpp4<kRowShl1>(x0 + 64 < a.nx ? rbuf[b] : zero, c[0]));   // lane n <- lane n+1
+                }
+#pragma unroll
+                for (int o = 0; o < R; ++o) {
+                    const int dy = rr - o;   // tap row of this input row for output row y + o
+                    if (dy < 0 || dy > 2)
+                        continue;
+                    const int tap = dz * 9 + dy * 3 + dx + 1;
+                    bf16x8 aop[2];
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+                        aop[m] = asOperand(wl[wSlot(tap, 16 * m + lr, lq)]);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            acc[o][m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop[m], bop[j], acc[o][m][j], 0, 0, 0);
+                }
+            }
+        }
+
+#pragma unroll
+        for (int o = 0; o < R; ++o) {
+            if (!((cmask >> o) & 1))
+                continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const size_t v = orow0 + static_cast<size_t>(o) * a.nx + x0 + 4 * lr + j;
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    const int co0 = 16 * m + 4 * lq;
+                    float out[4];
+                    for (int i = 0; i < 4; ++i)
+                        out[i] = acc[o][m][j][i] + bias[m][i];
+                    const u32x2 sv = kPrefetchSide ? side[kPrefetchSide ? o : 0][j][m]
+                                                   : (side_src ? *reinterpret_cast<const u32x2*>(side_src + v * kC + co0) : u32x2 { 0, 0 });
+                    if (a.res) {
+                        const u32x2 rv = sv;
+                        out[0] += fromBf16(rv[0] & 0xffffu);
+                        out[1] += fromBf16(rv[0] >> 16);
+                        out[2] += fromBf16(rv[1] & 0xffffu);
+                        out[3] += fromBf16(rv[1] >> 16);
+                    }
+                    if (a.relu)
+                        for (int i = 0; i < 4; ++i)
+                            out[i] = fmaxf(out[i], 0.0f);
+                    if (a.gate) {
+                        const u32x2 gv = a.res ? *reinterpret_cast<const u32x2*>(a.gate + v * kC + co0) : sv;
+                        const uint32_t h[4] = { gv[0] & 0xffffu, gv[0] >> 16, gv[1] & 0xffffu, gv[1] >> 16 };
+                        for (int i = 0; i < 4; ++i)
+                            if (h[i] == 0 || (h[i] & 0x8000u))
+                                out[i] = 0.0f;
+                    }
+                    if (a.y)
+                        *reinterpret_cast<u32x2*>(a.y + v * kC + co0) = u32x2 { pack2(out[0], out[1]), pack2(out[2], out[3]) };
+                    if (a.y32) {
+                        if (a.y32_channels == kC)
+                            *reinterpret_cast<f32x4*>(a.y32 + v * kC + co0) = f32x4 { out[0], out[1], out[2], out[3] };
+                        else if (co0 == 0)
+                            a.y32[v] = out[0];
+                    }
+                }
+            }
+        }
+        cur = nxt;
+        cmask = nmask;
+    }
+}
 
 struct WgradArgs {
     const uint16_t* x;       // layer input
@@ -771,6 +983,16 @@ bool convShift()
     return on;
 }
 
+// Row-pair convolution (conv3d_pair_kernel) unless DXG_CONV_PAIRS=0.
+bool convPairs()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("DXG_CONV_PAIRS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 struct Dims {
     int nb, nz, ny, nx;
     size_t voxels() const { return static_cast<size_t>(nb) * nz * ny * nx; }
@@ -797,12 +1019,15 @@ hipError_t launchConv(ConvArgs a, const Dims& d, hipStream_t s)
     a.nz = d.nz;
     a.ny = d.ny;
     a.nx = d.nx;
+    const bool pairs = convPairs();
     if (!a.tiles)
-        a.wave_tiles = d.nb * d.nz * d.ny * (d.nx / 64);
+        a.wave_tiles = d.nb * d.nz * (pairs ? (d.ny + kConvRows - 1) / kConvRows : d.ny) * (d.nx / 64);
     const int wg_tiles = (a.wave_tiles + kConvWaves - 1) / kConvWaves;
     const int grid = roundUp8(std::min(wg_tiles, 2 * computeUnits()));
     a.chunk = (wg_tiles + grid - 1) / grid;
-    if (convShift())
+    if (pairs)
+        hipLaunchKernelGGL(conv3d_rows_kernel<DXG_CONV_ROWS>, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
+    else if (convShift())
         hipLaunchKernelGGL(conv3d_kernel<true>, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
     else
         hipLaunchKernelGGL(conv3d_kernel<false>, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
@@ -887,6 +1112,25 @@ std::vector<int> bodyTiles(const uint8_t* mat, int nx, int ny, int nz, int nxp)
         if (c[i])
             tiles.push_back(static_cast<int>(i));
     return tiles;
+}
+
+// The row pairs of conv3d_pair_kernel that hold the given wave tiles
+// (ascending tile indices of a volume with ny rows and nxt tiles per row), as
+// pair * 4 + rows to store (bit 0 row y, bit 1 row y + 1), ascending.
+std::vector<int> pairTiles(const std::vector<int>& tiles, int ny, int nxt)
+{
+    const int nyp = (ny + kConvRows - 1) / kConvRows;
+    std::map<int, int> groups;
+    for (const int t : tiles) {
+        const int r = t / nxt, xt = t - r * nxt;
+        const int y = r % ny, plane = r / ny;
+        groups[(plane * nyp + y / kConvRows) * nxt + xt] |= 1 << (y % kConvRows);
+    }
+    std::vector<int> out;
+    out.reserve(groups.size());
+    for (const auto& [g, mask] : groups)
+        out.push_back((g << kConvRows) | mask);
+    return out;
 }
 
 template <typename T>
@@ -1363,7 +1607,9 @@ int dxg_denoiser_apply(dxg_denoiser* d, const dxg_world* w, double* dose_mgy)
     const bool skip = !(env && env[0] == '0');
     if (skip) {
         if (d->tiles_generation != v.generation || !d->tiles.p) {
-            const std::vector<int> t = bodyTiles(v.material_host, v.nx, v.ny, v.nz, dm.nx);
+            std::vector<int> t = bodyTiles(v.material_host, v.nx, v.ny, v.nz, dm.nx);
+            if (convPairs())
+                t = pairTiles(t, dm.ny, dm.nx / 64);
             HIP_TRY(d->tiles.reserve(std::max<size_t>(t.size(), 1)));
             if (!t.empty())
                 HIP_TRY(hipMemcpyAsync(d->tiles.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, d->stream));
