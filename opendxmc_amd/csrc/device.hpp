@@ -76,6 +76,7 @@ struct DevParams {
     // voxel grid of packed words, flat index x + nx (y + ny z)
     const uint32_t* voxels;
     int nx, ny, nz, n_mat;
+    int lut_n;   // energy bins of the attenuation and majorant tables staged in LDS (the beam's highest energy, <= kLutN)
     float bmin[3];
     float bmax[3];
     float inv_spacing[3];

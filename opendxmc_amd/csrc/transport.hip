@@ -44,6 +44,7 @@
 //   (level 1: x^2 drawn from F^2(x), accepted with (1+cos^2)/2).
 #include "device.hpp"
 
+#include <algorithm>
 #include <mutex>
 
 namespace dxg {
@@ -71,6 +72,12 @@ constexpr float kTwoPi = 6.283185307179586f;
 #define DXG_TRANSPORT_WAVES_PER_EU 6
 #endif
 constexpr int kBlock = DXG_TRANSPORT_BLOCK;
+// Resident work-groups per CU at most: the occupancy target.  Instances
+// that fit in 72 VGPRs and small LDS tables would allow seven; seven
+// measured slower (C2 10.67 against 10.50 ms per 1e8 histories, the 64
+// batched instances 28.5 against 26.3 ms).
+constexpr int kMaxBlocksPerCU = DXG_TRANSPORT_WAVES_PER_EU * 4 * 64 / DXG_TRANSPORT_BLOCK;   // 4 SIMDs per CU
+static_assert(kMaxBlocksPerCU >= 1, "work-groups per CU");
 #define DXG_TRANSPORT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(DXG_TRANSPORT_WAVES_PER_EU)))
 constexpr int kMaxCross = 8;    // macro-cell crossings a lane walks per iteration
 constexpr int kIaShellsd = 5;    // impulse-approximation shells per material
@@ -420,11 +427,11 @@ enum FlightEnd { kFlightCollide = 0, kFlightBox = 1, kFlightEscape = 2 };
 // (vi receives the voxel), at the box's face (the photon continues there with
 // Woodcock steps: flights are memoryless, so the rest is drawn anew) or at the
 // grid's edge.  One random number.
-__device__ inline FlightEnd outerFlight(const DevParams& P, const float* mu_tab, int ej, float et, Rng& rng, float& px, float& py,
-                                        float& pz, float dx, float dy, float dz, int& vi)
+__device__ inline FlightEnd outerFlight(const DevParams& P, const float* mu_tab, int lstride, int ej, float et, Rng& rng, float& px,
+                                        float& py, float& pz, float dx, float dy, float dz, int& vi)
 {
     const int ma = wordMaterial(P.outer_word);
-    const float mu_a = lerpTab(mu_tab + ma * kLutNd, ej, et) * wordDensity(P.outer_word);
+    const float mu_a = lerpTab(mu_tab + ma * lstride, ej, et) * wordDensity(P.outer_word);
     const float s = -hwLog(rngU(rng)) * hwRcp(mu_a);
     const float ix = hwRcp(dx), iy = hwRcp(dy), iz = hwRcp(dz);
     float t0 = 0.0f, t1 = __builtin_inff();
@@ -483,19 +490,37 @@ __device__ inline unsigned ldsAcquire(const unsigned* a)
 
 }  // namespace
 
+// LDS words before the beam blob: the majorant and, when LDS_LUT, the
+// attenuation table, lut_n bins each.
+template <bool LDS_LUT>
+__device__ inline int blobOffset(const DevParams& P)
+{
+    return P.lut_n + (LDS_LUT ? P.n_mat * P.lut_n : 0);
+}
+
 // Stages one instance's tables into LDS: the majorant, the attenuation table
-// when LDS_LUT, and the beam blob.  Returns the attenuation table to read.
+// when LDS_LUT (rows of lut_n bins: the beam's energies need no more), and the
+// beam blob.  Returns the attenuation table to read; its row stride is
+// lutStride<LDS_LUT>(P).
+template <bool LDS_LUT>
+__device__ inline int lutStride(const DevParams& P)
+{
+    return LDS_LUT ? P.lut_n : kLutNd;
+}
+
 template <bool LDS_LUT>
 __device__ inline const float* stageTables(const DevParams& P, uint32_t* lds)
 {
     float* s_maj = reinterpret_cast<float*>(lds);
-    float* s_mu = s_maj + kLutNd;
-    uint32_t* s_blob = lds + kLutNd + (LDS_LUT ? P.n_mat * kLutNd : 0);
-    for (int i = threadIdx.x; i < kLutNd; i += blockDim.x)
+    float* s_mu = s_maj + P.lut_n;
+    uint32_t* s_blob = lds + blobOffset<LDS_LUT>(P);
+    for (int i = threadIdx.x; i < P.lut_n; i += blockDim.x)
         s_maj[i] = P.majorant[i];
-    if (LDS_LUT)
-        for (int i = threadIdx.x; i < P.n_mat * kLutNd; i += blockDim.x)
-            s_mu[i] = P.mu_tot[i];
+    if (LDS_LUT)   // rows of lut_n bins (of kLutNd in memory)
+        for (int i = threadIdx.x; i < P.n_mat * P.lut_n; i += blockDim.x) {
+            const int m = i / P.lut_n;
+            s_mu[i] = P.mu_tot[m * kLutNd + (i - m * P.lut_n)];
+        }
     for (int i = threadIdx.x; i < P.bb.words; i += blockDim.x)
         s_blob[i] = P.blob[i];
     if (P.cell_ratio && P.cell_lds) {
@@ -515,6 +540,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                                              double* e_wave, SourceQueue& Q)
 {
     const int lane = static_cast<int>(__lane_id());
+    const int lstride = lutStride<LDS_LUT>(P);   // row stride of mu_tab
     float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 1, E = 0, w = 0;
     float maj = 1, et = 0;   // majorant at the photon's energy (its reciprocal without CELLS: energyBin)
     int ej = 0, vi = 0;
@@ -679,7 +705,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                                 int sej, svi;
                                 float set, smaj;
                                 energyBin<CELLS>(sE, s_maj, sej, set, smaj);
-                                const FlightEnd fe = outerFlight(P, mu_tab, sej, set, sr, sx, sy, sz, sdx, sdy, sdz, svi);
+                                const FlightEnd fe = outerFlight(P, mu_tab, lstride, sej, set, sr, sx, sy, sz, sdx, sdy, sdz, svi);
                                 if (fe == kFlightEscape) {
                                     hit = false;
                                     atomicAdd(&e_wave[3], 1.0);   // uncollided exit
@@ -984,7 +1010,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                     const uint32_t word = P.voxels[vi];
                     const int m = wordMaterial(word);
                     const float rho = wordDensity(word);
-                    const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
+                    const float mu = lerpTab(mu_tab + m * lstride, ej, et) * rho;
                     mat = m;
                     if (!(rngU(rng) * mu_c > mu))
                         state = kInteract;
@@ -1003,7 +1029,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
         // few lanes beside the Woodcock steps of the rest.
         if (n_outer > 0 && (n_outer >= P.outer_threshold || n_track == 0)) {
             if (state == kOuter) {
-                const FlightEnd end = outerFlight(P, mu_tab, ej, et, rng, px, py, pz, dx, dy, dz, vi);
+                const FlightEnd end = outerFlight(P, mu_tab, lstride, ej, et, rng, px, py, pz, dx, dy, dz, vi);
                 if (end == kFlightCollide) {
                     mat = wordMaterial(P.outer_word);
                     state = kInteract;
@@ -1035,7 +1061,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 const uint32_t word = P.voxels[vi];
                 const int m = wordMaterial(word);
                 const float rho = wordDensity(word);
-                const float mu = lerpTab(mu_tab + m * kLutNd, ej, et) * rho;
+                const float mu = lerpTab(mu_tab + m * lstride, ej, et) * rho;
                 mat = m;
                 if (P.outer_on && word == P.outer_word)   // outer medium: inside the box or out of it
                     outer = !inBox(P.outer_lo, P.outer_hi, px, py, pz);
@@ -1089,7 +1115,7 @@ __global__ void __launch_bounds__(kBlock) DXG_TRANSPORT_OCCUPANCY transport_kern
         queueReset(s_queue);
     __syncthreads();
     const float* s_maj = reinterpret_cast<const float*>(lds);
-    const uint32_t* s_blob = lds + kLutNd + (LDS_LUT ? P.n_mat * kLutNd : 0);
+    const uint32_t* s_blob = lds + blobOffset<LDS_LUT>(P);
     runHistories<LDS_LUT, LEVEL, CELLS>(P, s_maj, mu_tab, s_blob, s_energy[wave], s_queue);
 }
 
@@ -1119,7 +1145,7 @@ __global__ void __launch_bounds__(kBlock) DXG_TRANSPORT_OCCUPANCY transport_batc
             queueReset(s_queue);
         __syncthreads();
         const float* s_maj = reinterpret_cast<const float*>(lds);
-        const uint32_t* s_blob = lds + kLutNd + (LDS_LUT ? P.n_mat * kLutNd : 0);
+        const uint32_t* s_blob = lds + blobOffset<LDS_LUT>(P);
         runHistories<LDS_LUT, LEVEL, CELLS>(P, s_maj, mu_tab, s_blob, s_energy[wave], s_queue);
         __syncthreads();   // every wavefront is done with this instance's tables
         if (threadIdx.x == 0) {
@@ -1258,7 +1284,7 @@ __global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__
 size_t transportLdsBytes(const DevParams& p)
 {
     const size_t cells = p.cell_ratio && p.cell_lds ? (static_cast<size_t>(p.cnx) * p.cny * p.cnz + 3) / 4 : 0;
-    return sizeof(uint32_t) * (static_cast<size_t>(kLutNd) + (p.lds_lut ? static_cast<size_t>(p.n_mat) * kLutNd : 0)
+    return sizeof(uint32_t) * (static_cast<size_t>(p.lut_n) + (p.lds_lut ? static_cast<size_t>(p.n_mat) * p.lut_n : 0)
                                + static_cast<size_t>(p.bb.words) + cells);
 }
 
@@ -1316,7 +1342,7 @@ int transportBatchBlocksPerCU(bool lds_lut, int level, bool cells, size_t lds_by
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batchInstance(lds_lut, level, cells), kBlock, lds_bytes) != hipSuccess)
         return 1;
-    return n > 0 ? n : 1;
+    return std::clamp(n, 1, kMaxBlocksPerCU);
 }
 
 int transportBlocksPerCU(bool lds_lut, int level, bool cells, size_t lds_bytes)
@@ -1324,7 +1350,7 @@ int transportBlocksPerCU(bool lds_lut, int level, bool cells, size_t lds_bytes)
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transportInstance(lds_lut, level, cells), kBlock, lds_bytes) != hipSuccess)
         return 1;
-    return n > 0 ? n : 1;
+    return std::clamp(n, 1, kMaxBlocksPerCU);
 }
 
 void launchTraverse(const DevParams& p, uint64_t n_rays, const float* pos, const float* dir, uint64_t n_steps,

@@ -187,6 +187,7 @@ void fillParams(const dxg_world* w, dxg::DevParams& p)
     p.ny = static_cast<int>(w->dims[1]);
     p.nz = static_cast<int>(w->dims[2]);
     p.n_mat = static_cast<int>(w->mats.size());
+    p.lut_n = dxg::kLutN;
     for (int a = 0; a < 3; ++a) {
         const float half = 0.5f * (static_cast<float>(w->dims[a]) * static_cast<float>(w->spacing[a]));
         p.bmin[a] = -half;
@@ -520,6 +521,7 @@ struct HostBeam {
     std::vector<uint32_t> blob;
     dxg::BeamBlob bb {};
     uint64_t total = 0;
+    double e_max = 0.0;   // highest spectrum energy (keV): no photon of the beam ever has more
 };
 
 // Validates a beam and builds its device tables on the host.
@@ -546,6 +548,7 @@ int buildHostBeam(const dxg_beam* beam, HostBeam& hb)
                 return fail(DXG_ERR_ARGUMENT, "spectrum weights must be finite and non-negative");
             wv[k] = spec.weight[k];
             tot += wv[k];
+            hb.e_max = std::max(hb.e_max, e);
         }
         if (!(tot > 0))
             return fail(DXG_ERR_ARGUMENT, "spectrum weights sum to zero");
@@ -700,6 +703,14 @@ int runParams(dxg_world* w, const dxg_beam* beam, const HostBeam& hb, uint64_t s
             p.ppe = 0;
     p.inv_ppe = p.ppe ? 1.0 / static_cast<double>(p.ppe) : 0.0;
     p.bb = bb;
+    // The LDS tables stop at the beam's highest energy: energies only fall
+    // from there (Compton, fluorescence) or stay (Rayleigh), and a photon at
+    // E reads bin j = (int)((E - E0) * (1 / dE)) (FP32, as the kernel computes
+    // it) and the next, as with the full table; one bin of margin.
+    {
+        const float f = (static_cast<float>(hb.e_max) - static_cast<float>(dxg::kLutE0)) * static_cast<float>(1.0 / dxg::kLutDE);
+        p.lut_n = std::min(dxg::kLutN, static_cast<int>(f) + 3);
+    }
     p.seed = seed;
     p.rng_offset = w->histories;
     p.dose_scale = 1.0f;
