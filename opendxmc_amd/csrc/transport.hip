@@ -564,7 +564,9 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
     double* const tally = P.tally + (P.tally_replicas > 1 ? static_cast<size_t>(blockIdx.x % P.tally_replicas) * P.tally_stride : 0);
 
     // wavefront-uniform event counters (popcounts of ballots, kept in SGPRs)
-    unsigned long long n_steps = 0, n_real = 0, n_dep = 0, n_hist = 0, n_cross = 0;
+    // (32 bits: a wavefront's share of one launch of at most ~1e8 histories
+    // stays far below 2^32 events; 32-bit counters leave SGPRs to the loop)
+    unsigned n_steps = 0, n_real = 0, n_dep = 0, n_hist = 0, n_cross = 0;
 #ifdef DXG_PHASE_CLOCKS
     // per phase (refill, interact, outer flights, step): wavefront clock
     // cycles, working lanes and iterations; diagnostics builds only
@@ -723,7 +725,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                             Q.ring[1][slot] = make_float4(sdy, sdz, sE, sw);
                             Q.ring[2][slot] = make_float4(__uint_as_float(sr.s0), __uint_as_float(sr.s1), __uint_as_float(sr.s2), __uint_as_float(sr.s3));
                         }
-                        n_hist += static_cast<unsigned long long>(n);
+                        n_hist += static_cast<unsigned>(n);
                         if (lane == 0)   // publish, after the ring writes
                             __hip_atomic_fetch_add(&Q.tail, static_cast<unsigned>(__popcll(m_hit)), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
@@ -1018,7 +1020,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
             }
             n_steps += __popcll(__ballot(tentative));
             for (int b = 0; b < 4; ++b)   // crossed <= kMaxCross < 16: sum it bit by bit
-                n_cross += static_cast<unsigned long long>(__popcll(__ballot((crossed >> b) & 1))) << b;
+                n_cross += static_cast<unsigned>(__popcll(__ballot((crossed >> b) & 1))) << b;
             DXG_PHASE_END(3, n_track);
             continue;
         }
@@ -1080,12 +1082,12 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
     const double e_emit = e_wave[0], e_esc = e_wave[1], e_dep = e_wave[2];
     const unsigned long long n_unc = static_cast<unsigned long long>(e_wave[3]);
     if (lane == 0) {
-        atomicAdd(&P.counters->steps, n_steps);
-        atomicAdd(&P.counters->real_events, n_real);
-        atomicAdd(&P.counters->deposits, n_dep);
-        atomicAdd(&P.counters->histories, n_hist);
+        atomicAdd(&P.counters->steps, static_cast<unsigned long long>(n_steps));
+        atomicAdd(&P.counters->real_events, static_cast<unsigned long long>(n_real));
+        atomicAdd(&P.counters->deposits, static_cast<unsigned long long>(n_dep));
+        atomicAdd(&P.counters->histories, static_cast<unsigned long long>(n_hist));
         atomicAdd(&P.counters->uncollided, n_unc);
-        atomicAdd(&P.counters->crossings, n_cross);
+        atomicAdd(&P.counters->crossings, static_cast<unsigned long long>(n_cross));
 #ifdef DXG_PHASE_CLOCKS
         for (int k = 0; k < 4; ++k) {
             atomicAdd(&P.counters->phase_cycles[k], phase_cycles[k]);
