@@ -97,3 +97,81 @@ def run_both(ph, beam, level, seed=7, oracle=None, threads=16):
     se, se2, _, oc = oracle.transport(ph.dims, ph.spacing, ph.density, ph.material, ph.materials, ex, spectra, bows,
                                       level=level, seed=seed + 1000, threads=threads)
     return (e, e2), (se, se2), hg, oc["histories"], counters, oc
+
+
+BATCHES = 8   # independent batches per side in batched_pair
+
+
+def batched(run_batch, n_batches, labels, n_regions):
+    """Sum of (e, e2) and histories over batches, and per batch the total and
+    the region sums (voxels labelled 0..n_regions-1), per history."""
+    e = e2 = None
+    h = 0
+    sums = []
+    for b in range(n_batches):
+        be, be2, bh = run_batch(b)
+        sel = labels >= 0
+        regions = np.bincount(labels[sel], weights=be[sel], minlength=n_regions)
+        sums.append(np.concatenate(([be.sum()], regions)) / bh)
+        e = be if e is None else e + be
+        e2 = be2 if e2 is None else e2 + be2
+        h += bh
+    return (e, e2), h, np.array(sums)
+
+
+def batched_pair(oracle, ph, beam, gpu_per_batch, oracle_per_batch, level, labels, n_regions, threads=16):
+    """GPU and oracle runs of one beam in BATCHES independent batches each (other
+    seeds); returns the summed scores, the histories and the per-batch total
+    and region sums per history of both sides."""
+    world, grid = make_world(ph, level)
+    n_ex = beam.numberOfExposures()
+    assert labels.shape == (int(np.prod(ph.dims)),)
+
+    def gpu_batch(b):
+        grid.clearScores()
+        beam.setNumberOfParticlesPerExposure(int(gpu_per_batch // n_ex))
+        assert dx.Transport(seed=501 + b)(world, beam, None, False)
+        e, e2, _ = grid.energyScored()
+        return e, e2, grid.histories()
+
+    ex, spectra, bows = beam_inputs(beam)
+    ex = ex.copy()
+    ex["n_particles"] = int(oracle_per_batch // len(ex))
+
+    def oracle_batch(b):
+        se, se2, _, oc = oracle.transport(ph.dims, ph.spacing, ph.density, ph.material, ph.materials, ex, spectra,
+                                          bows, level=level, seed=9001 + b, threads=threads)
+        return se, se2, oc["histories"]
+
+    g, hg, sg = batched(gpu_batch, BATCHES, labels, n_regions)
+    o, ho, so = batched(oracle_batch, BATCHES, labels, n_regions)
+    return g, o, hg, ho, sg, so
+
+
+def check_regions(tag, g, o, hg, ho, sg, so, threshold, min_voxels, region_names=(), tol_total=0.002, tol_region=0.01):
+    """Voxels above threshold x max to the parity criterion; the total and each
+    region sum within tol and 4 standard errors of the difference, the
+    standard errors from the spread of the batch sums.  g and o may be
+    coarsened (coarsen) where single voxels hold too few events."""
+    v = compare(g, o, hg, ho, threshold=threshold)
+    diff = sg.mean(axis=0) - so.mean(axis=0)
+    se = np.sqrt(sg.var(axis=0, ddof=1) / len(sg) + so.var(axis=0, ddof=1) / len(so))
+    rel, z = diff / so.mean(axis=0), diff / se
+    print(f"[regions] {tag}: voxels >{threshold:.1%} of max: {v.n_voxels}, outside 3 SE {v.n_fail}; total per "
+          f"history gpu {sg[:, 0].mean():.6g} oracle {so[:, 0].mean():.6g} keV, rel diff {rel[0]:+.4%} "
+          f"+- {se[0] / so[:, 0].mean():.4%} (z {z[0]:+.2f}); histories {hg:.3g} / {ho:.3g}", flush=True)
+    for k, name in enumerate(region_names, start=1):
+        print(f"[regions] {tag}: {name}: {so[:, k].mean() / so[:, 0].mean():.2%} of the energy, rel diff "
+              f"{rel[k]:+.4%} +- {se[k] / so[:, k].mean():.4%} (z {z[k]:+.2f})", flush=True)
+    assert v.n_voxels >= min_voxels
+    assert v.ok_voxels, str(v)
+    assert abs(rel[0]) < tol_total and abs(z[0]) < 4.0, (rel[0], z[0])
+    for k, name in enumerate(region_names, start=1):
+        assert abs(rel[k]) < tol_region and abs(z[k]) < 4.0, (name, rel[k], z[k])
+
+
+def coarsen(scores, dims, f):
+    """(sum e, sum e^2) summed over blocks of f x f x f voxels (dims x, y, z,
+    each a multiple of f); deposits stay independent, so the second moments add."""
+    nx, ny, nz = dims
+    return tuple(a.reshape(nz // f, f, ny // f, f, nx // f, f).sum(axis=(1, 3, 5)).ravel() for a in scores)

@@ -20,73 +20,9 @@ import pytest
 import opendxmc_amd as dx
 from opendxmc_amd import phantoms
 from opendxmc_amd.bowtie import DEFAULT_CT_BOWTIE, read_filters
-from parity import beam_inputs, compare, make_world
+from parity import batched_pair, check_regions
 
 pytestmark = pytest.mark.gpu
-
-BATCHES = 8
-
-
-def batched(run_batch, n_batches, labels, n_regions):
-    """Sum of (e, e2) and histories over batches, and per batch the total and
-    the region sums (voxels labelled 0..n_regions-1), per history."""
-    e = e2 = None
-    h = 0
-    sums = []
-    for b in range(n_batches):
-        be, be2, bh = run_batch(b)
-        sel = labels >= 0
-        regions = np.bincount(labels[sel], weights=be[sel], minlength=n_regions)
-        sums.append(np.concatenate(([be.sum()], regions)) / bh)
-        e = be if e is None else e + be
-        e2 = be2 if e2 is None else e2 + be2
-        h += bh
-    return (e, e2), h, np.array(sums)
-
-
-def run_pair(oracle, ph, beam, gpu_per_batch, oracle_per_batch, level, labels, n_regions):
-    world, grid = make_world(ph, level)
-    n_ex = beam.numberOfExposures()
-    assert labels.shape == (int(np.prod(ph.dims)),)
-
-    def gpu_batch(b):
-        grid.clearScores()
-        beam.setNumberOfParticlesPerExposure(int(gpu_per_batch // n_ex))
-        assert dx.Transport(seed=501 + b)(world, beam, None, False)
-        e, e2, _ = grid.energyScored()
-        return e, e2, grid.histories()
-
-    ex, spectra, bows = beam_inputs(beam)
-    ex = ex.copy()
-    ex["n_particles"] = int(oracle_per_batch // len(ex))
-
-    def oracle_batch(b):
-        se, se2, _, oc = oracle.transport(ph.dims, ph.spacing, ph.density, ph.material, ph.materials, ex, spectra,
-                                          bows, level=level, seed=9001 + b, threads=16)
-        return se, se2, oc["histories"]
-
-    g, hg, sg = batched(gpu_batch, BATCHES, labels, n_regions)
-    o, ho, so = batched(oracle_batch, BATCHES, labels, n_regions)
-    return g, o, hg, ho, sg, so
-
-
-def check(tag, g, o, hg, ho, sg, so, threshold, min_voxels, region_names=()):
-    v = compare(g, o, hg, ho, threshold=threshold)
-    diff = sg.mean(axis=0) - so.mean(axis=0)
-    se = np.sqrt(sg.var(axis=0, ddof=1) / len(sg) + so.var(axis=0, ddof=1) / len(so))
-    rel, z = diff / so.mean(axis=0), diff / se
-    print(f"[precision] {tag}: voxels >{threshold:.1%} of max: {v.n_voxels}, outside 3 SE {v.n_fail}; total per "
-          f"history gpu {sg[:, 0].mean():.6g} oracle {so[:, 0].mean():.6g} keV, rel diff {rel[0]:+.4%} "
-          f"+- {se[0] / so[:, 0].mean():.4%} (z {z[0]:+.2f}); histories {hg:.3g} / {ho:.3g}", flush=True)
-    for k, name in enumerate(region_names, start=1):
-        print(f"[precision] {tag}: {name}: {so[:, k].mean() / so[:, 0].mean():.2%} of the energy, rel diff "
-              f"{rel[k]:+.4%} +- {se[k] / so[:, k].mean():.4%} (z {z[k]:+.2f})", flush=True)
-    assert v.n_voxels >= min_voxels
-    assert v.ok_voxels, str(v)
-    assert abs(rel[0]) < 0.002 and abs(z[0]) < 4.0, (rel[0], z[0])
-    for k, name in enumerate(region_names, start=1):
-        assert abs(rel[k]) < 0.01 and abs(z[k]) < 4.0, (name, rel[k], z[k])
-
 
 # Shells of the pencil beam's scatter: Chebyshev distance (in voxels) from the
 # axis column, which the beam at x = y = 0.05 cm runs through (voxel 32 of 64
@@ -108,10 +44,10 @@ def test_pencil_fp32_vs_fp64_scatter(oracle, level):
     ph = phantoms.water_cube(64, 0.1)
     beam = dx.PencilBeam((0.05, 0.05, -10.0), (0, 0, 1), 60.0)
     print(f"[precision] pencil 64^3 level {level}", flush=True)
-    res = run_pair(oracle, ph, beam, 1.25e7, 4e6, level, pencil_labels(), len(SHELLS))
+    res = batched_pair(oracle, ph, beam, 1.25e7, 4e6, level, pencil_labels(), len(SHELLS))
     names = [f"shell {a}-{b} voxels off the axis" if b else "axis column" for a, b in SHELLS]
     # voxels above 0.1 % of the maximum: ~2,000, nearly all off the axis
-    check(f"pencil level {level}", *res, threshold=1e-3, min_voxels=1500, region_names=names)
+    check_regions(f"pencil level {level}", *res, threshold=1e-3, min_voxels=1500, region_names=names)
 
 
 def test_ct_spiral_fp32_vs_fp64(oracle):
@@ -123,5 +59,5 @@ def test_ct_spiral_fp32_vs_fp64(oracle):
     beam.setBowtieFilter(read_filters()[DEFAULT_CT_BOWTIE])
     print("[precision] ct spiral 64x64x50", flush=True)
     z = np.repeat(np.arange(ph.dims[2]) * 4 // ph.dims[2], ph.dims[0] * ph.dims[1])   # four slabs along z
-    res = run_pair(oracle, ph, beam, 1.25e7, 2.5e6, 1, z, 4)
-    check("ct spiral", *res, threshold=0.01, min_voxels=20000, region_names=[f"z slab {k}" for k in range(4)])
+    res = batched_pair(oracle, ph, beam, 1.25e7, 2.5e6, 1, z, 4)
+    check_regions("ct spiral", *res, threshold=0.01, min_voxels=20000, region_names=[f"z slab {k}" for k in range(4)])

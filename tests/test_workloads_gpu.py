@@ -1,5 +1,10 @@
-"""The batched uneven workload (64 instances) and the full dual-source CT
+"""The batched uneven workload (64 instances), the CT spiral of the
+histories/s line (256 x 256 x 200 at 0.15 cm) and the full dual-source CT
 spiral (512 x 512 x 400 at 0.08 cm) on the device:
+  * CT spiral: its exact beam (541 exposures at 5 degrees) against the oracle
+    on a 64 x 64 x 50 grid of the same extent, and at full size in eight
+    independent batches per side: total and eight slabs along z within 4
+    standard errors, and 4 x 4 x 4-voxel blocks to the parity criterion;
   * batch: every instance's energy bookkeeping closes, its history count is
     its beam's, instances of each shape equal one dxg_transport_run each, and a
     DX and a CT sequential instance match the oracle;
@@ -20,7 +25,9 @@ from opendxmc_amd import batch_workload as bw
 from opendxmc_amd import denoise as dn
 from opendxmc_amd import denoise_study as study
 from opendxmc_amd import phantoms
-from parity import compare, make_world, run_both
+from bench import ct_workload
+from opendxmc_amd.bowtie import DEFAULT_CT_BOWTIE, read_filters
+from parity import batched_pair, check_regions, coarsen, compare, make_world, run_both
 
 pytestmark = pytest.mark.gpu
 
@@ -80,6 +87,37 @@ def test_batch_full_64_instances(oracle):
 
 def full_ct_beam(z_half: float, histories: float):
     return study.spiral_beam(z_half, 3.2, 120.0, True, histories)
+
+
+def c2_beam():
+    """The beam of the histories/s line: 120 kVp, 6 mm Al, W1 bowtie, 4 cm
+    collimation, pitch 1, 5 degree steps, SDD 119 cm, z from -15 to 15 cm."""
+    _, beam = ct_workload(1e6)
+    assert beam.numberOfExposures() == 541
+    return beam
+
+
+def z_slabs(dims, n):
+    nx, ny, nz = dims
+    return np.repeat(np.arange(nz) * n // nz, nx * ny)
+
+
+def test_c2_beam_on_reduced_grid(oracle):
+    ph = phantoms.ct_body((64, 64, 50), (0.6, 0.6, 0.6), seed=1)
+    print("[c2 reduced] 64x64x50 at 0.6 cm, the 541-exposure beam", flush=True)
+    res = batched_pair(oracle, ph, c2_beam(), 1.25e7, 2.5e6, 1, z_slabs(ph.dims, 4), 4)
+    check_regions("c2 reduced", *res, threshold=0.01, min_voxels=20000, region_names=[f"z slab {k}" for k in range(4)])
+
+
+def test_c2_full_size_against_oracle(oracle):
+    """256 x 256 x 200 at 0.15 cm: 1e8 GPU histories against 1e7 oracle
+    histories; a full-size voxel holds too few oracle events for the per-voxel
+    test, so it runs on 4 x 4 x 4 blocks."""
+    ph = phantoms.ct_body((256, 256, 200), (0.15, 0.15, 0.15), seed=1)
+    print("[c2 full] 256x256x200 at 0.15 cm", flush=True)
+    g, o, hg, ho, sg, so = batched_pair(oracle, ph, c2_beam(), 1.25e7, 1.25e6, 1, z_slabs(ph.dims, 8), 8)
+    check_regions("c2 full", coarsen(g, ph.dims, 4), coarsen(o, ph.dims, 4), hg, ho, sg, so, threshold=0.1, min_voxels=2000,
+                  region_names=[f"z slab {k}" for k in range(8)], tol_total=0.005, tol_region=0.02)
 
 
 def test_full_ct_spiral_beam_on_reduced_grid(oracle):
