@@ -35,7 +35,6 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
-#include <map>
 #include <random>
 #include <string>
 #include <vector>
@@ -1077,11 +1076,19 @@ std::vector<int> bodyTiles(const uint8_t* mat, int nx, int ny, int nz, int nxp)
     std::vector<uint8_t> b(rows * nxt, 0);
     for (size_t r = 0; r < rows; ++r) {
         const uint8_t* row = mat + r * nx;
-        for (int x = 0; x < nx; ++x)
-            if (row[x]) {
-                b[r * nxt + x / 64] = 1;
-                x = (x / 64 + 1) * 64 - 1;   // the rest of this tile is decided
+        for (int t = 0; t * 64 < nx; ++t) {   // a tile's 64 material bytes as eight 64-bit words
+            const int x0 = t * 64, n = std::min(64, nx - x0);
+            uint64_t any = 0;
+            int x = 0;
+            for (; x + 8 <= n; x += 8) {
+                uint64_t w8;
+                std::memcpy(&w8, row + x0 + x, 8);
+                any |= w8;
             }
+            for (; x < n; ++x)
+                any |= row[x0 + x];
+            b[r * nxt + t] = any != 0;
+        }
     }
     // x: kReach < 64, so one tile either side
     std::vector<uint8_t> c(b.size(), 0);
@@ -1119,17 +1126,20 @@ std::vector<int> bodyTiles(const uint8_t* mat, int nx, int ny, int nz, int nxp)
 // pair * 4 + rows to store (bit 0 row y, bit 1 row y + 1), ascending.
 std::vector<int> pairTiles(const std::vector<int>& tiles, int ny, int nxt)
 {
+    if (tiles.empty())
+        return {};
     const int nyp = (ny + kConvRows - 1) / kConvRows;
-    std::map<int, int> groups;
+    const int planes = tiles.back() / nxt / ny + 1;
+    std::vector<uint8_t> mask(static_cast<size_t>(planes) * nyp * nxt, 0);   // rows to store per group
     for (const int t : tiles) {
         const int r = t / nxt, xt = t - r * nxt;
         const int y = r % ny, plane = r / ny;
-        groups[(plane * nyp + y / kConvRows) * nxt + xt] |= 1 << (y % kConvRows);
+        mask[(static_cast<size_t>(plane) * nyp + y / kConvRows) * nxt + xt] |= static_cast<uint8_t>(1 << (y % kConvRows));
     }
     std::vector<int> out;
-    out.reserve(groups.size());
-    for (const auto& [g, mask] : groups)
-        out.push_back((g << kConvRows) | mask);
+    for (size_t g = 0; g < mask.size(); ++g)
+        if (mask[g])
+            out.push_back((static_cast<int>(g) << kConvRows) | mask[g]);
     return out;
 }
 
