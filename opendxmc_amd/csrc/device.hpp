@@ -121,6 +121,7 @@ struct DevParams {
     double* tally;
     unsigned long long tally_stride;
     int tally_replicas;
+    int tally_merge;      // wavefront deposits on one record merge in LDS before the atomics (small grids)
     float dose_scale;   // beam calibration applied to every deposit (1: none)
 
     // beam

@@ -257,6 +257,10 @@ __device__ inline void rotate(float& dx, float& dy, float& dz, float cost, float
     dz = nz * inv;
 }
 
+// LDS bytes per wavefront for the merge of tallyFlush: 64 hash slots (the
+// lane that owns each) and 64 lanes' record updates {w dE, (w dE)^2, events}.
+constexpr int kMergeWaveBytes = 64 * 4 + 64 * 3 * 8;
+
 // Scores the deposits of one interaction phase: each lane with deps > 0 made
 // deps (1 or 2) deposits e1 [, e2] at its voxel vi, and its record gains
 // (sum w dE, sum (w dE)^2, events) = (d1 + d2, d1^2 + d2^2, deps) with
@@ -265,20 +269,71 @@ __device__ inline void rotate(float& dx, float& dy, float& dz, float cost, float
 // one atomic instruction's lanes that fall in one 64-byte block travel as one
 // request; so a record update is spread over three consecutive lanes of one
 // instruction (one value each, all in the record's aligned 32 bytes) -- one
-// request per update where three separate atomics made three.
-__device__ inline void tallyFlush(const DevParams& P, double* tally, int deps, int vi, float e1, float e2)
+// request per update where three separate atomics made three.  With s_merge
+// (this wavefront's kMergeWaveBytes of LDS) updates of one record merge first.
+
+__device__ inline void tallyFlush(const DevParams& P, double* tally, int deps, int vi, float e1, float e2, unsigned char* s_merge)
 {
-    const unsigned long long m = __ballot(deps > 0);
-    const int k = __popcll(m);
-    if (k == 0)
+    const unsigned long long m0 = __ballot(deps > 0);
+    if (m0 == 0)
         return;
     const int lane = static_cast<int>(__lane_id());
+    const double scale = static_cast<double>(P.dose_scale);
+    if (s_merge) {
+        // Merge first (small grids, where many lanes deposit on few records:
+        // FP64 atomics on one record are applied one after another).  Each
+        // depositing lane claims the hash slot of its voxel; a lane whose slot
+        // went to another lane with the same voxel adds its update to that
+        // lane's in LDS and drops out; a lane that lost its slot to another
+        // voxel flushes its own.  One wavefront's LDS operations execute in
+        // program order, so the updates are in place before the adds and the
+        // adds before the reads.
+        int* own = reinterpret_cast<int*>(s_merge);
+        double* acc = reinterpret_cast<double*>(s_merge + 64 * 4);
+        const bool has = deps > 0;
+        const int hslot = static_cast<int>((static_cast<unsigned>(vi) * 2654435761u) >> 26);
+        if (has)
+            __hip_atomic_store(own + hslot, lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        const int w = has ? __hip_atomic_load(own + hslot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : lane;
+        const int vw = __builtin_amdgcn_ds_bpermute(w * 4, vi);
+        const bool dup = has && w != lane && vw == vi;
+        const double d1 = static_cast<double>(e1) * scale, d2 = static_cast<double>(e2) * scale;
+        const double u0 = d1 + d2, u1 = d1 * d1 + d2 * d2, u2 = static_cast<double>(deps);
+        if (has && !dup) {
+            acc[3 * lane] = u0;
+            acc[3 * lane + 1] = u1;
+            acc[3 * lane + 2] = u2;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        if (dup) {
+            atomicAdd(acc + 3 * w, u0);
+            atomicAdd(acc + 3 * w + 1, u1);
+            atomicAdd(acc + 3 * w + 2, u2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const unsigned long long m = __ballot(has && !dup);
+        const int k = __popcll(m);
+        const int rank = __popcll(m & ((1ull << lane) - 1ull));
+        const int dest = (has && !dup) ? rank : k + (lane - rank);
+        const int src_of = __builtin_amdgcn_ds_permute(dest * 4, lane);
+        for (int base = 0; base < 3 * k; base += 64) {
+            const int slot = base + lane;
+            const int q = slot / 3;
+            const int c = slot - 3 * q;
+            const int src = __builtin_amdgcn_ds_bpermute((q & 63) * 4, src_of);
+            const int idx = __builtin_amdgcn_ds_bpermute(src * 4, vi);
+            if (slot < 3 * k)
+                atomicAdd(tally + static_cast<size_t>(idx) * kTallyRecord + c, acc[3 * src + c]);
+        }
+        return;
+    }
+    const unsigned long long m = m0;
+    const int k = __popcll(m);
     const int rank = __popcll(m & ((1ull << lane) - 1ull));
     // lane q < k learns the lane of update q: updating lanes go to their rank,
     // the others to k and up, a permutation, so every destination is written once
     const int dest = deps > 0 ? rank : k + (lane - rank);
     const int src_of = __builtin_amdgcn_ds_permute(dest * 4, lane);
-    const double scale = static_cast<double>(P.dose_scale);
     for (int base = 0; base < 3 * k; base += 64) {   // wavefront-uniform rounds of 64 slots
         const int slot = base + lane;                  // slot = 3 * update + value
         const int q = slot / 3;
@@ -502,6 +557,15 @@ __device__ inline int blobOffset(const DevParams& P)
 // when LDS_LUT (rows of lut_n bins: the beam's energies need no more), and the
 // beam blob.  Returns the attenuation table to read; its row stride is
 // lutStride<LDS_LUT>(P).
+// Byte offset of the tally-merge area in dynamic LDS: after the beam blob and
+// the macro-cell bytes, 8-byte aligned; kMergeWaveBytes per wavefront.
+template <bool LDS_LUT>
+__device__ inline int mergeOffset(const DevParams& P)
+{
+    const int cells = P.cell_ratio && P.cell_lds ? (P.cnx * P.cny * P.cnz + 3) / 4 : 0;
+    return (4 * (blobOffset<LDS_LUT>(P) + P.bb.words + cells) + 7) & ~7;
+}
+
 template <bool LDS_LUT>
 __device__ inline int lutStride(const DevParams& P)
 {
@@ -537,7 +601,7 @@ __device__ inline const float* stageTables(const DevParams& P, uint32_t* lds)
 // then adds the wavefront's counters and energy sums to P.counters.
 template <bool LDS_LUT, int LEVEL, bool CELLS>
 __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_maj, const float* mu_tab, const uint32_t* s_blob,
-                                             double* e_wave, SourceQueue& Q)
+                                             double* e_wave, SourceQueue& Q, unsigned char* s_merge)
 {
     const int lane = static_cast<int>(__lane_id());
     const int lstride = lutStride<LDS_LUT>(P);   // row stride of mu_tab
@@ -933,7 +997,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 if (scatter_cost <= 1.0f)
                     rotate(dx, dy, dz, scatter_cost, kTwoPi * rngU(rng));
             }
-            tallyFlush(P, tally, deps, vi, e1, e2);
+            tallyFlush(P, tally, deps, vi, e1, e2, s_merge);
             if (deps > 0)
                 atomicAdd(&e_wave[2], e_scored);
             n_dep += __popcll(__ballot(deps > 0)) + __popcll(__ballot(deps > 1));
@@ -1118,7 +1182,8 @@ __global__ void __launch_bounds__(kBlock) DXG_TRANSPORT_OCCUPANCY transport_kern
     __syncthreads();
     const float* s_maj = reinterpret_cast<const float*>(lds);
     const uint32_t* s_blob = lds + blobOffset<LDS_LUT>(P);
-    runHistories<LDS_LUT, LEVEL, CELLS>(P, s_maj, mu_tab, s_blob, s_energy[wave], s_queue);
+    unsigned char* const s_merge = P.tally_merge ? reinterpret_cast<unsigned char*>(lds) + mergeOffset<LDS_LUT>(P) + wave * kMergeWaveBytes : nullptr;
+    runHistories<LDS_LUT, LEVEL, CELLS>(P, s_maj, mu_tab, s_blob, s_energy[wave], s_queue, s_merge);
 }
 
 // Many instances (world + beam) in one persistent launch.  A block starts on
@@ -1148,7 +1213,8 @@ __global__ void __launch_bounds__(kBlock) DXG_TRANSPORT_OCCUPANCY transport_batc
         __syncthreads();
         const float* s_maj = reinterpret_cast<const float*>(lds);
         const uint32_t* s_blob = lds + blobOffset<LDS_LUT>(P);
-        runHistories<LDS_LUT, LEVEL, CELLS>(P, s_maj, mu_tab, s_blob, s_energy[wave], s_queue);
+        unsigned char* const s_merge = P.tally_merge ? reinterpret_cast<unsigned char*>(lds) + mergeOffset<LDS_LUT>(P) + wave * kMergeWaveBytes : nullptr;
+        runHistories<LDS_LUT, LEVEL, CELLS>(P, s_maj, mu_tab, s_blob, s_energy[wave], s_queue, s_merge);
         __syncthreads();   // every wavefront is done with this instance's tables
         if (threadIdx.x == 0) {
             int next = -1;
@@ -1286,8 +1352,9 @@ __global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__
 size_t transportLdsBytes(const DevParams& p)
 {
     const size_t cells = p.cell_ratio && p.cell_lds ? (static_cast<size_t>(p.cnx) * p.cny * p.cnz + 3) / 4 : 0;
-    return sizeof(uint32_t) * (static_cast<size_t>(p.lut_n) + (p.lds_lut ? static_cast<size_t>(p.n_mat) * p.lut_n : 0)
-                               + static_cast<size_t>(p.bb.words) + cells);
+    const size_t bytes = sizeof(uint32_t) * (static_cast<size_t>(p.lut_n) + (p.lds_lut ? static_cast<size_t>(p.n_mat) * p.lut_n : 0)
+                                             + static_cast<size_t>(p.bb.words) + cells);
+    return p.tally_merge ? ((bytes + 7) & ~static_cast<size_t>(7)) + static_cast<size_t>(kBlock / 64) * kMergeWaveBytes : bytes;
 }
 
 namespace {
