@@ -363,6 +363,53 @@ def test_results_do_not_depend_on_the_phase_schedule():
         np.testing.assert_allclose(s2, s0, rtol=1e-9, atol=1e-12 * s0.max())
 
 
+@pytest.mark.parametrize("small_grid", [False, True])
+def test_extreme_launch_settings_leave_events_unchanged(small_grid):
+    """The source queue and the phase thresholds decide only which lane runs a
+    history and when, never what it draws: refill at every dead lane or only
+    when all 64 are dead, interaction and outer flights at 1 or 64 parked
+    lanes, pool claims of 64 or 4096 histories, and (small grid) the tally
+    merge on and off all give the same event counts; the energy scores agree
+    up to the order of FP64 additions."""
+    if small_grid:
+        ph = phantoms.water_cube(16, 0.25)
+        beam = dx.PencilBeam((0.05, 0.05, -10.0), (0, 0, 1), 60.0)
+        beam.setNumberOfParticlesPerExposure(3_000_000)
+    else:
+        ph = small_body()
+        beam = dx.CTSpiralBeam((0, 0, -12), (0, 0, 12), {13: 6.0})
+        beam.setCollimation(4.0)
+        beam.setStepAngleDeg(10.0)
+        beam.setNumberOfParticlesPerExposure(4_000_000 // beam.numberOfExposures())
+    settings = [
+        {},
+        {"DXG_REFILL": "1", "DXG_INTERACT": "1", "DXG_OUTERT": "1", "DXG_BATCH": "64"},
+        {"DXG_REFILL": "64", "DXG_INTERACT": "64", "DXG_OUTERT": "64", "DXG_BATCH": "4096"},
+        {"DXG_REFILL": "8", "DXG_INTERACT": "40", "DXG_OUTERT": "4", "DXG_TALLY_MERGE": "0" if small_grid else "1"},
+    ]
+    ref = None
+    for env in settings:
+        world, grid = make_world(ph, 1)
+        os.environ.update(env)
+        try:
+            assert dx.Transport(seed=23)(world, beam, None, False)
+        finally:
+            for k in env:
+                os.environ.pop(k, None)
+        c = grid.counters()
+        e, e2, n = grid.energyScored()
+        print(f"[settings] {env or 'defaults'}: refill {c['refill_threshold']}, interact {c['interact_threshold']}, "
+              f"outer {c['outer_threshold']}, batch {c['batch']}, steps {c['steps']}, deposits {c['deposits']}", flush=True)
+        got = ({k: c[k] for k in ("histories", "steps", "real_events", "deposits", "uncollided")}, e, e2, n)
+        if ref is None:
+            ref = got
+            continue
+        assert got[0] == ref[0], (env, got[0], ref[0])
+        np.testing.assert_array_equal(got[3], ref[3])
+        np.testing.assert_allclose(got[1], ref[1], rtol=1e-9, atol=1e-12 * ref[1].max())
+        np.testing.assert_allclose(got[2], ref[2], rtol=1e-9, atol=1e-12 * ref[2].max())
+
+
 def test_autotuned_launch_settings_leave_results_unchanged():
     """The first run of a beam layout on a world build times each candidate
     launch setting on short launches and keeps the fastest; the events of the
