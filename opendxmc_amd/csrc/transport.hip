@@ -60,7 +60,6 @@ constexpr int kRayInvNd = 1024;
 constexpr float kInvHc = 1.0f / 12.398419843320026f;
 constexpr float kInvMeC2 = 1.0f / 510.99895f;
 constexpr float kCutoff = 1.0f;
-constexpr float kTwoPi = 6.283185307179586f;
 // Work-group size and the occupancy target of the transport kernels
 // (compile-time).  Six wavefronts per SIMD is what the LDS tables allow
 // (about 26 KB per work-group of 256); the target keeps the macro-cell
@@ -234,11 +233,20 @@ __device__ inline void clipSlab(float p, float d, float id, float lo, float hi, 
     }
 }
 
-__device__ inline void rotate(float& dx, float& dy, float& dz, float cost, float phi)
+// Sine and cosine of 2 pi u (u in revolutions, |u| <= 256): the hardware's
+// v_sin_f32 / v_cos_f32 take revolutions, so no scaling by 2 pi and back.
+__device__ inline void sinCosRev(float u, float& s, float& c)
+{
+    s = __builtin_amdgcn_sinf(u);
+    c = __builtin_amdgcn_cosf(u);
+}
+
+// Turns the direction by the polar cosine cost and the azimuth 2 pi u.
+__device__ inline void rotate(float& dx, float& dy, float& dz, float cost, float u)
 {
     const float sint = hwSqrt(fmaxf(0.0f, 1.0f - cost * cost));
     float sp, cp;
-    __sincosf(phi, &sp, &cp);
+    sinCosRev(u, sp, cp);
     float nx, ny, nz;
     if (fabsf(dz) > 0.99999f) {
         nx = sint * cp;
@@ -876,7 +884,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         const float ct = 2.0f * rngU(rng) - 1.0f;
                         const float st = hwSqrt(fmaxf(0.0f, 1.0f - ct * ct));
                         float sp, cp;
-                        __sincosf(kTwoPi * rngU(rng), &sp, &cp);
+                        sinCosRev(rngU(rng), sp, cp);
                         dx = st * cp;
                         dy = st * sp;
                         dz = ct;
@@ -902,7 +910,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         eps = hwSqrt(e0 * e0 + (1.0f - e0 * e0) * r2);
                     const float t = (1.0f - eps) * hwRcp(k * eps);
                     const float sin2 = t * (2.0f - t);
-                    const float g = 1.0f - eps * sin2 / (1.0f + eps * eps);
+                    const float g = 1.0f - eps * sin2 * hwRcp(1.0f + eps * eps);
                     bool ok = !(r3 > g);
                     if (ok) {
                         cost = 1.0f - t;
@@ -984,7 +992,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         const float fy = rngU(rng) * amax * static_cast<float>(kRayInvNd - 1);
                         const int kk = min(static_cast<int>(fy), kRayInvNd - 2);
                         const float u = fminf(umax, lerpTab(P.ray_inv + m * kRayInvNd, kk, fy - static_cast<float>(kk)) * (kXMaxf * kXMaxf));
-                        cost = 1.0f - 2.0f * u / umax;
+                        cost = 1.0f - 2.0f * u * hwRcp(umax);
                         ok = 2.0f * rngU(rng) <= 1.0f + cost * cost;
                     }
                     if (ok)
@@ -995,7 +1003,7 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                 // Compton and Rayleigh lanes turn together (the azimuth is the
                 // last draw of both processes)
                 if (scatter_cost <= 1.0f)
-                    rotate(dx, dy, dz, scatter_cost, kTwoPi * rngU(rng));
+                    rotate(dx, dy, dz, scatter_cost, rngU(rng));
             }
             tallyFlush(P, tally, deps, vi, e1, e2, s_merge);
             if (deps > 0)
