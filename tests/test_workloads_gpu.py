@@ -165,15 +165,16 @@ def test_full_ct_spiral_full_size_properties():
 
 def test_shipped_denoiser_on_held_out_phantom():
     """The shipped network on a phantom seed it was not trained on (training
-    used seeds 1000-1011): at 4e8 histories the denoised error against a
+    used seeds 1000-1023): at 4e8 histories the denoised error against a
     6e10-history reference (root-mean-square relative error over body voxels
     above 20 % of the maximum, with the reference's own variance removed) is
-    below 3.5 % and below half the plain run's."""
+    below 2.5 % and below a fifth of the plain run's (measured: 1.8 % against
+    21 %)."""
     ph = phantoms.ct_body((512, 512, 400), (0.08, 0.08, 0.08), seed=7)
     net = dn.Denoiser.pretrained()
     ev = study.evaluate(net, 6e10, (4e8,), (4e8,), seed=7, ph=ph,
                         log=lambda m: print(f"[c5 denoiser] {m}", flush=True))
     plain, den = ev["plain"][0], ev["denoised"][0]
     print(f"[c5 denoiser] plain {plain['error_rms_corrected']:.4f}, denoised {den['error_rms_corrected']:.4f}", flush=True)
-    assert den["error_rms_corrected"] < 0.035
-    assert den["error_rms_corrected"] < 0.5 * plain["error_rms_corrected"]
+    assert den["error_rms_corrected"] < 0.025
+    assert den["error_rms_corrected"] < 0.2 * plain["error_rms_corrected"]
