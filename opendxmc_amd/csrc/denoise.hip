@@ -397,9 +397,28 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_kernel(ConvArgs a)
 // bit 1 row y + 1 (a pair index counts (n, z, y / 2, x / 64) like a tile
 // index with ceil(ny / 2) rows); without a list every pair is computed and
 // row y + 1 is stored when it exists.
-template <int R>
+// The epilogue's configuration as template bits (EPI >= 0), so that a layer's
+// code holds only its own epilogue: the runtime tests of the residual, gate,
+// activation and output kind cost instructions (and branches) on every store,
+// and the wavefront's instruction fetch is what the convolution waits on.
+// EPI < 0 reads the configuration from the arguments.
+constexpr int kEpiRelu = 1, kEpiRes = 2, kEpiGate = 4, kEpiY = 8, kEpiY32 = 16, kEpiY32All = 32;
+
+template <int EPI>
+__device__ inline bool epiHas(int flag, bool runtime)
+{
+    return EPI < 0 ? runtime : (EPI & flag) != 0;
+}
+
+template <int R, int EPI>
 __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu(2))) void conv3d_rows_kernel(ConvArgs a)
 {
+    const bool e_relu = epiHas<EPI>(kEpiRelu, a.relu != 0);
+    const bool e_res = epiHas<EPI>(kEpiRes, a.res != nullptr);
+    const bool e_gate = epiHas<EPI>(kEpiGate, a.gate != nullptr);
+    const bool e_y = epiHas<EPI>(kEpiY, a.y != nullptr);
+    const bool e_y32 = epiHas<EPI>(kEpiY32 | kEpiY32All, a.y32 != nullptr);
+    const bool e_y32_all = epiHas<EPI>(kEpiY32All, a.y32_channels == kC);
     __shared__ u32x4 wl[kTaps * 32 * 4];   // 55,296 B
     const int tid = threadIdx.x;
     for (int i = tid; i < kTaps * 32 * 4; i += 64 * kConvWaves) {
@@ -485,8 +504,8 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
         // (up to two rows: more would not leave registers for the ring)
         constexpr bool kPrefetchSide = R <= 2;
         u32x2 side[kPrefetchSide ? R : 1][4][2];
-        const uint16_t* side_src = a.res ? a.res : a.gate;
-        if (kPrefetchSide && side_src) {
+        const uint16_t* side_src = e_res ? a.res : (e_gate ? a.gate : nullptr);
+        if (kPrefetchSide && (e_res || e_gate)) {
 #pragma unroll
             for (int o = 0; o < R; ++o) {
                 if (!((cmask >> o) & 1))
@@ -557,28 +576,28 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
                     for (int i = 0; i < 4; ++i)
                         out[i] = acc[o][m][j][i] + bias[m][i];
                     const u32x2 sv = kPrefetchSide ? side[kPrefetchSide ? o : 0][j][m]
-                                                   : (side_src ? *reinterpret_cast<const u32x2*>(side_src + v * kC + co0) : u32x2 { 0, 0 });
-                    if (a.res) {
+                                                   : ((e_res || e_gate) ? *reinterpret_cast<const u32x2*>(side_src + v * kC + co0) : u32x2 { 0, 0 });
+                    if (e_res) {
                         const u32x2 rv = sv;
                         out[0] += fromBf16(rv[0] & 0xffffu);
                         out[1] += fromBf16(rv[0] >> 16);
                         out[2] += fromBf16(rv[1] & 0xffffu);
                         out[3] += fromBf16(rv[1] >> 16);
                     }
-                    if (a.relu)
+                    if (e_relu)
                         for (int i = 0; i < 4; ++i)
                             out[i] = fmaxf(out[i], 0.0f);
-                    if (a.gate) {
-                        const u32x2 gv = a.res ? *reinterpret_cast<const u32x2*>(a.gate + v * kC + co0) : sv;
+                    if (e_gate) {
+                        const u32x2 gv = e_res ? *reinterpret_cast<const u32x2*>(a.gate + v * kC + co0) : sv;
                         const uint32_t h[4] = { gv[0] & 0xffffu, gv[0] >> 16, gv[1] & 0xffffu, gv[1] >> 16 };
                         for (int i = 0; i < 4; ++i)
                             if (h[i] == 0 || (h[i] & 0x8000u))
                                 out[i] = 0.0f;
                     }
-                    if (a.y)
+                    if (e_y)
                         *reinterpret_cast<u32x2*>(a.y + v * kC + co0) = u32x2 { pack2(out[0], out[1]), pack2(out[2], out[3]) };
-                    if (a.y32) {
-                        if (a.y32_channels == kC)
+                    if (e_y32) {
+                        if (e_y32_all)
                             *reinterpret_cast<f32x4*>(a.y32 + v * kC + co0) = f32x4 { out[0], out[1], out[2], out[3] };
                         else if (co0 == 0)
                             a.y32[v] = out[0];
@@ -588,6 +607,30 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
         }
         cur = nxt;
         cmask = nmask;
+    }
+}
+
+// The row kernel for a launch's epilogue: the configurations the network and
+// the probe use are compiled on their own; any other reads its flags at run time.
+const void* rowsKernel(const ConvArgs& a)
+{
+    const int epi = (a.relu ? kEpiRelu : 0) | (a.res ? kEpiRes : 0) | (a.gate ? kEpiGate : 0) | (a.y ? kEpiY : 0)
+                    | (a.y32 ? (a.y32_channels == kC ? kEpiY32All : kEpiY32) : 0);
+    switch (epi) {
+    case kEpiRelu | kEpiY:
+        return reinterpret_cast<const void*>(&conv3d_rows_kernel<kConvRows, kEpiRelu | kEpiY>);
+    case kEpiRes | kEpiRelu | kEpiY:
+        return reinterpret_cast<const void*>(&conv3d_rows_kernel<kConvRows, kEpiRes | kEpiRelu | kEpiY>);
+    case kEpiY32:
+        return reinterpret_cast<const void*>(&conv3d_rows_kernel<kConvRows, kEpiY32>);
+    case kEpiGate | kEpiY:
+        return reinterpret_cast<const void*>(&conv3d_rows_kernel<kConvRows, kEpiGate | kEpiY>);
+    case kEpiRes | kEpiGate | kEpiY:
+        return reinterpret_cast<const void*>(&conv3d_rows_kernel<kConvRows, kEpiRes | kEpiGate | kEpiY>);
+    case kEpiY32All:
+        return reinterpret_cast<const void*>(&conv3d_rows_kernel<kConvRows, kEpiY32All>);
+    default:
+        return reinterpret_cast<const void*>(&conv3d_rows_kernel<kConvRows, -1>);
     }
 }
 
@@ -1024,9 +1067,10 @@ hipError_t launchConv(ConvArgs a, const Dims& d, hipStream_t s)
     const int wg_tiles = (a.wave_tiles + kConvWaves - 1) / kConvWaves;
     const int grid = roundUp8(std::min(wg_tiles, 2 * computeUnits()));
     a.chunk = (wg_tiles + grid - 1) / grid;
-    if (pairs)
-        hipLaunchKernelGGL(conv3d_rows_kernel<DXG_CONV_ROWS>, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
-    else if (convShift())
+    if (pairs) {
+        void* args[] = { &a };
+        (void)hipLaunchKernel(rowsKernel(a), dim3(grid), dim3(64 * kConvWaves), args, 0, s);
+    } else if (convShift())
         hipLaunchKernelGGL(conv3d_kernel<true>, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
     else
         hipLaunchKernelGGL(conv3d_kernel<false>, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
