@@ -238,6 +238,14 @@ int dxg_debug_traverse(const dxg_world* w, uint64_t n_rays, const float* pos, co
  * number of particles.  Scores are not touched. */
 int dxg_debug_source(dxg_world* w, const dxg_beam* beam, uint64_t seed, uint64_t n, float* out);
 
+/* ---- Compton scattering angle as the transport kernels sample it
+ * (DXMClib's Klein-Nishina sampling in interactions<5, C>, absent from the
+ * reference tree; caller simulationpipeline.cpp:159-170): cos(theta) at
+ * energy_kev (1-200 keV) for each uniform u[i] in [0, 1), by the kernels' own
+ * inverse-CDF table and FP32 interpolation, evaluated on the host (no GPU
+ * needed).  The level-1 S(x) rejection is not applied. */
+int dxg_debug_compton_cos(double energy_kev, uint64_t n, const float* u, float* cos_out);
+
 /* ---- Profiling aid (no counterpart in the reference): `iters` launches of a
  * plain float4 copy kernel over `bytes` bytes (multiple of 16) between two
  * fresh device buffers.  Under a hardware-counter profiler its FETCH_SIZE and

@@ -71,6 +71,36 @@ struct BeamBlob {
 // three lanes as a single memory request.
 constexpr int kTallyRecord = 4;
 
+// Klein-Nishina inverse-CDF table (physics.hpp kKnE0 .. kKnN): rows at
+// E_r = 1 + 4 r keV, nodes at u_i = i / (kKnNDev - 1), entries {cos(u_i),
+// cos(u_{i+1}) - cos(u_i)}.
+constexpr int kKnRowsDev = 51;
+constexpr int kKnNDev = 128;
+constexpr float kKnInvDEDev = 0.25f;   // 1 / (4 keV)
+
+// cos(theta) of a Compton scatter at energy e (keV, 1 <= e <= 201) from a
+// uniform u in [0, 1): the inverse Klein-Nishina CDF, interpolated linearly in
+// u within the rows either side of e and then linearly in e.  Shared by the
+// transport kernels and the host-side table check, so both evaluate the same
+// FP32 operations.
+__host__ __device__ inline float knCos(const float2* tab, float e, float u)
+{
+#pragma clang fp contract(off)
+    const float fe = fminf((e - 1.0f) * kKnInvDEDev, static_cast<float>(kKnRowsDev - 1));
+    const int r0 = static_cast<int>(fe);
+    const int r = r0 < kKnRowsDev - 2 ? r0 : kKnRowsDev - 2;
+    const float te = fe - static_cast<float>(r);
+    const float fu = u * static_cast<float>(kKnNDev - 1);
+    const int i0 = static_cast<int>(fu);
+    const int i = i0 < kKnNDev - 2 ? i0 : kKnNDev - 2;
+    const float tu = fu - static_cast<float>(i);
+    const float2 a = tab[r * kKnNDev + i];
+    const float2 b = tab[(r + 1) * kKnNDev + i];
+    const float ca = fmaf(tu, a.y, a.x);
+    const float cb = fmaf(tu, b.y, b.x);
+    return fmaf(te, cb - ca, ca);
+}
+
 // Everything one transport launch reads.  Passed by value as a kernel argument.
 struct DevParams {
     // voxel grid of packed words, flat index x + nx (y + ny z)
@@ -97,6 +127,7 @@ struct DevParams {
     const float* s_table;
     const float* ray_fwd;
     const float* ray_inv;
+    const float2* kn_inv;   // Klein-Nishina inverse CDF, kKnRowsDev x kKnNDev (knCos)
     // correction level 2 (see physics.hpp DeviceTables)
     const float4* ia_shell;
     const float* ia_inv;

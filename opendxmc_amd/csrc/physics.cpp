@@ -219,6 +219,59 @@ bool makeMaterial(const PhysicsList& pl, const int32_t* Z, const double* w, uint
     return true;
 }
 
+// Klein-Nishina per unit cos(theta), up to a constant: P^2 (P + 1/P - sin^2)
+// with P = E'/E = 1 / (1 + k (1 - cos)), k = E / m_e c^2.  The cumulative
+// integral over cos in [-1, 1] is taken by Simpson's rule on kFine intervals,
+// and each node u_i is inverted by linear interpolation inside its interval
+// (interval widths of 2/kFine put the inversion error near 1e-8 in cos).
+void buildComptonTable(std::vector<float>& out)
+{
+    constexpr int kFine = 16384;
+    out.assign(static_cast<size_t>(kKnRows) * kKnN * 2, 0.f);
+    std::vector<double> cum(kFine + 1);
+    std::vector<double> node(kKnN);
+    for (int r = 0; r < kKnRows; ++r) {
+        const double k = (kKnE0 + r * kKnDE) / kMeC2KeV;
+        const auto pdf = [k](double c) {
+            const double P = 1.0 / (1.0 + k * (1.0 - c));
+            return P * P * (P + 1.0 / P - (1.0 - c * c));
+        };
+        const double h = 2.0 / kFine;
+        cum[0] = 0;
+        for (int b = 1; b <= kFine; ++b) {
+            const double c0 = -1.0 + (b - 1) * h;
+            cum[static_cast<size_t>(b)] = cum[static_cast<size_t>(b) - 1] + h / 6.0 * (pdf(c0) + 4.0 * pdf(c0 + 0.5 * h) + pdf(c0 + h));
+        }
+        const double tot = cum[kFine];
+        node[0] = -1.0;
+        node[kKnN - 1] = 1.0;
+        size_t b = 1;
+        for (int i = 1; i < kKnN - 1; ++i) {
+            const double target = tot * i / (kKnN - 1);
+            while (b < static_cast<size_t>(kFine) && cum[b] < target)
+                ++b;
+            const double span = cum[b] - cum[b - 1];
+            const double f = span > 0 ? (target - cum[b - 1]) / span : 0.0;
+            node[static_cast<size_t>(i)] = -1.0 + (static_cast<double>(b - 1) + f) * h;
+        }
+        for (int i = 0; i < kKnN; ++i) {
+            float* e = &out[(static_cast<size_t>(r) * kKnN + i) * 2];
+            e[0] = static_cast<float>(node[static_cast<size_t>(i)]);
+            e[1] = i + 1 < kKnN ? static_cast<float>(node[static_cast<size_t>(i) + 1]) - e[0] : 0.f;
+        }
+    }
+}
+
+const std::vector<float>& comptonTable()
+{
+    static const std::vector<float> t = [] {
+        std::vector<float> v;
+        buildComptonTable(v);
+        return v;
+    }();
+    return t;
+}
+
 void buildTables(const PhysicsList& pl, const std::vector<MixMaterial>& mats, const std::vector<double>& rho_max, int level,
                  DeviceTables& t)
 {
@@ -264,6 +317,7 @@ void buildTables(const PhysicsList& pl, const std::vector<MixMaterial>& mats, co
             t.region_gain[static_cast<size_t>(m)] =
                 std::max(t.region_gain[static_cast<size_t>(m)], hi / static_cast<double>(t.majorant[static_cast<size_t>(j)]));
         }
+    t.kn_inv = comptonTable();
     // Sampling tables for correction level 1.
     constexpr int kFine = 16384;
     t.s_table.assign(static_cast<size_t>(nm) * kSfN, 0.f);

@@ -25,6 +25,15 @@ constexpr double kXMax = 16.0;
 constexpr int kSfN = 1024;   // nodes of the S(x) and F^2 forward tables
 constexpr int kRayInvN = 1024;   // nodes of the inverse F^2 table
 
+// Klein-Nishina inverse-CDF table: rows at E_r = kKnE0 + r kKnDE (keV), r <
+// kKnRows; nodes at u_i = i / (kKnN - 1).  Entry (r, i) holds cos(theta) with
+// P(cos <= c) = u_i under the Klein-Nishina distribution at E_r, and the step
+// to node i + 1 (0 at the last node).
+constexpr double kKnE0 = 1.0;
+constexpr double kKnDE = 4.0;
+constexpr int kKnRows = 51;   // E_r up to kLutEMax + 1
+constexpr int kKnN = 128;
+
 // Photons below this energy deposit what is left and stop.
 constexpr double kCutoffKeV = 1.0;
 
@@ -86,6 +95,7 @@ struct DeviceTables {
     std::vector<float> s_table;    // [m][k] S(x_k)/S(inf), x_k = k kXMax/(kSfN-1)
     std::vector<float> ray_fwd;    // [m][k] A(x_k)/A(kXMax), A(x) = int_0^{x^2} F^2 du
     std::vector<float> ray_inv;    // [m][k] u(y_k)/kXMax^2, y_k = k/(kRayInvN-1)
+    std::vector<float> kn_inv;     // [r][i][2] Klein-Nishina inverse CDF (buildComptonTable)
     // correction level 2
     std::vector<float> ia_shell;   // [m][kIaShells][4]: cumulative electron fraction, binding (keV), p0 (m_e c), 0
     std::vector<float> ia_inv;     // [k] x(u_k) of the unit Compton profile (8/3pi)(1+x^2)^-3, u_k = k/(kIaInvN-1)
@@ -119,6 +129,12 @@ std::vector<IaShell> impulseShells(const PhysicsList& pl, const MixMaterial& m);
 
 // Unit Compton profile CDF, G(x) = int_-inf^x (8/3pi)(1+t^2)^-3 dt.
 double comptonProfileCdf(double x);
+
+// The Klein-Nishina inverse-CDF table, kKnRows x kKnN pairs (see kKnE0):
+// material-independent, integrated and inverted in FP64.
+void buildComptonTable(std::vector<float>& out);
+// The same table, built once per process.
+const std::vector<float>& comptonTable();
 
 void buildTables(const PhysicsList& pl, const std::vector<MixMaterial>& mats, const std::vector<double>& rho_max,
                  int level, DeviceTables& t);

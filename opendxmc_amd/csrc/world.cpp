@@ -104,6 +104,7 @@ struct dxg_world {
     DevBuf<float> d_mu_tot;
     DevBuf<float4> d_mu_parts;
     DevBuf<float> d_majorant, d_s, d_rfwd, d_rinv;
+    DevBuf<float2> d_kn;
     DevBuf<float4> d_ia_shell;
     DevBuf<float> d_ia_inv, d_fl_cum, d_fl_line;
     DevBuf<double> d_tally;   // one record of dxg::kTallyRecord doubles per voxel, `replicas` copies
@@ -220,6 +221,7 @@ void fillParams(const dxg_world* w, dxg::DevParams& p)
     p.s_table = w->d_s.ptr;
     p.ray_fwd = w->d_rfwd.ptr;
     p.ray_inv = w->d_rinv.ptr;
+    p.kn_inv = w->d_kn.ptr;
     p.ia_shell = w->d_ia_shell.ptr;
     p.ia_inv = w->d_ia_inv.ptr;
     p.fl_cum = w->d_fl_cum.ptr;
@@ -429,6 +431,13 @@ int dxg_world_build(dxg_world* w)
     HIP_TRY(w->d_s.upload(w->tables.s_table, w->stream));
     HIP_TRY(w->d_rfwd.upload(w->tables.ray_fwd, w->stream));
     HIP_TRY(w->d_rinv.upload(w->tables.ray_inv, w->stream));
+    {
+        static_assert(dxg::kKnRowsDev == dxg::kKnRows && dxg::kKnNDev == dxg::kKnN && dxg::kKnInvDEDev * dxg::kKnDE == 1.0,
+                      "Klein-Nishina table shape");
+        std::vector<float2> kn(w->tables.kn_inv.size() / 2);
+        std::memcpy(kn.data(), w->tables.kn_inv.data(), w->tables.kn_inv.size() * sizeof(float));
+        HIP_TRY(w->d_kn.upload(kn, w->stream));
+    }
     if (w->level == 2) {
         std::vector<float4> shells(w->tables.ia_shell.size() / 4);
         std::memcpy(shells.data(), w->tables.ia_shell.data(), w->tables.ia_shell.size() * sizeof(float));
@@ -1156,6 +1165,23 @@ void dxg_progress_message(const dxg_progress* p, char* buf, uint64_t cap)
     const size_t k = std::min<size_t>(m.size(), cap - 1);
     std::memcpy(buf, m.data(), k);
     buf[k] = '\0';
+}
+
+int dxg_debug_compton_cos(double energy_kev, uint64_t n, const float* u, float* cos_out)
+{
+    if ((!u && n) || (!cos_out && n))
+        return fail(DXG_ERR_ARGUMENT, "dxg_debug_compton_cos: null argument");
+    if (!(energy_kev >= dxg::kKnE0 && energy_kev <= dxg::kLutEMax))
+        return fail(DXG_ERR_ARGUMENT, "dxg_debug_compton_cos: energy outside 1-200 keV");
+    const auto& t = dxg::comptonTable();
+    const float2* tab = reinterpret_cast<const float2*>(t.data());
+    const float e = static_cast<float>(energy_kev);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (!(u[i] >= 0.0f && u[i] < 1.0f))
+            return fail(DXG_ERR_ARGUMENT, "dxg_debug_compton_cos: u must lie in [0, 1)");
+        cos_out[i] = dxg::knCos(tab, e, u[i]);
+    }
+    return DXG_OK;
 }
 
 int dxg_material_attenuation(const dxg_material* m, const double* energy_kev, uint64_t n, double* out)

@@ -767,6 +767,16 @@ def sample_source(world: World, beam, n: int, seed: int = 1) -> np.ndarray:
     return out
 
 
+def compton_cos(energy_kev: float, u) -> np.ndarray:
+    """cos(theta) of Compton scatters at energy_kev (1-200 keV) for uniforms u
+    in [0, 1), by the transport kernels' Klein-Nishina table and FP32
+    interpolation, evaluated on the host (no device needed)."""
+    u = np.ascontiguousarray(u, dtype=np.float32).ravel()
+    out = np.zeros_like(u)
+    N.check(N.lib().dxg_debug_compton_cos(float(energy_kev), u.size, N.ptr(u, C.c_float), N.ptr(out, C.c_float)))
+    return out
+
+
 class TransportBatch:
     """Many (world, beam) instances in one persistent device launch, replayed
     from a captured HIP graph (no counterpart in the reference).  Each

@@ -127,12 +127,15 @@ def ctdi_phantom(diameter: float):
 @pytest.mark.gpu
 def test_calibrated_axial_scan_reproduces_ctdiw():
     """A sequential (axial) scan calibrated to CTDIw = 10 mGy, run on the CTDI
-    phantom itself with another particle count, measures CTDIw = 10 mGy."""
+    phantom itself with another seed, measures CTDIw = 10 mGy.  2e6 particles
+    per exposure (7.2e7 per rotation) put the standard error of each run's
+    CTDIw near 0.2 %, so the 2 % bound is statistical noise only well past 5
+    standard errors (at 3e5 per exposure one seed pair landed at 2.0 %)."""
     beam = dx.CTSequentialBeam((0, 0, 0))
     beam.setCollimation(4.0)
     beam.setStepAngleDeg(10.0)
     beam.setBowtieFilter(read_filters()[DEFAULT_CT_BOWTIE])
-    beam.setNumberOfParticlesPerExposure(300_000)
+    beam.setNumberOfParticlesPerExposure(2_000_000)
     beam.setCTDIw(10.0)
     beam.setCTDIdiameter(32.0)
     f = beam.calibrationFactor()
