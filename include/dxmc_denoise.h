@@ -75,7 +75,9 @@ int dxg_denoiser_apply(dxg_denoiser* d, const dxg_world* w, double* dose_mgy);
  * bias ignored);  mode 2: dw[co][tap][ci] = sum_v dy[v][co] x_shift[v][ci]
  * with x = the input and `bias_or_dy` the output gradient (N x D x H x W x 32);
  * out then holds the 27648 weight gradients [co][tap][ci] and the 32 bias
- * gradients.  Modes 0 and 1 write N x D x H x W x 32 floats.  Inputs are rounded
+ * gradients.  Mode 3: the network's first layer, y = relu(conv(x) + bias)
+ * rounded to BF16, for x whose channels 4..31 are zero (the four features).
+ * Modes 0, 1 and 3 write N x D x H x W x 32 floats.  Inputs are rounded
  * to BF16 first; mode 2 needs a y extent that is a multiple of 4. */
 int dxg_conv3d_probe(int mode, const float* x, const float* w, const float* bias_or_dy, uint64_t n, const uint64_t dims[3],
                      float* out);

@@ -127,6 +127,7 @@ struct ConvArgs {
     int wave_tiles;          // tiles (or row pairs) to compute: nb*nz*ny*(nx/64), or the length of `tiles`
     int chunk;               // work-group tiles (4 wave tiles) per work-group
     const int* tiles;        // null: every tile; else the (ascending) tile indices to compute (pair entries: conv3d_pair_kernel)
+    int in4;                 // the network's first layer: x is the 4-channel compact input (8 bytes per voxel), conv3d_first_kernel
 };
 
 // Weight fragment (tap, out channel co, 16-B chunk q) lives at chunk
@@ -419,6 +420,9 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
     const bool e_y = epiHas<EPI>(kEpiY, a.y != nullptr);
     const bool e_y32 = epiHas<EPI>(kEpiY32 | kEpiY32All, a.y32 != nullptr);
     const bool e_y32_all = epiHas<EPI>(kEpiY32All, a.y32_channels == kC);
+    // the single-channel FP32 output (the last layer) needs output channels
+    // 0-15 only: the second channel half's MFMAs and weight reads are skipped
+    constexpr int kHalves = EPI == kEpiY32 ? 1 : 2;
     __shared__ u32x4 wl[kTaps * 32 * 4];   // 55,296 B
     const int tid = threadIdx.x;
     for (int i = tid; i < kTaps * 32 * 4; i += 64 * kConvWaves) {
@@ -551,10 +555,10 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
                     const int tap = dz * 9 + dy * 3 + dx + 1;
                     bf16x8 aop[2];
 #pragma unroll
-                    for (int m = 0; m < 2; ++m)
+                    for (int m = 0; m < kHalves; ++m)
                         aop[m] = asOperand(wl[wSlot(tap, 16 * m + lr, lq)]);
 #pragma unroll
-                    for (int m = 0; m < 2; ++m)
+                    for (int m = 0; m < kHalves; ++m)
 #pragma unroll
                         for (int j = 0; j < 4; ++j)
                             acc[o][m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop[m], bop[j], acc[o][m][j], 0, 0, 0);
@@ -570,7 +574,7 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
             for (int j = 0; j < 4; ++j) {
                 const size_t v = orow0 + static_cast<size_t>(o) * a.nx + x0 + 4 * lr + j;
 #pragma unroll
-                for (int m = 0; m < 2; ++m) {
+                for (int m = 0; m < kHalves; ++m) {
                     const int co0 = 16 * m + 4 * lq;
                     float out[4];
                     for (int i = 0; i < 4; ++i)
@@ -607,6 +611,149 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
         }
         cur = nxt;
         cmask = nmask;
+    }
+}
+
+// First layer: the input has four feature channels (in the 32-channel
+// tensor the other 28 are zero), so a 3x3x3 tap uses 4 of the 32 K-values of
+// an MFMA and the general kernel spends 7/8 of its MFMAs on zeros.  Here K
+// runs over (tap, channel): 27 taps x 4 channels = 108 values, padded to 128
+// = four MFMA K-chunks, so an output row costs 4 x 2 x 4 MFMAs instead of
+// 27 x 2 x 4.  The input is the features' compact copy (4 BF16 channels, 8
+// bytes per voxel).  Lane (lr, lq) holds K-values 8 lq .. 8 lq + 7 of a chunk
+// c: taps 8c + 2lq and 8c + 2lq + 1, read as two 8-byte loads from the
+// neighbouring voxels of voxel x0 + 16 j + lr (taps past 26 and voxels outside
+// the volume read as zero), so the 16 lanes of a quarter read 128 contiguous
+// bytes.  The weight fragments (the layer's 27 x 32 x 4 used weights) stay in
+// registers.  Same tiles (row pairs) and epilogue (bias, ReLU, BF16 output)
+// as conv3d_rows_kernel; results equal up to FP32 summation order.
+__global__ __launch_bounds__(64 * kConvWaves) void conv3d_first_kernel(ConvArgs a)
+{
+    constexpr int R = kConvRows;
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, lr = lane & 15, lq = lane >> 4;
+
+    // weight fragments: aw[m][c] = W[tap][co = 16 m + lr][ci] for the eight
+    // (tap, ci) of chunk c and quarter lq
+    bf16x8 aw[2][4];
+    // the lane's two taps per chunk: voxel offset from the tile row's corner
+    // (z - 1, y - 1, x0 - 1), so that every offset is non-negative; taps past
+    // 26 take the centre voxel and are dropped
+    const int corner = (a.ny + 1) * a.nx + 1;
+    unsigned toff[4][2];
+    int tdx[4][2], tdy[4][2], tdz[4][2];
+    bool tval[4][2];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int t = 8 * c + 2 * lq + k;
+            tval[c][k] = t < kTaps;
+            const int tt = tval[c][k] ? t : 13;
+            tdz[c][k] = tt / 9 - 1;
+            tdy[c][k] = (tt / 3) % 3 - 1;
+            tdx[c][k] = tt % 3 - 1;
+            toff[c][k] = static_cast<unsigned>(corner + (tdz[c][k] * a.ny + tdy[c][k]) * a.nx + tdx[c][k]);
+        }
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            u32x4 v;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                u32x2 w2 = { 0u, 0u };
+                if (tval[c][k])
+                    w2 = *reinterpret_cast<const u32x2*>(a.w + (static_cast<size_t>(8 * c + 2 * lq + k) * 32 + 16 * m + lr) * kC);
+                v[2 * k] = w2[0];
+                v[2 * k + 1] = w2[1];
+            }
+            aw[m][c] = asOperand(v);
+        }
+    }
+    float bias[2][4];
+    for (int m = 0; m < 2; ++m)
+        for (int i = 0; i < 4; ++i)
+            bias[m][i] = a.bias ? a.bias[16 * m + 4 * lq + i] : 0.0f;
+
+    const int g = gridDim.x;
+    const int logical = (blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
+    const int wg_tiles = (a.wave_tiles + kConvWaves - 1) / kConvWaves;
+    const int begin = logical * a.chunk;
+    const int end = min(begin + a.chunk, wg_tiles);
+    const int nxs = a.nx >> 6, nyp = (a.ny + R - 1) / R;
+    for (int wt = begin; wt < end; ++wt) {
+        const int kt = wt * kConvWaves + wave;
+        if (kt >= a.wave_tiles)
+            break;
+        int t = kt, mask;
+        if (a.tiles) {
+            t = a.tiles[kt] >> R;
+            mask = a.tiles[kt] & ((1 << R) - 1);
+        }
+        int r = t / nxs;
+        const int x0 = (t - r * nxs) * 64;
+        const int y = R * (r % nyp);
+        r /= nyp;
+        const int z = r % a.nz, n = r / a.nz;
+        if (!a.tiles)
+            mask = (1 << min(R, a.ny - y)) - 1;
+        const size_t plane = static_cast<size_t>(n) * a.nz * a.ny * a.nx;
+        const bool interior = x0 > 0 && x0 + 64 < a.nx;   // no tap of the row leaves the volume along x
+#pragma unroll
+        for (int o = 0; o < R; ++o) {
+            if (!((mask >> o) & 1))
+                continue;
+            const int yo = y + o;
+            const size_t orow = plane + (static_cast<size_t>(z) * a.ny + yo) * a.nx + x0;
+            // the row's corner voxel (may lie before the tensor: only offsets from it are read)
+            const uint16_t* base = a.x + (static_cast<long long>(orow) - corner) * 4;
+            bool yz[4][2];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+                    yz[c][k] = tval[c][k] && static_cast<unsigned>(yo + tdy[c][k]) < static_cast<unsigned>(a.ny)
+                               && static_cast<unsigned>(z + tdz[c][k]) < static_cast<unsigned>(a.nz);
+            f32x4 acc[2][4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int xl = 16 * j + lr;   // the output voxel within the row
+                bf16x8 bop[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    u32x4 v;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const bool in = yz[c][k] && (interior || static_cast<unsigned>(x0 + xl + tdx[c][k]) < static_cast<unsigned>(a.nx));
+                        const unsigned vo = static_cast<unsigned>(xl) + (in ? toff[c][k] : static_cast<unsigned>(corner));
+                        const u32x2 x2 = *reinterpret_cast<const u32x2*>(base + vo * 4u);
+                        v[2 * k] = in ? x2[0] : 0u;
+                        v[2 * k + 1] = in ? x2[1] : 0u;
+                    }
+                    bop[c] = asOperand(v);
+                }
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    acc[m][j] = f32x4 { 0, 0, 0, 0 };
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw[m][c], bop[c], acc[m][j], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const size_t v = orow + 16 * j + lr;
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    float out[4];
+                    for (int i = 0; i < 4; ++i) {
+                        out[i] = acc[m][j][i] + bias[m][i];
+                        if (a.relu)
+                            out[i] = fmaxf(out[i], 0.0f);
+                    }
+                    *reinterpret_cast<u32x2*>(a.y + v * kC + 16 * m + 4 * lq) = u32x2 { pack2(out[0], out[1]), pack2(out[2], out[3]) };
+                }
+            }
+        }
     }
 }
 
@@ -837,14 +984,16 @@ __global__ __launch_bounds__(256) void adam_kernel(float* p, float* m, float* v,
 
 // Features (4 FP32 per voxel) -> 32-channel BF16 input tensor and an FP32 copy
 // of feature 0 (the relative dose the network corrects).
-__global__ __launch_bounds__(256) void pack_features_kernel(const float* f, uint16_t* xin, float* feat0, size_t nv)
+__global__ __launch_bounds__(256) void pack_features_kernel(const float* f, uint16_t* xin, uint16_t* xin4, float* feat0, size_t nv)
 {
     const size_t v = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
     if (v >= nv)
         return;
     const f32x4 q = *reinterpret_cast<const f32x4*>(f + 4 * v);
     u32x4* dst = reinterpret_cast<u32x4*>(xin + v * kC);
-    dst[0] = u32x4 { pack2(q[0], q[1]), pack2(q[2], q[3]), 0u, 0u };
+    const u32x2 f4 = { pack2(q[0], q[1]), pack2(q[2], q[3]) };
+    dst[0] = u32x4 { f4[0], f4[1], 0u, 0u };
+    *reinterpret_cast<u32x2*>(xin4 + 4 * v) = f4;
     dst[1] = u32x4 { 0u, 0u, 0u, 0u };
     dst[2] = u32x4 { 0u, 0u, 0u, 0u };
     dst[3] = u32x4 { 0u, 0u, 0u, 0u };
@@ -941,6 +1090,7 @@ struct WorldFeatArgs {
     double histories;
     const double* acc;      // dose_mean_kernel sums
     uint16_t* xin;          // 32-channel BF16 network input
+    uint16_t* xin4;         // its four feature channels alone (8 bytes per voxel)
     float* feat0;
     float* features;        // optional FP32 copy (nvp x 4)
 };
@@ -975,7 +1125,9 @@ __global__ __launch_bounds__(256) void world_features_kernel(WorldFeatArgs a)
         f[3] = rel;
     }
     u32x4* dst = reinterpret_cast<u32x4*>(a.xin + v * kC);
-    dst[0] = u32x4 { pack2(f[0], f[1]), pack2(f[2], f[3]), 0u, 0u };
+    const u32x2 f4 = { pack2(f[0], f[1]), pack2(f[2], f[3]) };
+    dst[0] = u32x4 { f4[0], f4[1], 0u, 0u };
+    *reinterpret_cast<u32x2*>(a.xin4 + 4 * v) = f4;
     dst[1] = u32x4 { 0u, 0u, 0u, 0u };
     dst[2] = u32x4 { 0u, 0u, 0u, 0u };
     dst[3] = u32x4 { 0u, 0u, 0u, 0u };
@@ -1035,10 +1187,28 @@ bool convPairs()
     return on;
 }
 
+// The first layer by conv3d_first_kernel (its four input channels packed
+// over the taps) unless DXG_CONV_FIRST=0.
+int convFirst()
+{
+    static const int on = [] {
+        const char* e = std::getenv("DXG_CONV_FIRST");
+        return !(e && e[0] == '0') ? 1 : 0;
+    }();
+    return on;
+}
+
 struct Dims {
     int nb, nz, ny, nx;
     size_t voxels() const { return static_cast<size_t>(nb) * nz * ny * nx; }
 };
+
+// Whether the first layer runs conv3d_first_kernel on the compact input: row
+// pairs on, and a plane small enough for its 32-bit offsets.
+bool firstLayerCompact(const Dims& d)
+{
+    return convFirst() && convPairs() && static_cast<long long>(d.ny + 1) * d.nx < (1ll << 27);
+}
 
 int checkDims(const char* fn, uint64_t batch, const uint64_t* dims, Dims& d)
 {
@@ -1067,7 +1237,14 @@ hipError_t launchConv(ConvArgs a, const Dims& d, hipStream_t s)
     const int wg_tiles = (a.wave_tiles + kConvWaves - 1) / kConvWaves;
     const int grid = roundUp8(std::min(wg_tiles, 2 * computeUnits()));
     a.chunk = (wg_tiles + grid - 1) / grid;
-    if (pairs) {
+    // the first layer reads the compact 4-channel input: only its own kernel
+    // can (its 32-bit element offsets from a row's corner stay below 2^31
+    // while (ny + 1) nx < 2^27)
+    if (a.in4) {
+        if (!pairs || !a.y || a.y32 || a.res || a.gate || static_cast<long long>(d.ny + 1) * d.nx >= (1ll << 27))
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL(conv3d_first_kernel, dim3(grid), dim3(64 * kConvWaves), 0, s, a);
+    } else if (pairs) {
         void* args[] = { &a };
         (void)hipLaunchKernel(rowsKernel(a), dim3(grid), dim3(64 * kConvWaves), args, 0, s);
     } else if (convShift())
@@ -1221,7 +1398,7 @@ struct dxg_denoiser {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DBuf<float> params, adam_m, adam_v, grads;
     DBuf<uint16_t> wf, wd;
-    DBuf<uint16_t> xin, acts, gbuf;
+    DBuf<uint16_t> xin, xin4, acts, gbuf;
     DBuf<float> feat0, y32, target, mask, features, lpart, part, part_b, sums, wemb;
     DBuf<double> wacc, wdose;
     DBuf<int> tiles;
@@ -1264,10 +1441,11 @@ struct dxg_denoiser {
         const size_t nv = d.voxels();
         HIP_TRY(features.reserve(4 * nv));
         HIP_TRY(xin.reserve(nv * kC));
+        HIP_TRY(xin4.reserve(nv * 4));
         HIP_TRY(feat0.reserve(nv));
         HIP_TRY(y32.reserve(nv));
         HIP_TRY(hipMemcpyAsync(features.p, features_host, 4 * nv * sizeof(float), hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(pack_features_kernel, dim3((nv + 255) / 256), dim3(256), 0, stream, features.p, xin.p, feat0.p, nv);
+        hipLaunchKernelGGL(pack_features_kernel, dim3((nv + 255) / 256), dim3(256), 0, stream, features.p, xin.p, xin4.p, feat0.p, nv);
         HIP_TRY(hipGetLastError());
         return DXG_OK;
     }
@@ -1287,7 +1465,8 @@ struct dxg_denoiser {
         auto aT = [&](int b) { return acts.p + (keep ? static_cast<size_t>(b) : 0) * ts; };
         auto hT = [&](int b) { return acts.p + (keep ? static_cast<size_t>(9 + b) : 1) * ts; };
         ConvArgs c {};
-        c.x = xin.p;
+        c.in4 = firstLayerCompact(d);   // the four features, packed over the taps
+        c.x = c.in4 ? xin4.p : xin.p;
         c.w = layerW(0);
         c.bias = layerBias(0);
         c.y = aT(0);
@@ -1345,6 +1524,7 @@ struct dxg_denoiser {
         const size_t nvp = dm.voxels();
         const size_t nv = static_cast<size_t>(v.nx) * v.ny * v.nz;
         HIP_TRY(xin.reserve(nvp * kC));
+        HIP_TRY(xin4.reserve(nvp * 4));
         HIP_TRY(feat0.reserve(nvp));
         HIP_TRY(y32.reserve(nvp));
         HIP_TRY(wacc.reserve(2));
@@ -1366,6 +1546,7 @@ struct dxg_denoiser {
         a.histories = static_cast<double>(v.histories);
         a.acc = wacc.p;
         a.xin = xin.p;
+        a.xin4 = xin4.p;
         a.feat0 = feat0.p;
         a.features = keep_fp32 ? features.p : nullptr;
         hipLaunchKernelGGL(world_features_kernel, dim3((nvp + 255) / 256), dim3(256), 0, stream, a);
@@ -1709,8 +1890,8 @@ int dxg_conv3d_probe(int mode, const float* x, const float* w, const float* bias
 {
     if (!x || !w || !out || (mode != 1 && !bias_or_dy))
         return fail(DXG_ERR_ARGUMENT, "dxg_conv3d_probe: null argument");
-    if (mode < 0 || mode > 2)
-        return fail(DXG_ERR_ARGUMENT, "dxg_conv3d_probe: mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3)
+        return fail(DXG_ERR_ARGUMENT, "dxg_conv3d_probe: mode must be 0, 1, 2 or 3");
     Dims dm {};
     if (int rc = checkDims("dxg_conv3d_probe", n, dims, dm); rc != DXG_OK)
         return rc;
@@ -1724,7 +1905,7 @@ int dxg_conv3d_probe(int mode, const float* x, const float* w, const float* bias
         xh[i] = toBf16(x[i]);
     std::vector<float> layer(kLayerP, 0.0f);
     std::copy(w, w + kLayerW, layer.begin());
-    if (mode == 0)
+    if (mode == 0 || mode == 3)
         std::copy(bias_or_dy, bias_or_dy + kC, layer.begin() + kLayerW);
     DBuf<uint16_t> dx, dy, wf, wd;
     DBuf<float> dp, dout, part, part_b;
@@ -1750,6 +1931,35 @@ int dxg_conv3d_probe(int mode, const float* x, const float* w, const float* bias
         HIP_TRY(sums.reserve(static_cast<size_t>(kReduceSlices) * kLayerP));
         HIP_TRY(launchWgrad(dx.p, dy.p, part.p, part_b.p, sums.p, dout.p, dm, nullptr));
         HIP_TRY(hipMemcpy(out, dout.p, kLayerP * sizeof(float), hipMemcpyDeviceToHost));
+        return DXG_OK;
+    }
+    if (mode == 3) {
+        // the network's first layer: channels 4..31 of x must be zero
+        for (size_t i = 0; i < ts; ++i)
+            if (i % kC >= 4 && x[i] != 0.0f)
+                return fail(DXG_ERR_ARGUMENT, "dxg_conv3d_probe: mode 3 needs channels 4..31 of x to be zero");
+        if (!firstLayerCompact(dm))
+            return fail(DXG_ERR_ARGUMENT, "dxg_conv3d_probe: mode 3 needs row pairs and a plane below 2^27 voxels");
+        std::vector<uint16_t> x4(nv * 4);
+        for (size_t v = 0; v < nv; ++v)
+            for (int k = 0; k < 4; ++k)
+                x4[4 * v + k] = xh[v * kC + k];
+        DBuf<uint16_t> yb, dx4;
+        HIP_TRY(yb.reserve(ts));
+        HIP_TRY(dx4.reserve(nv * 4));
+        HIP_TRY(hipMemcpy(dx4.p, x4.data(), nv * 4 * sizeof(uint16_t), hipMemcpyHostToDevice));
+        ConvArgs c {};
+        c.x = dx4.p;
+        c.w = wf.p;
+        c.bias = dp.p + kLayerW;
+        c.y = yb.p;
+        c.relu = 1;
+        c.in4 = 1;
+        HIP_TRY(launchConv(c, dm, nullptr));
+        std::vector<uint16_t> yh(ts);
+        HIP_TRY(hipMemcpy(yh.data(), yb.p, ts * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < ts; ++i)
+            out[i] = fromBf16(yh[i]);
         return DXG_OK;
     }
     HIP_TRY(dout.reserve(ts));

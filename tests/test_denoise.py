@@ -61,7 +61,7 @@ def test_probe_rejects_bad_shapes():
         _probe(2, x, w, x, 1, (2, 6, 64))
     assert e.value.code == N.DXG_ERR_ARGUMENT and "multiple of 4" in str(e.value)
     with pytest.raises(N.NativeError) as e:
-        N.check(N.lib().dxg_conv3d_probe(3, _pf(x), _pf(w), _pf(b), 1,
+        N.check(N.lib().dxg_conv3d_probe(4, _pf(x), _pf(w), _pf(b), 1,
                                          np.array([2, 6, 64], dtype=np.uint64).ctypes.data_as(C.POINTER(N.u64)),
                                          _pf(x)))
     assert e.value.code == N.DXG_ERR_ARGUMENT
@@ -175,6 +175,29 @@ def test_conv_forward_matches_torch(n, dims):
     want = ref.conv(torch.from_numpy(_bf16(x)), _torch_w(w), torch.from_numpy(b)).numpy()
     # FP32 accumulation of 864 BF16 products in another order: ~1e-6 relative
     np.testing.assert_allclose(out, want, rtol=0, atol=2e-5 * np.abs(want).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,dims", CONV_CASES + [(1, (3, 5, 192))])
+def test_first_layer_kernel_matches_torch(n, dims):
+    """The first layer's kernel (four input channels packed over the taps):
+    relu(conv + bias) rounded to BF16, against torch on the same BF16 inputs.
+    The extra case has an odd row count (a half-filled row pair) and an x tile
+    with neighbours on both sides."""
+    x, w, b = _conv_inputs(15, n, dims)
+    x[..., 4:] = 0.0
+    out = _probe(3, x, w.reshape(-1), b, n, dims).reshape(x.shape)
+    want = torch.relu(ref.conv(torch.from_numpy(_bf16(x)), _torch_w(w), torch.from_numpy(b))).numpy()
+    # FP32 sums of 108 products in another order, then BF16 rounding: within one
+    # BF16 step (2^-8 relative) of the rounded reference
+    want_bf = _bf16(want)
+    tol = np.abs(want_bf) * 2.0**-7 + 2e-5 * np.abs(want).max()
+    bad = np.abs(out - want_bf) > tol
+    assert not bad.any(), f"{int(bad.sum())} of {out.size} outputs off"
+    # and the channels feed it: an input in channel 3 alone changes the output
+    x2 = x.copy()
+    x2[..., 3] += 1.0
+    assert np.abs(_probe(3, x2, w.reshape(-1), b, n, dims) - out.ravel()).max() > 1e-3
 
 
 @pytest.mark.gpu
