@@ -40,14 +40,23 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-PROFILE = ROOT / "profiles" / "transport_kernel_ct_spiral_1e8.md"
+# the committed transport profiles with measured fabric traffic, per workload
+TRAFFIC_PROFILES = {
+    "ct-spiral-256": "profiles/transport_kernel_ct_spiral_1e8.md",
+    "pencil-64": "profiles/transport_kernel_pencil_64_1e8.md",
+    "ct-spiral-512-dual": "profiles/transport_kernel_ct_spiral_dual_512x512x400.md",
+}
 
 
-def measured_traffic_per_history():
-    """Fabric traffic per history from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes (copy-kernel calibrated); None when not profiled."""
+def measured_traffic_per_history(workload: str = "ct-spiral-256"):
+    """Fabric traffic per history of a workload from its committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (copy-kernel calibrated); None when not
+    profiled."""
+    rel = TRAFFIC_PROFILES.get(workload)
+    if rel is None:
+        return None
     try:
-        for line in PROFILE.read_text().splitlines():
+        for line in (ROOT / rel).read_text().splitlines():
             if line.startswith("traffic_bytes_per_history:"):
                 return float(line.split(":", 1)[1])
     except OSError:
@@ -425,8 +434,7 @@ def main() -> int:
     wall = time.perf_counter() - t0
     bytes_alg = 5.0 * steps + 24.0 * deposits
     achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
-    # the committed counter profile covers the default workload only
-    tph = measured_traffic_per_history() if args.workload == "ct-spiral-256" else None
+    tph = measured_traffic_per_history(args.workload)
     hist_per_launch = hist / max(launches, 1)
     traffic = tph * hist_per_launch if tph is not None else None
     result = {
@@ -460,8 +468,8 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "traffic_source": "profiles/transport_kernel_ct_spiral_1e8.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
-                              "per history, times histories per launch)",
+            "traffic_source": (f"{TRAFFIC_PROFILES[args.workload]} (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, per history, "
+                               "times histories per launch)") if traffic is not None else None,
             "kernel": "transport_kernel",
             "kernel_ms_per_launch": kernel_ms / max(launches, 1),
             "bytes_per_history": bytes_alg / max(hist, 1),

@@ -26,6 +26,13 @@ def test_more_than_one_gpu_exits_with_a_message():
 def test_transport_traffic_per_history_is_read_from_the_profile():
     t = bench.measured_traffic_per_history()
     assert t is not None and 50.0 < t < 5000.0
+    # every profiled workload has its own figure; the C5 grid misses the cache
+    # far more often than C2's, the uniform pencil cube reads no voxel at all
+    per = {w: bench.measured_traffic_per_history(w) for w in bench.TRAFFIC_PROFILES}
+    assert per["ct-spiral-256"] == t
+    assert per["ct-spiral-512-dual"] > 2 * t
+    assert 0.0 < per["pencil-64"] < t
+    assert bench.measured_traffic_per_history("ct-segmentation") is None
 
 
 def test_host_cpu_description():
