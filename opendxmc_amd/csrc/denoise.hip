@@ -1364,6 +1364,121 @@ std::vector<int> pairTiles(const std::vector<int>& tiles, int ny, int nxt)
     return out;
 }
 
+// ---- the row-pair tile list on the device (the same list as pairTiles(bodyTiles(...)))
+
+// flags[r * nxt + t] = 1 where tile t of row r (64 voxels along x) holds a
+// voxel of material != 0: one wavefront per tile.
+__global__ __launch_bounds__(256) void tile_flags_kernel(const uint32_t* voxels, int nx, int nxt, size_t tiles, uint8_t* flags)
+{
+    const size_t w = (static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    if (w >= tiles)
+        return;   // the whole wavefront
+    const size_t r = w / static_cast<size_t>(nxt);
+    const int x = static_cast<int>(w - r * nxt) * 64 + lane;
+    const bool body = x < nx && (voxels[r * static_cast<size_t>(nx) + x] & 0xFFu) != 0;
+    const unsigned long long any = __ballot(body);
+    if (lane == 0)
+        flags[w] = any != 0;
+}
+
+// out[(z, y, t)] = OR of src over t +- dt, y +- dy, z +- dz (clamped to the volume)
+__global__ __launch_bounds__(256) void tile_dilate_kernel(const uint8_t* src, uint8_t* out, int nz, int ny, int nxt, int dz, int dy, int dt)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    const size_t n = static_cast<size_t>(nz) * ny * nxt;
+    if (i >= n)
+        return;
+    const int t = static_cast<int>(i % nxt);
+    const size_t r = i / nxt;
+    const int y = static_cast<int>(r % ny), z = static_cast<int>(r / ny);
+    uint8_t v = 0;
+    for (int zz = max(z - dz, 0); zz <= min(z + dz, nz - 1) && !v; ++zz)
+        for (int yy = max(y - dy, 0); yy <= min(y + dy, ny - 1) && !v; ++yy)
+            for (int tt = max(t - dt, 0); tt <= min(t + dt, nxt - 1); ++tt)
+                v |= src[(static_cast<size_t>(zz) * ny + yy) * nxt + tt];
+    out[i] = v;
+}
+
+// The store mask of row pair g = (z * nyp + yp) * nxt + t: bit o for row 2 yp + o.
+__device__ inline int pairMask(const uint8_t* f, size_t g, int ny, int nyp, int nxt)
+{
+    const int t = static_cast<int>(g % nxt);
+    const size_t q = g / nxt;
+    const int yp = static_cast<int>(q % nyp);
+    const size_t z = q / nyp;
+    const size_t row0 = z * ny + 2 * yp;
+    int m = f[row0 * nxt + t];
+    if (2 * yp + 1 < ny)
+        m |= f[(row0 + 1) * nxt + t] << 1;
+    return m;
+}
+
+// Row pairs with a row to store, per 256-pair block: counts, then the blocks'
+// exclusive offsets (one work-group, in order), then the entries (g << 2 | mask)
+// written in ascending order.
+__global__ __launch_bounds__(256) void pair_count_kernel(const uint8_t* f, size_t groups, int ny, int nyp, int nxt, int* counts)
+{
+    const size_t g = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int keep = g < groups && pairMask(f, g, ny, nyp, nxt) != 0;
+    const int n = __syncthreads_count(keep);
+    if (threadIdx.x == 0)
+        counts[blockIdx.x] = n;
+}
+
+__global__ __launch_bounds__(256) void pair_scan_kernel(const int* counts, int blocks, int* offsets, int* total)
+{
+    __shared__ int s_wave[4];
+    __shared__ int s_base;
+    if (threadIdx.x == 0)
+        s_base = 0;
+    __syncthreads();
+    const int lane = static_cast<int>(threadIdx.x & 63), wave = static_cast<int>(threadIdx.x >> 6);
+    for (int b0 = 0; b0 < blocks; b0 += 256) {
+        const int b = b0 + static_cast<int>(threadIdx.x);
+        const int c = b < blocks ? counts[b] : 0;
+        // inclusive scan within the wavefront
+        int x = c;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d);
+            if (lane >= d)
+                x += y;
+        }
+        if (lane == 63)
+            s_wave[wave] = x;
+        __syncthreads();
+        int before = s_base;
+        for (int k = 0; k < wave; ++k)
+            before += s_wave[k];
+        if (b < blocks)
+            offsets[b] = before + x - c;
+        __syncthreads();
+        if (threadIdx.x == 255)
+            s_base = before + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *total = s_base;
+}
+
+__global__ __launch_bounds__(256) void pair_write_kernel(const uint8_t* f, size_t groups, int ny, int nyp, int nxt, const int* offsets,
+                                                         int* list)
+{
+    __shared__ int s_wave[4];
+    const size_t g = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int m = g < groups ? pairMask(f, g, ny, nyp, nxt) : 0;
+    const unsigned long long bits = __ballot(m != 0);
+    const int lane = static_cast<int>(threadIdx.x & 63), wave = static_cast<int>(threadIdx.x >> 6);
+    if (lane == 0)
+        s_wave[wave] = __popcll(bits);
+    __syncthreads();
+    int base = offsets[blockIdx.x];
+    for (int k = 0; k < wave; ++k)
+        base += s_wave[k];
+    if (m != 0)
+        list[base + __popcll(bits & ((1ull << lane) - 1ull))] = static_cast<int>(g << kConvRows) | m;
+}
+
 template <typename T>
 struct DBuf {
     T* p = nullptr;
@@ -1402,6 +1517,8 @@ struct dxg_denoiser {
     DBuf<float> feat0, y32, target, mask, features, lpart, part, part_b, sums, wemb;
     DBuf<double> wacc, wdose;
     DBuf<int> tiles;
+    DBuf<uint8_t> tflag_a, tflag_b;     // device tile list: tile flags before / after dilation
+    DBuf<int> tcounts, toffsets, ttotal;
     uint64_t tiles_generation = 0;   // world generation the tile list belongs to
     bool tiles_zeroed = false;       // acts and y32 are zero outside the tile list ...
     const void* zeroed_acts = nullptr;   // ... in these allocations
@@ -1504,6 +1621,60 @@ struct dxg_denoiser {
         c.tiles = tile_list;
         c.wave_tiles = tile_count;
         HIP_TRY(launchConv(c, d, stream));
+        return DXG_OK;
+    }
+
+    // The row-pair list of the world's body tiles, built on the device: the
+    // same list as pairTiles(bodyTiles(...)) (tile flags from the material
+    // bytes of the voxel words, box dilation by one tile along x and kReach
+    // rows and planes, ordered compaction of the pairs with a row to store).
+    // n receives its length.
+    int devicePairTiles(const dxg::WorldView& v, const Dims& dm, int& n)
+    {
+        static_assert(kConvRows == 2, "pair masks of two rows");
+        const int nxt = dm.nx / 64, nyp = (v.ny + 1) / 2;
+        const size_t ntiles = static_cast<size_t>(v.nz) * v.ny * nxt;
+        const size_t groups = static_cast<size_t>(v.nz) * nyp * nxt;
+        const int blocks = static_cast<int>((groups + 255) / 256);
+        HIP_TRY(tflag_a.reserve(ntiles));
+        HIP_TRY(tflag_b.reserve(ntiles));
+        HIP_TRY(tcounts.reserve(static_cast<size_t>(blocks)));
+        HIP_TRY(toffsets.reserve(static_cast<size_t>(blocks)));
+        HIP_TRY(ttotal.reserve(1));
+        HIP_TRY(tiles.reserve(std::max<size_t>(groups, 1)));
+        hipLaunchKernelGGL(tile_flags_kernel, dim3(static_cast<unsigned>((ntiles * 64 + 255) / 256)), dim3(256), 0, stream, v.voxels,
+                           v.nx, nxt, ntiles, tflag_a.p);
+        const dim3 tgrid(static_cast<unsigned>((ntiles + 255) / 256));
+        hipLaunchKernelGGL(tile_dilate_kernel, tgrid, dim3(256), 0, stream, tflag_a.p, tflag_b.p, v.nz, v.ny, nxt, 0, kReach, 1);
+        hipLaunchKernelGGL(tile_dilate_kernel, tgrid, dim3(256), 0, stream, tflag_b.p, tflag_a.p, v.nz, v.ny, nxt, kReach, 0, 0);
+        hipLaunchKernelGGL(pair_count_kernel, dim3(blocks), dim3(256), 0, stream, tflag_a.p, groups, v.ny, nyp, nxt, tcounts.p);
+        hipLaunchKernelGGL(pair_scan_kernel, dim3(1), dim3(256), 0, stream, tcounts.p, blocks, toffsets.p, ttotal.p);
+        hipLaunchKernelGGL(pair_write_kernel, dim3(blocks), dim3(256), 0, stream, tflag_a.p, groups, v.ny, nyp, nxt, toffsets.p,
+                           tiles.p);
+        HIP_TRY(hipGetLastError());
+        int total = 0;
+        HIP_TRY(hipMemcpyAsync(&total, ttotal.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        n = total;
+        return DXG_OK;
+    }
+
+    // The body-tile list the convolutions take: on the device for row pairs
+    // unless DXG_DENOISE_TILES_HOST=1 (or always on the host for single rows).
+    int buildTiles(const dxg::WorldView& v, const Dims& dm, bool host)
+    {
+        if (!host && convPairs()) {
+            if (int rc = devicePairTiles(v, dm, n_tiles); rc != DXG_OK)
+                return rc;
+        } else {
+            std::vector<int> t = bodyTiles(v.material_host, v.nx, v.ny, v.nz, dm.nx);
+            if (convPairs())
+                t = pairTiles(t, dm.ny, dm.nx / 64);
+            HIP_TRY(tiles.reserve(std::max<size_t>(t.size(), 1)));
+            if (!t.empty())
+                HIP_TRY(hipMemcpyAsync(tiles.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+            n_tiles = static_cast<int>(t.size());
+        }
         return DXG_OK;
     }
 
@@ -1842,13 +2013,9 @@ int dxg_denoiser_apply(dxg_denoiser* d, const dxg_world* w, double* dose_mgy)
     const bool skip = !(env && env[0] == '0');
     if (skip) {
         if (d->tiles_generation != v.generation || !d->tiles.p) {
-            std::vector<int> t = bodyTiles(v.material_host, v.nx, v.ny, v.nz, dm.nx);
-            if (convPairs())
-                t = pairTiles(t, dm.ny, dm.nx / 64);
-            HIP_TRY(d->tiles.reserve(std::max<size_t>(t.size(), 1)));
-            if (!t.empty())
-                HIP_TRY(hipMemcpyAsync(d->tiles.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, d->stream));
-            d->n_tiles = static_cast<int>(t.size());
+            const char* th = std::getenv("DXG_DENOISE_TILES_HOST");
+            if (int rc = d->buildTiles(v, dm, th && th[0] == '1'); rc != DXG_OK)
+                return rc;
             d->tiles_generation = v.generation;
             d->tiles_zeroed = false;
         }
@@ -1882,6 +2049,29 @@ int dxg_denoiser_apply(dxg_denoiser* d, const dxg_world* w, double* dose_mgy)
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
     d->last_ms = ms;
+    return DXG_OK;
+}
+
+int dxg_denoiser_debug_tiles(dxg_denoiser* d, const dxg_world* w, int host, int32_t* out, uint64_t cap, uint64_t* count)
+{
+    if (!d || !w || !count || (!out && cap))
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_debug_tiles: null argument");
+    dxg::WorldView v;
+    if (int rc = dxg::worldView(w, v); rc != DXG_OK)
+        return rc;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != v.device)
+        return fail(DXG_ERR_ARGUMENT, "denoiser and world live on different devices");
+    const Dims dm { 1, v.nz, v.ny, (v.nx + 63) / 64 * 64 };
+    if (int rc = d->buildTiles(v, dm, host != 0); rc != DXG_OK)
+        return rc;
+    d->tiles_generation = ~0ull;   // the next apply builds its own list
+    *count = static_cast<uint64_t>(d->n_tiles);
+    const size_t k = std::min<size_t>(cap, static_cast<size_t>(d->n_tiles));
+    if (k)
+        HIP_TRY(hipMemcpyAsync(out, d->tiles.p, k * sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
     return DXG_OK;
 }
 

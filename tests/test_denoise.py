@@ -386,3 +386,55 @@ def test_apply_on_body_tiles_equals_full_volume(monkeypatch):
     np.testing.assert_allclose(part[body], full[body], rtol=1e-12, atol=0)
     np.testing.assert_allclose(part2[body], full[body], rtol=1e-12, atol=0)
     print(f"full volume {t_full:.2f} ms, body tiles {t_part:.2f} ms")
+
+
+def _built_world(dims, spacing, material, density, mats=({7: 0.76, 8: 0.24}, {1: 0.111894, 8: 0.888106})):
+    import opendxmc_amd as dx
+    world = dx.World()
+    grid = world.addItem(dx.AAVoxelGrid(correction=1))
+    grid.setData(dims, density.ravel(), material.ravel(), [dx.Material.byWeight(m) for m in mats])
+    grid.setSpacing(spacing)
+    world.build()
+    return world, grid
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["phantom", "ragged", "empty", "full", "thin", "specks"])
+def test_device_tile_list_equals_host_scan(case):
+    """The row-pair tile list built on the device (tile flags, dilation by one
+    x tile and 19 rows / planes, ordered compaction) equals the host scan's,
+    entry for entry: a seeded body, x not a multiple of 64 with an odd row
+    count, no body at all, body everywhere, fewer planes than the reach, and
+    single body voxels near the volume's faces and corners."""
+    from opendxmc_amd import phantoms
+    rng = np.random.default_rng(7)
+    mats = ({7: 0.76, 8: 0.24}, {1: 0.111894, 8: 0.888106})
+    if case == "phantom":
+        ph = phantoms.ct_body((448, 96, 40), (0.4, 0.4, 0.4), seed=4)
+        dims, mat, mats = ph.dims, ph.material.reshape(ph.dims[::-1]).astype(np.uint8), ph.materials
+    else:
+        dims = {"ragged": (100, 37, 30), "empty": (128, 16, 8), "full": (64, 9, 5), "thin": (200, 50, 3),
+                "specks": (300, 81, 45)}[case]
+        nx, ny, nz = dims
+        mat = np.zeros((nz, ny, nx), dtype=np.uint8)
+        if case == "ragged":
+            mat[rng.random(mat.shape) < 0.002] = 1
+        elif case == "full":
+            mat[:] = 1
+        elif case == "thin":
+            mat[1, 25, 130] = 1
+        elif case == "specks":
+            for z, y, x in [(0, 0, 0), (44, 80, 299), (20, 40, 64), (0, 80, 63), (44, 0, 192), (22, 1, 255)]:
+                mat[z, y, x] = 1
+    dens = np.where(mat != 0, 1.0, 0.0012)
+    world, grid = _built_world(dims, (0.3, 0.3, 0.3), mat, dens, mats)
+    net = dn.Denoiser(seed=1)
+    dev = net.debug_tiles(grid)
+    host = net.debug_tiles(grid, host=True)
+    print(f"{case}: {host.size} row pairs")
+    np.testing.assert_array_equal(dev, host)
+    if case == "empty":
+        assert host.size == 0
+    if case == "full":
+        nxt, nyp = 1, (dims[1] + 1) // 2
+        assert host.size == dims[2] * nyp * nxt
