@@ -70,11 +70,13 @@ int dxg_denoiser_world_features(dxg_denoiser* d, const dxg_world* w, float* feat
 int dxg_denoiser_apply(dxg_denoiser* d, const dxg_world* w, double* dose_mgy);
 
 /* Test aid (no counterpart in the reference): the body-tile list that
- * dxg_denoiser_apply runs the convolutions on, built on the device (host = 0)
- * or by the host scan (host = 1).  With row pairs (the default) an entry is
+ * dxg_denoiser_apply runs convolution `layer` (0..17) on — the tiles within
+ * 19 - layer voxels of material != 0 — built on the device (host = 0) or by
+ * the host scan (host = 1).  With row pairs (the default) an entry is
  * pair << 2 | rows to store, pair = (z * ceil(ny / 2) + y / 2) * nx_tiles + x / 64,
  * ascending.  Up to cap entries go to out; *count receives the length. */
-int dxg_denoiser_debug_tiles(dxg_denoiser* d, const dxg_world* w, int host, int32_t* out, uint64_t cap, uint64_t* count);
+int dxg_denoiser_debug_tiles(dxg_denoiser* d, const dxg_world* w, int host, int layer, int32_t* out, uint64_t cap,
+                             uint64_t* count);
 
 /* ---- numerics probes (tests): one convolution on host arrays.
  * x: N x D x H x W x 32, w: 32 x 27 x 32 [co][tap][ci], bias 32, y out.

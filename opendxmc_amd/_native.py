@@ -226,7 +226,7 @@ SIGNATURES = [
     ("dxg_denoiser_last_ms", C.c_double, [C.c_void_p]),
     ("dxg_denoiser_world_features", C.c_int, [C.c_void_p, C.c_void_p, PF, u64]),
     ("dxg_denoiser_apply", C.c_int, [C.c_void_p, C.c_void_p, PD]),
-    ("dxg_denoiser_debug_tiles", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int32), u64, C.POINTER(u64)]),
+    ("dxg_denoiser_debug_tiles", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int32), u64, C.POINTER(u64)]),
     ("dxg_conv3d_probe", C.c_int, [C.c_int, PF, PF, PF, u64, C.POINTER(u64), PF]),
     ("dxg_mfma_probe", C.c_int, [PF, PF, PF]),
     ("dxg_ct_segmentation_tables", C.c_int, [C.POINTER(Tube), C.POINTER(Material), PD, u64, C.POINTER(Material),

@@ -1286,11 +1286,13 @@ hipError_t launchWgrad(const uint16_t* x, const uint16_t* g, float* part, float*
 constexpr int kReach = kLayers + 1;
 
 // Wave tiles (64 voxels along x of one row) of a padded world volume that lie
-// within kReach voxels (Chebyshev) of a voxel with material != 0, ascending.
+// within `reach` voxels (Chebyshev) of a voxel with material != 0, ascending.
 // Body-voxel predictions computed on these tiles alone equal the full
-// computation: outside them the activations stay zero, and that difference
-// moves inward by one voxel per layer, so it never reaches the body.
-std::vector<int> bodyTiles(const uint8_t* mat, int nx, int ny, int nz, int nxp)
+// computation: outside them the activations are not computed, and wrong
+// values there move inward by one voxel per layer, so layer l's outputs are
+// exact within kReach - l voxels of the body when layer l runs on the tiles
+// within that reach (layerReach).
+std::vector<int> bodyTiles(const uint8_t* mat, int nx, int ny, int nz, int nxp, int reach = kReach)
 {
     const int nxt = nxp / 64;
     const size_t rows = static_cast<size_t>(nz) * ny;
@@ -1311,12 +1313,12 @@ std::vector<int> bodyTiles(const uint8_t* mat, int nx, int ny, int nz, int nxp)
             b[r * nxt + t] = any != 0;
         }
     }
-    // x: kReach < 64, so one tile either side
+    // x: reach < 64, so one tile either side
     std::vector<uint8_t> c(b.size(), 0);
     for (size_t r = 0; r < rows; ++r)
         for (int t = 0; t < nxt; ++t)
             c[r * nxt + t] = b[r * nxt + t] | (t > 0 ? b[r * nxt + t - 1] : 0) | (t + 1 < nxt ? b[r * nxt + t + 1] : 0);
-    // y and z: box dilation by kReach through running counts
+    // y and z: box dilation by reach through running counts
     auto dilate = [&](std::vector<uint8_t>& src, int n, size_t stride, size_t outer_n, size_t outer_stride, size_t inner_n) {
         std::vector<uint8_t> dst(src.size(), 0);
         std::vector<int> pre(static_cast<size_t>(n) + 1);
@@ -1327,7 +1329,7 @@ std::vector<int> bodyTiles(const uint8_t* mat, int nx, int ny, int nz, int nxp)
                 for (int k = 0; k < n; ++k)
                     pre[static_cast<size_t>(k) + 1] = pre[static_cast<size_t>(k)] + src[base + k * stride];
                 for (int k = 0; k < n; ++k) {
-                    const int lo = std::max(0, k - kReach), hi = std::min(n, k + kReach + 1);
+                    const int lo = std::max(0, k - reach), hi = std::min(n, k + reach + 1);
                     dst[base + k * stride] = pre[static_cast<size_t>(hi)] - pre[static_cast<size_t>(lo)] > 0;
                 }
             }
@@ -1341,6 +1343,10 @@ std::vector<int> bodyTiles(const uint8_t* mat, int nx, int ny, int nz, int nxp)
             tiles.push_back(static_cast<int>(i));
     return tiles;
 }
+
+// The reach of layer l's tile list: its outputs feed kLayers - 1 - l more
+// convolutions, so they matter within kReach - l voxels of the body.
+constexpr int layerReach(int l) { return kReach - l; }
 
 // The row pairs of conv3d_pair_kernel that hold the given wave tiles
 // (ascending tile indices of a volume with ny rows and nxt tiles per row), as
@@ -1516,14 +1522,15 @@ struct dxg_denoiser {
     DBuf<uint16_t> xin, xin4, acts, gbuf;
     DBuf<float> feat0, y32, target, mask, features, lpart, part, part_b, sums, wemb;
     DBuf<double> wacc, wdose;
-    DBuf<int> tiles;
-    DBuf<uint8_t> tflag_a, tflag_b;     // device tile list: tile flags before / after dilation
+    DBuf<int> tiles;                    // kLayers lists of up to tiles_stride entries, layer l's at l * tiles_stride
+    size_t tiles_stride = 0;
+    int layer_tiles[kLayers] = {};      // entries of each layer's list
+    DBuf<uint8_t> tflag_a, tflag_b, tflag_c;   // device tile lists: flags, then after the y and z dilations
     DBuf<int> tcounts, toffsets, ttotal;
     uint64_t tiles_generation = 0;   // world generation the tile list belongs to
     bool tiles_zeroed = false;       // acts and y32 are zero outside the tile list ...
     const void* zeroed_acts = nullptr;   // ... in these allocations
     const void* zeroed_y32 = nullptr;
-    int n_tiles = 0;
     uint64_t step = 0;
     bool weights_ready = false;
     double last_ms = 0.0;
@@ -1569,13 +1576,19 @@ struct dxg_denoiser {
 
     // Forward pass.  With keep, activations a_0..a_8 and h_0..h_7 are kept
     // for the backward pass (17 tensors); otherwise two tensors are reused.
-    // Tiles the convolutions compute (null: all); set by apply for one pass.
-    const int* tile_list = nullptr;
-    int tile_count = 0;
+    // Tiles the convolutions compute (off: all); set by apply for one pass.
+    bool use_tiles = false;
+    void setTiles(ConvArgs& c, int layer) const
+    {
+        if (use_tiles) {
+            c.tiles = tiles.p + static_cast<size_t>(layer) * tiles_stride;
+            c.wave_tiles = layer_tiles[layer];
+        }
+    }
 
     int forward(const Dims& d, bool keep)
     {
-        if (!tile_list)
+        if (!use_tiles)
             tiles_zeroed = false;   // a full pass writes every tile
         const size_t nv = d.voxels(), ts = nv * kC;
         HIP_TRY(acts.reserve((keep ? 17 : 2) * ts));
@@ -1588,8 +1601,7 @@ struct dxg_denoiser {
         c.bias = layerBias(0);
         c.y = aT(0);
         c.relu = 1;
-        c.tiles = tile_list;
-        c.wave_tiles = tile_count;
+        setTiles(c, 0);
         HIP_TRY(launchConv(c, d, stream));
         for (int b = 0; b < kBlocks; ++b) {
             c = ConvArgs {};
@@ -1598,8 +1610,7 @@ struct dxg_denoiser {
             c.bias = layerBias(2 * b + 1);
             c.y = hT(b);
             c.relu = 1;
-            c.tiles = tile_list;
-            c.wave_tiles = tile_count;
+            setTiles(c, 2 * b + 1);
             HIP_TRY(launchConv(c, d, stream));
             c = ConvArgs {};
             c.x = hT(b);
@@ -1608,8 +1619,7 @@ struct dxg_denoiser {
             c.res = aT(b);
             c.y = aT(b + 1);
             c.relu = 1;
-            c.tiles = tile_list;
-            c.wave_tiles = tile_count;
+            setTiles(c, 2 * b + 2);
             HIP_TRY(launchConv(c, d, stream));
         }
         c = ConvArgs {};
@@ -1618,63 +1628,73 @@ struct dxg_denoiser {
         c.bias = layerBias(kLayers - 1);
         c.y32 = y32.p;
         c.y32_channels = 1;
-        c.tiles = tile_list;
-        c.wave_tiles = tile_count;
+        setTiles(c, kLayers - 1);
         HIP_TRY(launchConv(c, d, stream));
         return DXG_OK;
     }
 
-    // The row-pair list of the world's body tiles, built on the device: the
-    // same list as pairTiles(bodyTiles(...)) (tile flags from the material
-    // bytes of the voxel words, box dilation by one tile along x and kReach
-    // rows and planes, ordered compaction of the pairs with a row to store).
-    // n receives its length.
-    int devicePairTiles(const dxg::WorldView& v, const Dims& dm, int& n)
+    // The row-pair lists of the world's body tiles for every layer (reach
+    // layerReach(l)), built on the device: the same lists as
+    // pairTiles(bodyTiles(..., layerReach(l))).  Tile flags from the material
+    // bytes of the voxel words once, then per layer a box dilation by one x
+    // tile and the reach in rows, another in planes, and an ordered
+    // compaction of the pairs with a row to store.
+    int devicePairTiles(const dxg::WorldView& v, const Dims& dm)
     {
         static_assert(kConvRows == 2, "pair masks of two rows");
         const int nxt = dm.nx / 64, nyp = (v.ny + 1) / 2;
         const size_t ntiles = static_cast<size_t>(v.nz) * v.ny * nxt;
         const size_t groups = static_cast<size_t>(v.nz) * nyp * nxt;
         const int blocks = static_cast<int>((groups + 255) / 256);
+        tiles_stride = std::max<size_t>(groups, 1);
         HIP_TRY(tflag_a.reserve(ntiles));
         HIP_TRY(tflag_b.reserve(ntiles));
+        HIP_TRY(tflag_c.reserve(ntiles));
         HIP_TRY(tcounts.reserve(static_cast<size_t>(blocks)));
         HIP_TRY(toffsets.reserve(static_cast<size_t>(blocks)));
-        HIP_TRY(ttotal.reserve(1));
-        HIP_TRY(tiles.reserve(std::max<size_t>(groups, 1)));
+        HIP_TRY(ttotal.reserve(kLayers));
+        HIP_TRY(tiles.reserve(kLayers * tiles_stride));
         hipLaunchKernelGGL(tile_flags_kernel, dim3(static_cast<unsigned>((ntiles * 64 + 255) / 256)), dim3(256), 0, stream, v.voxels,
                            v.nx, nxt, ntiles, tflag_a.p);
         const dim3 tgrid(static_cast<unsigned>((ntiles + 255) / 256));
-        hipLaunchKernelGGL(tile_dilate_kernel, tgrid, dim3(256), 0, stream, tflag_a.p, tflag_b.p, v.nz, v.ny, nxt, 0, kReach, 1);
-        hipLaunchKernelGGL(tile_dilate_kernel, tgrid, dim3(256), 0, stream, tflag_b.p, tflag_a.p, v.nz, v.ny, nxt, kReach, 0, 0);
-        hipLaunchKernelGGL(pair_count_kernel, dim3(blocks), dim3(256), 0, stream, tflag_a.p, groups, v.ny, nyp, nxt, tcounts.p);
-        hipLaunchKernelGGL(pair_scan_kernel, dim3(1), dim3(256), 0, stream, tcounts.p, blocks, toffsets.p, ttotal.p);
-        hipLaunchKernelGGL(pair_write_kernel, dim3(blocks), dim3(256), 0, stream, tflag_a.p, groups, v.ny, nyp, nxt, toffsets.p,
-                           tiles.p);
+        for (int l = 0; l < kLayers; ++l) {
+            const int r = layerReach(l);
+            hipLaunchKernelGGL(tile_dilate_kernel, tgrid, dim3(256), 0, stream, tflag_a.p, tflag_b.p, v.nz, v.ny, nxt, 0, r, 1);
+            hipLaunchKernelGGL(tile_dilate_kernel, tgrid, dim3(256), 0, stream, tflag_b.p, tflag_c.p, v.nz, v.ny, nxt, r, 0, 0);
+            hipLaunchKernelGGL(pair_count_kernel, dim3(blocks), dim3(256), 0, stream, tflag_c.p, groups, v.ny, nyp, nxt, tcounts.p);
+            hipLaunchKernelGGL(pair_scan_kernel, dim3(1), dim3(256), 0, stream, tcounts.p, blocks, toffsets.p, ttotal.p + l);
+            hipLaunchKernelGGL(pair_write_kernel, dim3(blocks), dim3(256), 0, stream, tflag_c.p, groups, v.ny, nyp, nxt, toffsets.p,
+                               tiles.p + static_cast<size_t>(l) * tiles_stride);
+        }
         HIP_TRY(hipGetLastError());
-        int total = 0;
-        HIP_TRY(hipMemcpyAsync(&total, ttotal.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(layer_tiles, ttotal.p, kLayers * sizeof(int), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
-        n = total;
         return DXG_OK;
     }
 
-    // The body-tile list the convolutions take: on the device for row pairs
+    // The body-tile lists the convolutions take: on the device for row pairs
     // unless DXG_DENOISE_TILES_HOST=1 (or always on the host for single rows).
     int buildTiles(const dxg::WorldView& v, const Dims& dm, bool host)
     {
-        if (!host && convPairs()) {
-            if (int rc = devicePairTiles(v, dm, n_tiles); rc != DXG_OK)
-                return rc;
-        } else {
-            std::vector<int> t = bodyTiles(v.material_host, v.nx, v.ny, v.nz, dm.nx);
+        if (!host && convPairs())
+            return devicePairTiles(v, dm);
+        std::vector<int> t[kLayers];
+        size_t most = 1;
+        for (int l = 0; l < kLayers; ++l) {
+            t[l] = bodyTiles(v.material_host, v.nx, v.ny, v.nz, dm.nx, layerReach(l));
             if (convPairs())
-                t = pairTiles(t, dm.ny, dm.nx / 64);
-            HIP_TRY(tiles.reserve(std::max<size_t>(t.size(), 1)));
-            if (!t.empty())
-                HIP_TRY(hipMemcpyAsync(tiles.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-            n_tiles = static_cast<int>(t.size());
+                t[l] = pairTiles(t[l], dm.ny, dm.nx / 64);
+            most = std::max(most, t[l].size());
         }
+        tiles_stride = most;
+        HIP_TRY(tiles.reserve(kLayers * tiles_stride));
+        for (int l = 0; l < kLayers; ++l) {
+            if (!t[l].empty())
+                HIP_TRY(hipMemcpyAsync(tiles.p + static_cast<size_t>(l) * tiles_stride, t[l].data(), t[l].size() * sizeof(int),
+                                       hipMemcpyHostToDevice, stream));
+            layer_tiles[l] = static_cast<int>(t[l].size());
+        }
+        HIP_TRY(hipStreamSynchronize(stream));   // the host vectors go out of scope
         return DXG_OK;
     }
 
@@ -2028,12 +2048,10 @@ int dxg_denoiser_apply(dxg_denoiser* d, const dxg_world* w, double* dose_mgy)
             d->zeroed_acts = d->acts.p;
             d->zeroed_y32 = d->y32.p;
         }
-        d->tile_list = d->tiles.p;
-        d->tile_count = d->n_tiles;
+        d->use_tiles = true;
     }
-    const int frc = d->n_tiles > 0 || !skip ? d->forward(dm, false) : DXG_OK;
-    d->tile_list = nullptr;
-    d->tile_count = 0;
+    const int frc = d->layer_tiles[0] > 0 || !skip ? d->forward(dm, false) : DXG_OK;
+    d->use_tiles = false;
     if (frc != DXG_OK)
         return frc;
     HIP_TRY(d->wdose.reserve(nv));
@@ -2052,10 +2070,13 @@ int dxg_denoiser_apply(dxg_denoiser* d, const dxg_world* w, double* dose_mgy)
     return DXG_OK;
 }
 
-int dxg_denoiser_debug_tiles(dxg_denoiser* d, const dxg_world* w, int host, int32_t* out, uint64_t cap, uint64_t* count)
+int dxg_denoiser_debug_tiles(dxg_denoiser* d, const dxg_world* w, int host, int layer, int32_t* out, uint64_t cap,
+                             uint64_t* count)
 {
     if (!d || !w || !count || (!out && cap))
         return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_debug_tiles: null argument");
+    if (layer < 0 || layer >= kLayers)
+        return fail(DXG_ERR_ARGUMENT, "dxg_denoiser_debug_tiles: layer must be 0..17");
     dxg::WorldView v;
     if (int rc = dxg::worldView(w, v); rc != DXG_OK)
         return rc;
@@ -2067,10 +2088,11 @@ int dxg_denoiser_debug_tiles(dxg_denoiser* d, const dxg_world* w, int host, int3
     if (int rc = d->buildTiles(v, dm, host != 0); rc != DXG_OK)
         return rc;
     d->tiles_generation = ~0ull;   // the next apply builds its own list
-    *count = static_cast<uint64_t>(d->n_tiles);
-    const size_t k = std::min<size_t>(cap, static_cast<size_t>(d->n_tiles));
+    *count = static_cast<uint64_t>(d->layer_tiles[layer]);
+    const size_t k = std::min<size_t>(cap, static_cast<size_t>(d->layer_tiles[layer]));
     if (k)
-        HIP_TRY(hipMemcpyAsync(out, d->tiles.p, k * sizeof(int), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipMemcpyAsync(out, d->tiles.p + static_cast<size_t>(layer) * d->tiles_stride, k * sizeof(int),
+                               hipMemcpyDeviceToHost, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
     return DXG_OK;
 }

@@ -125,15 +125,15 @@ class Denoiser:
         N.check(N.lib().dxg_denoiser_apply(self._h, grid._h, p))
         return out
 
-    def debug_tiles(self, grid, host: bool = False) -> np.ndarray:
-        """The body-tile list apply() runs the convolutions on (row-pair
-        entries pair << 2 | rows to store), built on the device or, with
-        host=True, by the host scan."""
+    def debug_tiles(self, grid, host: bool = False, layer: int = 0) -> np.ndarray:
+        """The body-tile list apply() runs convolution `layer` on (row-pair
+        entries pair << 2 | rows to store; the tiles within 19 - layer voxels
+        of the body), built on the device or, with host=True, by the host scan."""
         n = N.u64(0)
-        N.check(N.lib().dxg_denoiser_debug_tiles(self._h, grid._h, int(host), None, 0, C.byref(n)))
+        N.check(N.lib().dxg_denoiser_debug_tiles(self._h, grid._h, int(host), int(layer), None, 0, C.byref(n)))
         out = np.zeros(max(int(n.value), 1), dtype=np.int32)
-        N.check(N.lib().dxg_denoiser_debug_tiles(self._h, grid._h, int(host), out.ctypes.data_as(C.POINTER(C.c_int32)),
-                                                 int(n.value), C.byref(n)))
+        N.check(N.lib().dxg_denoiser_debug_tiles(self._h, grid._h, int(host), int(layer),
+                                                 out.ctypes.data_as(C.POINTER(C.c_int32)), int(n.value), C.byref(n)))
         return out[:int(n.value)]
 
     @classmethod

@@ -401,11 +401,11 @@ def _built_world(dims, spacing, material, density, mats=({7: 0.76, 8: 0.24}, {1:
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", ["phantom", "ragged", "empty", "full", "thin", "specks"])
 def test_device_tile_list_equals_host_scan(case):
-    """The row-pair tile list built on the device (tile flags, dilation by one
-    x tile and 19 rows / planes, ordered compaction) equals the host scan's,
-    entry for entry: a seeded body, x not a multiple of 64 with an odd row
-    count, no body at all, body everywhere, fewer planes than the reach, and
-    single body voxels near the volume's faces and corners."""
+    """The row-pair tile lists built on the device (tile flags, dilation by one
+    x tile and 19 - layer rows / planes, ordered compaction) equal the host
+    scan's, entry for entry: a seeded body, x not a multiple of 64 with an odd
+    row count, no body at all, body everywhere, fewer planes than the reach,
+    and single body voxels near the volume's faces and corners."""
     from opendxmc_amd import phantoms
     rng = np.random.default_rng(7)
     mats = ({7: 0.76, 8: 0.24}, {1: 0.111894, 8: 0.888106})
@@ -429,12 +429,16 @@ def test_device_tile_list_equals_host_scan(case):
     dens = np.where(mat != 0, 1.0, 0.0012)
     world, grid = _built_world(dims, (0.3, 0.3, 0.3), mat, dens, mats)
     net = dn.Denoiser(seed=1)
-    dev = net.debug_tiles(grid)
-    host = net.debug_tiles(grid, host=True)
-    print(f"{case}: {host.size} row pairs")
-    np.testing.assert_array_equal(dev, host)
-    if case == "empty":
-        assert host.size == 0
-    if case == "full":
-        nxt, nyp = 1, (dims[1] + 1) // 2
-        assert host.size == dims[2] * nyp * nxt
+    sizes = []
+    for layer in (0, 1, 9, 16, 17):   # the lists shrink with the layers' remaining reach (19 - layer voxels)
+        dev = net.debug_tiles(grid, layer=layer)
+        host = net.debug_tiles(grid, host=True, layer=layer)
+        np.testing.assert_array_equal(dev, host)
+        sizes.append(host.size)
+        if case == "empty":
+            assert host.size == 0
+        if case == "full":
+            nxt, nyp = 1, (dims[1] + 1) // 2
+            assert host.size == dims[2] * nyp * nxt
+    print(f"{case}: {sizes} row pairs for layers 0, 1, 9, 16, 17")
+    assert sizes == sorted(sizes, reverse=True)
