@@ -1285,68 +1285,78 @@ hipError_t launchWgrad(const uint16_t* x, const uint16_t* g, float* part, float*
 // the body depends on nothing farther than this.
 constexpr int kReach = kLayers + 1;
 
-// Wave tiles (64 voxels along x of one row) of a padded world volume that lie
-// within `reach` voxels (Chebyshev) of a voxel with material != 0, ascending.
-// Body-voxel predictions computed on these tiles alone equal the full
-// computation: outside them the activations are not computed, and wrong
-// values there move inward by one voxel per layer, so layer l's outputs are
-// exact within kReach - l voxels of the body when layer l runs on the tiles
-// within that reach (layerReach).
-std::vector<int> bodyTiles(const uint8_t* mat, int nx, int ny, int nz, int nxp, int reach = kReach)
+// Body-tile lists.  Layer l's outputs feed 17 - l more convolutions, so they
+// matter within layerReach(l) = kReach - l voxels (Chebyshev) of a voxel with
+// material != 0; computing layer l on the wave tiles (64 voxels along x of
+// one row) that hold such a voxel keeps the body's predictions equal to the
+// full computation: values outside a layer's tiles are wrong, and wrong
+// values move inward by one voxel per layer, never reaching that radius.
+constexpr int layerReach(int l) { return kReach - l; }
+
+// A tile's span: the first and last x offset (0..63) of its body voxels,
+// packed as first | last << 8; first = 255 (last = 0) when it has none.
+constexpr uint16_t kNoSpan = 255;
+__host__ __device__ inline uint16_t spanJoin(uint16_t a, uint16_t b)
 {
-    const int nxt = nxp / 64;
-    const size_t rows = static_cast<size_t>(nz) * ny;
-    std::vector<uint8_t> b(rows * nxt, 0);
-    for (size_t r = 0; r < rows; ++r) {
-        const uint8_t* row = mat + r * nx;
-        for (int t = 0; t * 64 < nx; ++t) {   // a tile's 64 material bytes as eight 64-bit words
-            const int x0 = t * 64, n = std::min(64, nx - x0);
-            uint64_t any = 0;
-            int x = 0;
-            for (; x + 8 <= n; x += 8) {
-                uint64_t w8;
-                std::memcpy(&w8, row + x0 + x, 8);
-                any |= w8;
-            }
-            for (; x < n; ++x)
-                any |= row[x0 + x];
-            b[r * nxt + t] = any != 0;
-        }
-    }
-    // x: reach < 64, so one tile either side
-    std::vector<uint8_t> c(b.size(), 0);
-    for (size_t r = 0; r < rows; ++r)
-        for (int t = 0; t < nxt; ++t)
-            c[r * nxt + t] = b[r * nxt + t] | (t > 0 ? b[r * nxt + t - 1] : 0) | (t + 1 < nxt ? b[r * nxt + t + 1] : 0);
-    // y and z: box dilation by reach through running counts
-    auto dilate = [&](std::vector<uint8_t>& src, int n, size_t stride, size_t outer_n, size_t outer_stride, size_t inner_n) {
-        std::vector<uint8_t> dst(src.size(), 0);
-        std::vector<int> pre(static_cast<size_t>(n) + 1);
-        for (size_t o = 0; o < outer_n; ++o)
-            for (size_t i = 0; i < inner_n; ++i) {
-                const size_t base = o * outer_stride + i;
-                pre[0] = 0;
-                for (int k = 0; k < n; ++k)
-                    pre[static_cast<size_t>(k) + 1] = pre[static_cast<size_t>(k)] + src[base + k * stride];
-                for (int k = 0; k < n; ++k) {
-                    const int lo = std::max(0, k - reach), hi = std::min(n, k + reach + 1);
-                    dst[base + k * stride] = pre[static_cast<size_t>(hi)] - pre[static_cast<size_t>(lo)] > 0;
-                }
-            }
-        src.swap(dst);
-    };
-    dilate(c, ny, nxt, nz, static_cast<size_t>(ny) * nxt, nxt);                              // along y
-    dilate(c, nz, static_cast<size_t>(ny) * nxt, 1, 0, static_cast<size_t>(ny) * nxt);        // along z
-    std::vector<int> tiles;
-    for (size_t i = 0; i < c.size(); ++i)
-        if (c[i])
-            tiles.push_back(static_cast<int>(i));
-    return tiles;
+    const unsigned lo = min(a & 0xFFu, b & 0xFFu), hi = max(a >> 8, b >> 8);
+    return static_cast<uint16_t>(lo | hi << 8);
+}
+// Whether tile t of a row holds a voxel within r voxels along x of a body
+// voxel, given the spans of the row's tiles already joined over the rows and
+// planes within r (r < 64: only the neighbouring tiles can reach).
+__host__ __device__ inline bool spanReaches(const uint16_t* row, int t, int nxt, int r)
+{
+    return (row[t] & 0xFFu) != kNoSpan || (t > 0 && static_cast<int>(row[t - 1] >> 8) >= 64 - r)
+           || (t + 1 < nxt && static_cast<int>(row[t + 1] & 0xFFu) < r);
 }
 
-// The reach of layer l's tile list: its outputs feed kLayers - 1 - l more
-// convolutions, so they matter within kReach - l voxels of the body.
-constexpr int layerReach(int l) { return kReach - l; }
+// The wave tiles of every layer, ascending, from the host copy of the
+// material array: spans per tile, joined over +-1 row and plane r times for
+// reach r (a box of half-width r is r boxes of half-width 1), then the x rule.
+void hostLayerTiles(const uint8_t* mat, int nx, int ny, int nz, int nxp, std::vector<int> (&out)[kLayers])
+{
+    const int nxt = nxp / 64;
+    const size_t rows = static_cast<size_t>(nz) * ny, n = rows * nxt;
+    std::vector<uint16_t> span(n, kNoSpan), tmp(n);
+    for (size_t r = 0; r < rows; ++r)
+        for (int t = 0; t * 64 < nx; ++t) {
+            const uint8_t* v = mat + r * nx + t * 64;
+            const int m = std::min(64, nx - t * 64);
+            for (int x = 0; x < m; ++x)
+                if (v[x])
+                    span[r * nxt + t] = spanJoin(span[r * nxt + t], static_cast<uint16_t>(x | x << 8));
+        }
+    for (int reach = 1; reach <= kReach; ++reach) {
+        for (size_t i = 0; i < n; ++i) {   // +-1 row
+            const size_t rr = i / nxt;
+            const int y = static_cast<int>(rr % ny);
+            uint16_t v = span[i];
+            if (y > 0)
+                v = spanJoin(v, span[i - nxt]);
+            if (y + 1 < ny)
+                v = spanJoin(v, span[i + nxt]);
+            tmp[i] = v;
+        }
+        const size_t plane = static_cast<size_t>(ny) * nxt;
+        for (size_t i = 0; i < n; ++i) {   // +-1 plane
+            const int z = static_cast<int>(i / plane);
+            uint16_t v = tmp[i];
+            if (z > 0)
+                v = spanJoin(v, tmp[i - plane]);
+            if (z + 1 < nz)
+                v = spanJoin(v, tmp[i + plane]);
+            span[i] = v;
+        }
+        const int l = kReach - reach;
+        if (l < 0 || l >= kLayers)
+            continue;
+        out[l].clear();
+        for (size_t r = 0; r < rows; ++r)
+            for (int t = 0; t < nxt; ++t)
+                if (spanReaches(&span[r * nxt], t, nxt, reach))
+                    out[l].push_back(static_cast<int>(r * nxt + t));
+    }
+}
 
 // The row pairs of conv3d_pair_kernel that hold the given wave tiles
 // (ascending tile indices of a volume with ny rows and nxt tiles per row), as
@@ -1370,11 +1380,10 @@ std::vector<int> pairTiles(const std::vector<int>& tiles, int ny, int nxt)
     return out;
 }
 
-// ---- the row-pair tile list on the device (the same list as pairTiles(bodyTiles(...)))
+// ---- the row-pair tile lists on the device (the same lists as pairTiles of hostLayerTiles)
 
-// flags[r * nxt + t] = 1 where tile t of row r (64 voxels along x) holds a
-// voxel of material != 0: one wavefront per tile.
-__global__ __launch_bounds__(256) void tile_flags_kernel(const uint32_t* voxels, int nx, int nxt, size_t tiles, uint8_t* flags)
+// span[r * nxt + t]: the span of tile t of row r (spanJoin), one wavefront per tile.
+__global__ __launch_bounds__(256) void tile_span_kernel(const uint32_t* voxels, int nx, int nxt, size_t tiles, uint16_t* span)
 {
     const size_t w = (static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
     const int lane = static_cast<int>(threadIdx.x & 63);
@@ -1383,27 +1392,44 @@ __global__ __launch_bounds__(256) void tile_flags_kernel(const uint32_t* voxels,
     const size_t r = w / static_cast<size_t>(nxt);
     const int x = static_cast<int>(w - r * nxt) * 64 + lane;
     const bool body = x < nx && (voxels[r * static_cast<size_t>(nx) + x] & 0xFFu) != 0;
-    const unsigned long long any = __ballot(body);
+    const unsigned long long m = __ballot(body);
     if (lane == 0)
-        flags[w] = any != 0;
+        span[w] = m ? static_cast<uint16_t>((__ffsll(static_cast<long long>(m)) - 1) | (63 - __clzll(static_cast<long long>(m))) << 8)
+                    : kNoSpan;
 }
 
-// out[(z, y, t)] = OR of src over t +- dt, y +- dy, z +- dz (clamped to the volume)
-__global__ __launch_bounds__(256) void tile_dilate_kernel(const uint8_t* src, uint8_t* out, int nz, int ny, int nxt, int dz, int dy, int dt)
+// out = src joined over +-1 along y (axis 0) or z (axis 1).
+__global__ __launch_bounds__(256) void tile_span_grow_kernel(const uint16_t* src, uint16_t* out, int nz, int ny, int nxt, int axis)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-    const size_t n = static_cast<size_t>(nz) * ny * nxt;
-    if (i >= n)
+    const size_t plane = static_cast<size_t>(ny) * nxt;
+    if (i >= plane * nz)
+        return;
+    uint16_t v = src[i];
+    if (axis == 0) {
+        const int y = static_cast<int>((i / nxt) % ny);
+        if (y > 0)
+            v = spanJoin(v, src[i - nxt]);
+        if (y + 1 < ny)
+            v = spanJoin(v, src[i + nxt]);
+    } else {
+        const int z = static_cast<int>(i / plane);
+        if (z > 0)
+            v = spanJoin(v, src[i - plane]);
+        if (z + 1 < nz)
+            v = spanJoin(v, src[i + plane]);
+    }
+    out[i] = v;
+}
+
+// flags = spanReaches at reach r
+__global__ __launch_bounds__(256) void tile_reach_kernel(const uint16_t* span, uint8_t* flags, size_t tiles, int nxt, int r)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= tiles)
         return;
     const int t = static_cast<int>(i % nxt);
-    const size_t r = i / nxt;
-    const int y = static_cast<int>(r % ny), z = static_cast<int>(r / ny);
-    uint8_t v = 0;
-    for (int zz = max(z - dz, 0); zz <= min(z + dz, nz - 1) && !v; ++zz)
-        for (int yy = max(y - dy, 0); yy <= min(y + dy, ny - 1) && !v; ++yy)
-            for (int tt = max(t - dt, 0); tt <= min(t + dt, nxt - 1); ++tt)
-                v |= src[(static_cast<size_t>(zz) * ny + yy) * nxt + tt];
-    out[i] = v;
+    flags[i] = spanReaches(span + (i - t), t, nxt, r);
 }
 
 // The store mask of row pair g = (z * nyp + yp) * nxt + t: bit o for row 2 yp + o.
@@ -1525,7 +1551,8 @@ struct dxg_denoiser {
     DBuf<int> tiles;                    // kLayers lists of up to tiles_stride entries, layer l's at l * tiles_stride
     size_t tiles_stride = 0;
     int layer_tiles[kLayers] = {};      // entries of each layer's list
-    DBuf<uint8_t> tflag_a, tflag_b, tflag_c;   // device tile lists: flags, then after the y and z dilations
+    DBuf<uint16_t> tspan_a, tspan_b;   // device tile lists: tile spans (before / after a join along y)
+    DBuf<uint8_t> tflag;               // tiles a layer computes
     DBuf<int> tcounts, toffsets, ttotal;
     uint64_t tiles_generation = 0;   // world generation the tile list belongs to
     bool tiles_zeroed = false;       // acts and y32 are zero outside the tile list ...
@@ -1633,12 +1660,11 @@ struct dxg_denoiser {
         return DXG_OK;
     }
 
-    // The row-pair lists of the world's body tiles for every layer (reach
-    // layerReach(l)), built on the device: the same lists as
-    // pairTiles(bodyTiles(..., layerReach(l))).  Tile flags from the material
-    // bytes of the voxel words once, then per layer a box dilation by one x
-    // tile and the reach in rows, another in planes, and an ordered
-    // compaction of the pairs with a row to store.
+    // The row-pair lists of the world's body tiles for every layer, built on
+    // the device: the same lists as pairTiles of hostLayerTiles.  Tile spans
+    // from the material bytes of the voxel words, joined over +-1 row and
+    // plane once per unit of reach, and at each layer's reach the x rule and
+    // an ordered compaction of the pairs with a row to store.
     int devicePairTiles(const dxg::WorldView& v, const Dims& dm)
     {
         static_assert(kConvRows == 2, "pair masks of two rows");
@@ -1647,23 +1673,26 @@ struct dxg_denoiser {
         const size_t groups = static_cast<size_t>(v.nz) * nyp * nxt;
         const int blocks = static_cast<int>((groups + 255) / 256);
         tiles_stride = std::max<size_t>(groups, 1);
-        HIP_TRY(tflag_a.reserve(ntiles));
-        HIP_TRY(tflag_b.reserve(ntiles));
-        HIP_TRY(tflag_c.reserve(ntiles));
+        HIP_TRY(tspan_a.reserve(ntiles));
+        HIP_TRY(tspan_b.reserve(ntiles));
+        HIP_TRY(tflag.reserve(ntiles));
         HIP_TRY(tcounts.reserve(static_cast<size_t>(blocks)));
         HIP_TRY(toffsets.reserve(static_cast<size_t>(blocks)));
         HIP_TRY(ttotal.reserve(kLayers));
         HIP_TRY(tiles.reserve(kLayers * tiles_stride));
-        hipLaunchKernelGGL(tile_flags_kernel, dim3(static_cast<unsigned>((ntiles * 64 + 255) / 256)), dim3(256), 0, stream, v.voxels,
-                           v.nx, nxt, ntiles, tflag_a.p);
+        hipLaunchKernelGGL(tile_span_kernel, dim3(static_cast<unsigned>((ntiles * 64 + 255) / 256)), dim3(256), 0, stream, v.voxels,
+                           v.nx, nxt, ntiles, tspan_a.p);
         const dim3 tgrid(static_cast<unsigned>((ntiles + 255) / 256));
-        for (int l = 0; l < kLayers; ++l) {
-            const int r = layerReach(l);
-            hipLaunchKernelGGL(tile_dilate_kernel, tgrid, dim3(256), 0, stream, tflag_a.p, tflag_b.p, v.nz, v.ny, nxt, 0, r, 1);
-            hipLaunchKernelGGL(tile_dilate_kernel, tgrid, dim3(256), 0, stream, tflag_b.p, tflag_c.p, v.nz, v.ny, nxt, r, 0, 0);
-            hipLaunchKernelGGL(pair_count_kernel, dim3(blocks), dim3(256), 0, stream, tflag_c.p, groups, v.ny, nyp, nxt, tcounts.p);
+        for (int reach = 1; reach <= kReach; ++reach) {
+            hipLaunchKernelGGL(tile_span_grow_kernel, tgrid, dim3(256), 0, stream, tspan_a.p, tspan_b.p, v.nz, v.ny, nxt, 0);
+            hipLaunchKernelGGL(tile_span_grow_kernel, tgrid, dim3(256), 0, stream, tspan_b.p, tspan_a.p, v.nz, v.ny, nxt, 1);
+            const int l = kReach - reach;
+            if (l < 0 || l >= kLayers)
+                continue;
+            hipLaunchKernelGGL(tile_reach_kernel, tgrid, dim3(256), 0, stream, tspan_a.p, tflag.p, ntiles, nxt, reach);
+            hipLaunchKernelGGL(pair_count_kernel, dim3(blocks), dim3(256), 0, stream, tflag.p, groups, v.ny, nyp, nxt, tcounts.p);
             hipLaunchKernelGGL(pair_scan_kernel, dim3(1), dim3(256), 0, stream, tcounts.p, blocks, toffsets.p, ttotal.p + l);
-            hipLaunchKernelGGL(pair_write_kernel, dim3(blocks), dim3(256), 0, stream, tflag_c.p, groups, v.ny, nyp, nxt, toffsets.p,
+            hipLaunchKernelGGL(pair_write_kernel, dim3(blocks), dim3(256), 0, stream, tflag.p, groups, v.ny, nyp, nxt, toffsets.p,
                                tiles.p + static_cast<size_t>(l) * tiles_stride);
         }
         HIP_TRY(hipGetLastError());
@@ -1679,9 +1708,9 @@ struct dxg_denoiser {
         if (!host && convPairs())
             return devicePairTiles(v, dm);
         std::vector<int> t[kLayers];
+        hostLayerTiles(v.material_host, v.nx, v.ny, v.nz, dm.nx, t);
         size_t most = 1;
         for (int l = 0; l < kLayers; ++l) {
-            t[l] = bodyTiles(v.material_host, v.nx, v.ny, v.nz, dm.nx, layerReach(l));
             if (convPairs())
                 t[l] = pairTiles(t[l], dm.ny, dm.nx / 64);
             most = std::max(most, t[l].size());
