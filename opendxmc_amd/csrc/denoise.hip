@@ -1089,7 +1089,7 @@ struct WorldFeatArgs {
     size_t nvp;             // padded voxels
     double histories;
     const double* acc;      // dose_mean_kernel sums
-    uint16_t* xin;          // 32-channel BF16 network input
+    uint16_t* xin;          // 32-channel BF16 network input (null: only the compact copy is needed)
     uint16_t* xin4;         // its four feature channels alone (8 bytes per voxel)
     float* feat0;
     float* features;        // optional FP32 copy (nvp x 4)
@@ -1124,13 +1124,15 @@ __global__ __launch_bounds__(256) void world_features_kernel(WorldFeatArgs a)
         }
         f[3] = rel;
     }
-    u32x4* dst = reinterpret_cast<u32x4*>(a.xin + v * kC);
     const u32x2 f4 = { pack2(f[0], f[1]), pack2(f[2], f[3]) };
-    dst[0] = u32x4 { f4[0], f4[1], 0u, 0u };
     *reinterpret_cast<u32x2*>(a.xin4 + 4 * v) = f4;
-    dst[1] = u32x4 { 0u, 0u, 0u, 0u };
-    dst[2] = u32x4 { 0u, 0u, 0u, 0u };
-    dst[3] = u32x4 { 0u, 0u, 0u, 0u };
+    if (a.xin) {
+        u32x4* dst = reinterpret_cast<u32x4*>(a.xin + v * kC);
+        dst[0] = u32x4 { f4[0], f4[1], 0u, 0u };
+        dst[1] = u32x4 { 0u, 0u, 0u, 0u };
+        dst[2] = u32x4 { 0u, 0u, 0u, 0u };
+        dst[3] = u32x4 { 0u, 0u, 0u, 0u };
+    }
     a.feat0[v] = f[0];
     if (a.features)
         *reinterpret_cast<f32x4*>(a.features + 4 * v) = f32x4 { f[0], f[1], f[2], f[3] };
@@ -1743,7 +1745,10 @@ struct dxg_denoiser {
         dm = { 1, v.nz, v.ny, nxp };
         const size_t nvp = dm.voxels();
         const size_t nv = static_cast<size_t>(v.nx) * v.ny * v.nz;
-        HIP_TRY(xin.reserve(nvp * kC));
+        // the first layer reads the compact copy alone unless it runs the general kernel
+        const bool full_input = !firstLayerCompact(dm);
+        if (full_input)
+            HIP_TRY(xin.reserve(nvp * kC));
         HIP_TRY(xin4.reserve(nvp * 4));
         HIP_TRY(feat0.reserve(nvp));
         HIP_TRY(y32.reserve(nvp));
@@ -1765,7 +1770,7 @@ struct dxg_denoiser {
         a.nvp = nvp;
         a.histories = static_cast<double>(v.histories);
         a.acc = wacc.p;
-        a.xin = xin.p;
+        a.xin = full_input ? xin.p : nullptr;
         a.xin4 = xin4.p;
         a.feat0 = feat0.p;
         a.features = keep_fp32 ? features.p : nullptr;
