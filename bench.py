@@ -96,7 +96,7 @@ def time_to_2pct(ref_histories: float, cpu_seconds: float | None):
     """Time to 2 % mean relative dose error on the full dual-source CT spiral
     (512 x 512 x 400 at 0.08 cm), measured in this run: a reference run of
     ref_histories, plain runs at 1e9, 4e9 and 1.6e10 histories (error a /
-    sqrt(H) fitted, time at the measured rate) and runs of 5e7 .. 1.6e9
+    sqrt(H) fitted, time at the measured rate) and runs of 2e7 .. 1.6e9
     histories denoised by the shipped network (2 % crossing interpolated on
     log-log axes; MC time plus the denoiser's features + inference + dose).
     With cpu_seconds, the oracle's rate on a sample of the same workload gives
@@ -111,7 +111,7 @@ def time_to_2pct(ref_histories: float, cpu_seconds: float | None):
     t0 = time.perf_counter()
     ph = phantoms.ct_body((512, 512, 400), (0.08, 0.08, 0.08), seed=1)
     net = dn.Denoiser.pretrained()
-    ev = study.evaluate(net, ref_histories, (1e9, 4e9, 1.6e10), (5e7, 1e8, 2e8, 4e8, 8e8, 1.6e9), seed=1, ph=ph, log=log)
+    ev = study.evaluate(net, ref_histories, (1e9, 4e9, 1.6e10), (2e7, 3e7, 5e7, 1e8, 4e8, 1.6e9), seed=1, ph=ph, log=log)
     del net
     out = {
         "workload": "CT spiral dual source 512x512x400 at 0.08 cm, seeded body, 2 x 120 kVp 6 mm Al, W1 bowtie, "

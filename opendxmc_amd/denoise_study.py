@@ -77,13 +77,15 @@ def run(world, grid, beam, seed: int) -> float:
 
 # ------------------------------------------------------------------ training data
 
-def generate(net: dn.Denoiser, n_phantoms: int, dims, hpv_target: float, hpv_inputs, patches: int, seed: int):
-    """Patches (features, target, mask) from engine-generated pairs."""
+def generate(net: dn.Denoiser, n_phantoms: int, dims, hpv_target: float, hpv_inputs, patches: int, seed: int,
+             phantom_seed0: int = 1000):
+    """Patches (features, target, mask) from engine-generated pairs of the
+    seeded phantoms phantom_seed0, phantom_seed0 + 1, ..."""
     rng = np.random.default_rng(seed)
     pool = []
     emb = None
     for i in range(n_phantoms):
-        ph = phantoms.ct_body(dims, (0.08, 0.08, 0.08), seed=1000 + i)
+        ph = phantoms.ct_body(dims, (0.08, 0.08, 0.08), seed=phantom_seed0 + i)
         if emb is None:
             emb = dn.material_embedding(ph.materials)
         world, grid = make_grid(ph)
@@ -268,7 +270,9 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=2e-4)
-    ap.add_argument("--load-weights")
+    ap.add_argument("--load-weights", help="evaluate these weights (no training)")
+    ap.add_argument("--init-weights", help="train starting from these weights")
+    ap.add_argument("--phantom-seed0", type=int, default=1000, help="seed of the first training phantom")
     ap.add_argument("--save-weights")
     ap.add_argument("--skip-eval", action="store_true")
     ap.add_argument("--ref-histories", type=float, default=2e11)
@@ -282,8 +286,11 @@ def main(argv=None) -> int:
     if args.load_weights:
         net.load(args.load_weights)
     else:
+        if args.init_weights:
+            net.load(args.init_weights)
         t0 = time.perf_counter()
-        pool = generate(net, args.phantoms, tuple(args.dims), args.hpv_target, args.hpv_inputs, args.patches, args.seed)
+        pool = generate(net, args.phantoms, tuple(args.dims), args.hpv_target, args.hpv_inputs, args.patches, args.seed,
+                        args.phantom_seed0)
         result["data_s"] = time.perf_counter() - t0
         result["patches"] = len(pool)
         t0 = time.perf_counter()
