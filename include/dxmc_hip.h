@@ -246,6 +246,11 @@ int dxg_debug_source(dxg_world* w, const dxg_beam* beam, uint64_t seed, uint64_t
  * needed).  The level-1 S(x) rejection is not applied. */
 int dxg_debug_compton_cos(double energy_kev, uint64_t n, const float* u, float* cos_out);
 
+/* The same for correction levels 1 and 2: cos(theta) of incoherent
+ * scattering (Klein-Nishina x S(x)/Z) in material m, by the kernels' own
+ * per-material table and FP32 interpolation, evaluated on the host. */
+int dxg_debug_compton_cos_material(const dxg_material* m, double energy_kev, uint64_t n, const float* u, float* cos_out);
+
 /* ---- Profiling aid (no counterpart in the reference): `iters` launches of a
  * plain float4 copy kernel over `bytes` bytes (multiple of 16) between two
  * fresh device buffers.  Under a hardware-counter profiler its FETCH_SIZE and

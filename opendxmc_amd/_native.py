@@ -207,6 +207,7 @@ SIGNATURES = [
                                      C.POINTER(C.c_float), C.POINTER(i64), C.POINTER(C.c_int32)]),
     ("dxg_debug_source", C.c_int, [C.c_void_p, C.POINTER(Beam), u64, u64, C.POINTER(C.c_float)]),
     ("dxg_debug_compton_cos", C.c_int, [C.c_double, u64, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("dxg_debug_compton_cos_material", C.c_int, [C.POINTER(Material), C.c_double, u64, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("dxg_calibration_dap", C.c_int, [C.POINTER(Beam), C.c_double, PD]),
     ("dxg_calibration_air_kerma", C.c_int, [C.POINTER(Beam), C.c_double, PD]),
     ("dxg_calibration_ctdi", C.c_int, [C.POINTER(Beam), C.c_double, C.c_double, C.c_double, u64, u64, PD, PD]),

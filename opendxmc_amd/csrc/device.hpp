@@ -101,6 +101,35 @@ __host__ __device__ inline float knCos(const float2* tab, float e, float u)
     return fmaf(te, cb - ca, ca);
 }
 
+// Klein-Nishina x S(x) tables (physics.hpp kKnsPerOctave, kKnsRows): rows at
+// E_r = 2^(r / 12) keV.
+constexpr int kKnsRowsDev = 93;
+constexpr float kKnsPerOctaveDev = 12.0f;
+
+// cos(theta) of an incoherent scatter (Klein-Nishina x S(x)/Z) on one
+// material's table (physics.hpp buildComptonSTable), given the row
+// coordinate 12 log2(e) and the node coordinate (1 - sqrt(1 - u)) (kKnNDev -
+// 1) for energy e (1 <= e <= 200 keV) and uniform u in [0, 1): interpolated
+// as in knCos.  The transport kernels compute the two coordinates with the
+// hardware's log2 and square root (about 1 ulp), the host-side table check
+// with the correctly rounded ones; the rest is this shared code.
+__host__ __device__ inline float knsLookup(const float2* tab, float row, float node)
+{
+#pragma clang fp contract(off)
+    const float fe = fminf(fmaxf(row, 0.0f), static_cast<float>(kKnsRowsDev - 1));
+    const int r0 = static_cast<int>(fe);
+    const int r = r0 < kKnsRowsDev - 2 ? r0 : kKnsRowsDev - 2;
+    const float te = fe - static_cast<float>(r);
+    const int i0 = static_cast<int>(node);
+    const int i = i0 < kKnNDev - 2 ? i0 : kKnNDev - 2;
+    const float tu = node - static_cast<float>(i);
+    const float2 a = tab[r * kKnNDev + i];
+    const float2 b = tab[(r + 1) * kKnNDev + i];
+    const float ca = fmaf(tu, a.y, a.x);
+    const float cb = fmaf(tu, b.y, b.x);
+    return fmaf(te, cb - ca, ca);
+}
+
 // Everything one transport launch reads.  Passed by value as a kernel argument.
 struct DevParams {
     // voxel grid of packed words, flat index x + nx (y + ny z)
@@ -124,10 +153,10 @@ struct DevParams {
     const float* mu_tot;
     const float4* mu_parts;
     const float* majorant;
-    const float* s_table;
     const float* ray_fwd;
     const float* ray_inv;
     const float2* kn_inv;   // Klein-Nishina inverse CDF, kKnRowsDev x kKnNDev (knCos)
+    const float2* kns_inv;  // per material, Klein-Nishina x S(x) inverse CDF, kKnsRowsDev x kKnNDev each (knsLookup; levels 1, 2)
     // correction level 2 (see physics.hpp DeviceTables)
     const float4* ia_shell;
     const float* ia_inv;

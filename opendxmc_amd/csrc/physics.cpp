@@ -262,6 +262,58 @@ void buildComptonTable(std::vector<float>& out)
     }
 }
 
+// S(x)/Z on a fine x grid (FP64, interpolated linearly: S is smooth), then
+// per row the cumulative integral of KN(cos) S(x(cos))/Z over cos by
+// Simpson's rule on kFine intervals, inverted at the table's nodes.
+void buildComptonSTable(const PhysicsList& pl, const MixMaterial& m, std::vector<float>& out)
+{
+    constexpr int kFine = 16384, kSx = 8192;
+    const double x_top = std::exp2(static_cast<double>(kKnsRows - 1) / kKnsPerOctave) / kHcKeVAng;   // x of backscatter at the top row
+    std::vector<double> sx(kSx + 1);
+    for (int k = 0; k <= kSx; ++k)
+        sx[static_cast<size_t>(k)] = std::min(1.0, m.S(pl, x_top * k / kSx) / m.electrons_per_gram);
+    const auto sOf = [&](double x) {
+        const double f = std::min(x / x_top, 1.0) * kSx;
+        const int k = std::min(static_cast<int>(f), kSx - 1);
+        return sx[static_cast<size_t>(k)] + (f - k) * (sx[static_cast<size_t>(k) + 1] - sx[static_cast<size_t>(k)]);
+    };
+    std::vector<double> cum(kFine + 1);
+    std::vector<double> node(kKnN);
+    const size_t base = out.size();
+    out.resize(base + static_cast<size_t>(kKnsRows) * kKnN * 2);
+    for (int r = 0; r < kKnsRows; ++r) {
+        const double e = std::exp2(static_cast<double>(r) / kKnsPerOctave), k = e / kMeC2KeV, xe = e / kHcKeVAng;
+        const auto pdf = [&](double c) {
+            const double P = 1.0 / (1.0 + k * (1.0 - c));
+            return P * P * (P + 1.0 / P - (1.0 - c * c)) * sOf(xe * std::sqrt(std::max(0.0, 0.5 * (1.0 - c))));
+        };
+        const double h = 2.0 / kFine;
+        cum[0] = 0;
+        for (int b = 1; b <= kFine; ++b) {
+            const double c0 = -1.0 + (b - 1) * h;
+            cum[static_cast<size_t>(b)] = cum[static_cast<size_t>(b) - 1] + h / 6.0 * (pdf(c0) + 4.0 * pdf(c0 + 0.5 * h) + pdf(c0 + h));
+        }
+        const double tot = cum[kFine];
+        node[0] = -1.0;
+        node[kKnN - 1] = 1.0;
+        size_t b = 1;
+        for (int i = 1; i < kKnN - 1; ++i) {
+            const double w = 1.0 - static_cast<double>(i) / (kKnN - 1);
+            const double target = tot * (1.0 - w * w);
+            while (b < static_cast<size_t>(kFine) && cum[b] < target)
+                ++b;
+            const double span = cum[b] - cum[b - 1];
+            const double f = span > 0 ? (target - cum[b - 1]) / span : 0.0;
+            node[static_cast<size_t>(i)] = -1.0 + (static_cast<double>(b - 1) + f) * h;
+        }
+        for (int i = 0; i < kKnN; ++i) {
+            float* o = &out[base + (static_cast<size_t>(r) * kKnN + i) * 2];
+            o[0] = static_cast<float>(node[static_cast<size_t>(i)]);
+            o[1] = i + 1 < kKnN ? static_cast<float>(node[static_cast<size_t>(i) + 1]) - o[0] : 0.f;
+        }
+    }
+}
+
 const std::vector<float>& comptonTable()
 {
     static const std::vector<float> t = [] {
@@ -318,18 +370,17 @@ void buildTables(const PhysicsList& pl, const std::vector<MixMaterial>& mats, co
                 std::max(t.region_gain[static_cast<size_t>(m)], hi / static_cast<double>(t.majorant[static_cast<size_t>(j)]));
         }
     t.kn_inv = comptonTable();
-    // Sampling tables for correction level 1.
+    t.kns_inv.clear();
+    if (level >= 1)
+        for (int m = 0; m < nm; ++m)
+            buildComptonSTable(pl, mats[static_cast<size_t>(m)], t.kns_inv);
+    // Rayleigh sampling tables (correction level 1 and 2).
     constexpr int kFine = 16384;
-    t.s_table.assign(static_cast<size_t>(nm) * kSfN, 0.f);
     t.ray_fwd.assign(static_cast<size_t>(nm) * kSfN, 0.f);
     t.ray_inv.assign(static_cast<size_t>(nm) * kRayInvN, 0.f);
     std::vector<double> ucum(kFine + 1), acum(kFine + 1);
     for (int m = 0; m < nm; ++m) {
         const MixMaterial& mat = mats[static_cast<size_t>(m)];
-        for (int k = 0; k < kSfN; ++k) {
-            const double x = kXMax * k / (kSfN - 1);
-            t.s_table[static_cast<size_t>(m) * kSfN + k] = static_cast<float>(std::min(1.0, mat.S(pl, x) / mat.electrons_per_gram));
-        }
         // cumulative A(u) = int_0^u F^2(sqrt(u')) du' by the trapezoid rule on a fine x grid
         ucum[0] = 0;
         acum[0] = 0;

@@ -22,7 +22,7 @@ constexpr int kLutN = 797;   // (kLutEMax - kLutE0) / kLutDE + 1
 
 // Form-factor sampling tables: x = sin(theta/2)/lambda in [0, kXMax] (1/Angstrom).
 constexpr double kXMax = 16.0;
-constexpr int kSfN = 1024;   // nodes of the S(x) and F^2 forward tables
+constexpr int kSfN = 1024;   // nodes of the F^2 forward table
 constexpr int kRayInvN = 1024;   // nodes of the inverse F^2 table
 
 // Klein-Nishina inverse-CDF table: rows at E_r = kKnE0 + r kKnDE (keV), r <
@@ -33,6 +33,8 @@ constexpr double kKnE0 = 1.0;
 constexpr double kKnDE = 4.0;
 constexpr int kKnRows = 51;   // E_r up to kLutEMax + 1
 constexpr int kKnN = 128;
+constexpr int kKnsPerOctave = 12;   // rows per octave of the Klein-Nishina x S(x) tables
+constexpr int kKnsRows = 93;        // E_r = 2^(r / 12) keV up to 2^(92/12) = 203 keV
 
 // Photons below this energy deposit what is left and stop.
 constexpr double kCutoffKeV = 1.0;
@@ -92,10 +94,10 @@ struct DeviceTables {
     std::vector<float> mu_tot;     // [m][j] total mass attenuation (cm^2/g), kLutN per material
     std::vector<float> mu_parts;   // [m][j][4]: pe, pe+inc, total, 0 (cm^2/g)
     std::vector<float> majorant;   // [j] 1/cm, valid over [E_j, E_j+1]
-    std::vector<float> s_table;    // [m][k] S(x_k)/S(inf), x_k = k kXMax/(kSfN-1)
     std::vector<float> ray_fwd;    // [m][k] A(x_k)/A(kXMax), A(x) = int_0^{x^2} F^2 du
     std::vector<float> ray_inv;    // [m][k] u(y_k)/kXMax^2, y_k = k/(kRayInvN-1)
     std::vector<float> kn_inv;     // [r][i][2] Klein-Nishina inverse CDF (buildComptonTable)
+    std::vector<float> kns_inv;    // [m][r][i][2] Klein-Nishina x S(x) inverse CDF per material, levels 1 and 2 (buildComptonSTable)
     // correction level 2
     std::vector<float> ia_shell;   // [m][kIaShells][4]: cumulative electron fraction, binding (keV), p0 (m_e c), 0
     std::vector<float> ia_inv;     // [k] x(u_k) of the unit Compton profile (8/3pi)(1+x^2)^-3, u_k = k/(kIaInvN-1)
@@ -135,6 +137,15 @@ double comptonProfileCdf(double x);
 void buildComptonTable(std::vector<float>& out);
 // The same table, built once per process.
 const std::vector<float>& comptonTable();
+
+// Klein-Nishina x S(x)/Z inverse-CDF table of one material (the angular
+// distribution of incoherent scattering at correction levels 1 and 2): rows
+// at E_r = 2^(r / kKnsPerOctave) keV, r < kKnsRows (S(x) scales with x ~ E sin(theta/2),
+// so the distribution's shape changes by the same amount per octave), nodes
+// at u_i = 1 - (1 - i/(kKnN-1))^2 (denser towards u = 1, where S(x) -> 0 makes
+// cos(u) steep); entry (r, i) = {cos(u_i), cos(u_{i+1}) - cos(u_i)}.  Appends
+// kKnsRows x kKnN pairs to out.
+void buildComptonSTable(const PhysicsList& pl, const MixMaterial& m, std::vector<float>& out);
 
 void buildTables(const PhysicsList& pl, const std::vector<MixMaterial>& mats, const std::vector<double>& rho_max,
                  int level, DeviceTables& t);

@@ -39,9 +39,10 @@
 // Algorithms (DXMClib's AAVoxelGrid::transport and interactions<5, C>, absent
 // from the reference tree; the caller is simulationpipeline.cpp:159-170):
 //   Woodcock tracking with a per-energy-bin majorant; photoelectric
-//   absorption scored as KERMA; Klein-Nishina by its inverse CDF, tabulated
-//   in FP64 on 4 keV x 128 nodes and interpolated bilinearly (level 1:
-//   rejection on S(x)/Z); Thomson by the exact inverse of its cubic CDF
+//   absorption scored as KERMA; Klein-Nishina (level 0) or Klein-Nishina x
+//   S(x)/Z of the material (levels 1, 2) by its inverse CDF, tabulated in
+//   FP64 on 128 nodes per row (rows every 4 keV, or 12 per octave) and
+//   interpolated bilinearly; Thomson by the exact inverse of its cubic CDF
 //   (level 1: x^2 drawn from F^2(x), accepted with (1+cos^2)/2).
 #include "device.hpp"
 
@@ -894,24 +895,22 @@ __device__ __forceinline__ void runHistories(const DevParams& P, const float* s_
                         state = kDead;
                     }
                 } else if (proc == 1) {
-                    // Compton, one trial: Klein-Nishina from its inverse-CDF
-                    // table (no rejection), level 1 rejects on S(x)/Z, level 2
-                    // adds the impulse approximation: a shell drawn by electron
-                    // fraction, p_z from its Compton profile, the Doppler-shifted E'
+                    // Compton: the angle from an inverse-CDF table with no
+                    // rejection, Klein-Nishina at level 0 and Klein-Nishina x
+                    // S(x)/Z of the voxel's material at levels 1 and 2; level 2
+                    // adds the impulse approximation (one trial): a shell drawn
+                    // by electron fraction, p_z from its Compton profile, the
+                    // Doppler-shifted E'
                     const float k = E * kInvMeC2;
-                    const float cost = knCos(P.kn_inv, E, rngU(rng));
+                    float cost;
+                    if (LEVEL == 0) {
+                        cost = knCos(P.kn_inv, E, rngU(rng));
+                    } else {
+                        const float node = (1.0f - hwSqrt(1.0f - rngU(rng))) * static_cast<float>(kKnNDev - 1);
+                        cost = knsLookup(P.kns_inv + m * (kKnsRowsDev * kKnNDev), __builtin_amdgcn_logf(E) * kKnsPerOctaveDev, node);
+                    }
                     float enew = E * hwRcp(1.0f + k * (1.0f - cost));
                     bool ok = true;
-                    if (LEVEL > 0) {
-                        const float x = E * kInvHc * hwSqrt(fmaxf(0.0f, 0.5f * (1.0f - cost)));
-                        const float fx = x * (static_cast<float>(kSfNd - 1) / kXMaxf);
-                        float sv = 1.0f;
-                        if (fx < static_cast<float>(kSfNd - 1)) {
-                            const int kk = static_cast<int>(fx);
-                            sv = lerpTab(P.s_table + m * kSfNd, kk, fx - static_cast<float>(kk));
-                        }
-                        ok = rngU(rng) <= sv;
-                    }
                     if (LEVEL == 2 && ok) {
                         const float us = rngU(rng);
                         const float4* sh = P.ia_shell + m * kIaShellsd;

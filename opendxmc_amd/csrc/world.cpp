@@ -103,8 +103,8 @@ struct dxg_world {
     uint64_t outer_box[6] = { 0, 0, 0, 0, 0, 0 };
     DevBuf<float> d_mu_tot;
     DevBuf<float4> d_mu_parts;
-    DevBuf<float> d_majorant, d_s, d_rfwd, d_rinv;
-    DevBuf<float2> d_kn;
+    DevBuf<float> d_majorant, d_rfwd, d_rinv;
+    DevBuf<float2> d_kn, d_kns;
     DevBuf<float4> d_ia_shell;
     DevBuf<float> d_ia_inv, d_fl_cum, d_fl_line;
     DevBuf<double> d_tally;   // one record of dxg::kTallyRecord doubles per voxel, `replicas` copies
@@ -218,10 +218,10 @@ void fillParams(const dxg_world* w, dxg::DevParams& p)
     p.mu_tot = w->d_mu_tot.ptr;
     p.mu_parts = w->d_mu_parts.ptr;
     p.majorant = w->d_majorant.ptr;
-    p.s_table = w->d_s.ptr;
     p.ray_fwd = w->d_rfwd.ptr;
     p.ray_inv = w->d_rinv.ptr;
     p.kn_inv = w->d_kn.ptr;
+    p.kns_inv = w->d_kns.ptr;
     p.ia_shell = w->d_ia_shell.ptr;
     p.ia_inv = w->d_ia_inv.ptr;
     p.fl_cum = w->d_fl_cum.ptr;
@@ -428,15 +428,20 @@ int dxg_world_build(dxg_world* w)
         HIP_TRY(w->d_mu_parts.upload(parts, w->stream));
     }
     HIP_TRY(w->d_majorant.upload(w->tables.majorant, w->stream));
-    HIP_TRY(w->d_s.upload(w->tables.s_table, w->stream));
     HIP_TRY(w->d_rfwd.upload(w->tables.ray_fwd, w->stream));
     HIP_TRY(w->d_rinv.upload(w->tables.ray_inv, w->stream));
     {
-        static_assert(dxg::kKnRowsDev == dxg::kKnRows && dxg::kKnNDev == dxg::kKnN && dxg::kKnInvDEDev * dxg::kKnDE == 1.0,
-                      "Klein-Nishina table shape");
+        static_assert(dxg::kKnRowsDev == dxg::kKnRows && dxg::kKnNDev == dxg::kKnN && dxg::kKnInvDEDev * dxg::kKnDE == 1.0
+                          && dxg::kKnsRowsDev == dxg::kKnsRows && dxg::kKnsPerOctaveDev == dxg::kKnsPerOctave,
+                      "Klein-Nishina table shapes");
         std::vector<float2> kn(w->tables.kn_inv.size() / 2);
         std::memcpy(kn.data(), w->tables.kn_inv.data(), w->tables.kn_inv.size() * sizeof(float));
         HIP_TRY(w->d_kn.upload(kn, w->stream));
+        if (!w->tables.kns_inv.empty()) {
+            std::vector<float2> kns(w->tables.kns_inv.size() / 2);
+            std::memcpy(kns.data(), w->tables.kns_inv.data(), w->tables.kns_inv.size() * sizeof(float));
+            HIP_TRY(w->d_kns.upload(kns, w->stream));
+        }
     }
     if (w->level == 2) {
         std::vector<float4> shells(w->tables.ia_shell.size() / 4);
@@ -1180,6 +1185,32 @@ int dxg_debug_compton_cos(double energy_kev, uint64_t n, const float* u, float* 
         if (!(u[i] >= 0.0f && u[i] < 1.0f))
             return fail(DXG_ERR_ARGUMENT, "dxg_debug_compton_cos: u must lie in [0, 1)");
         cos_out[i] = dxg::knCos(tab, e, u[i]);
+    }
+    return DXG_OK;
+}
+
+int dxg_debug_compton_cos_material(const dxg_material* m, double energy_kev, uint64_t n, const float* u, float* cos_out)
+{
+    if (!m || (!u && n) || (!cos_out && n))
+        return fail(DXG_ERR_ARGUMENT, "dxg_debug_compton_cos_material: null argument");
+    if (!(energy_kev >= dxg::kKnE0 && energy_kev <= dxg::kLutEMax))
+        return fail(DXG_ERR_ARGUMENT, "dxg_debug_compton_cos_material: energy outside 1-200 keV");
+    std::string err;
+    const dxg::PhysicsList* pl = physics(err);
+    if (!pl)
+        return fail(DXG_ERR_PHYSICS, err);
+    dxg::MixMaterial mm;
+    if (!dxg::makeMaterial(*pl, m->Z, m->mass_fraction, m->n, mm, err))
+        return fail(DXG_ERR_MATERIAL, err);
+    std::vector<float> t;
+    dxg::buildComptonSTable(*pl, mm, t);
+    const float2* tab = reinterpret_cast<const float2*>(t.data());
+    const float e = static_cast<float>(energy_kev);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (!(u[i] >= 0.0f && u[i] < 1.0f))
+            return fail(DXG_ERR_ARGUMENT, "dxg_debug_compton_cos_material: u must lie in [0, 1)");
+        cos_out[i] = dxg::knsLookup(tab, std::log2(e) * dxg::kKnsPerOctaveDev,
+                                    (1.0f - std::sqrt(1.0f - u[i])) * static_cast<float>(dxg::kKnNDev - 1));
     }
     return DXG_OK;
 }

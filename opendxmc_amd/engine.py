@@ -767,13 +767,21 @@ def sample_source(world: World, beam, n: int, seed: int = 1) -> np.ndarray:
     return out
 
 
-def compton_cos(energy_kev: float, u) -> np.ndarray:
+def compton_cos(energy_kev: float, u, material: dict[int, float] | None = None) -> np.ndarray:
     """cos(theta) of Compton scatters at energy_kev (1-200 keV) for uniforms u
-    in [0, 1), by the transport kernels' Klein-Nishina table and FP32
-    interpolation, evaluated on the host (no device needed)."""
+    in [0, 1), by the transport kernels' tables and FP32 interpolation,
+    evaluated on the host (no device needed): Klein-Nishina (correction
+    level 0), or with a material composition Klein-Nishina x S(x)/Z of that
+    material (levels 1 and 2)."""
     u = np.ascontiguousarray(u, dtype=np.float32).ravel()
     out = np.zeros_like(u)
-    N.check(N.lib().dxg_debug_compton_cos(float(energy_kev), u.size, N.ptr(u, C.c_float), N.ptr(out, C.c_float)))
+    if material is None:
+        N.check(N.lib().dxg_debug_compton_cos(float(energy_kev), u.size, N.ptr(u, C.c_float), N.ptr(out, C.c_float)))
+    else:
+        k = _Keep()
+        m = Material(material)._struct(k)
+        N.check(N.lib().dxg_debug_compton_cos_material(C.byref(m), float(energy_kev), u.size, N.ptr(u, C.c_float),
+                                                       N.ptr(out, C.c_float)))
     return out
 
 

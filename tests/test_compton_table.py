@@ -7,7 +7,11 @@ The closed form is pinned first against a direct numerical integration of the
 textbook cross-section dsigma/dOmega ~ P^2 (P + 1/P - sin^2 theta), P = E'/E
 (Klein & Nishina 1929), so a dropped term or a wrong sign in either fails.
 The table's bound (5e-5 in probability) is four times tighter than what a
-table row shifted by a quarter keV gives (1.9e-4 at 60 keV)."""
+table row shifted by a quarter keV gives (1.9e-4 at 60 keV).
+
+Correction levels 1 and 2 sample Klein-Nishina x S(x)/Z from one table per
+material (knsLookup, dxg_debug_compton_cos_material), checked against the CDF
+integrated with the oracle's own S(x) (the oracle is the checker here)."""
 from __future__ import annotations
 
 import numpy as np
@@ -109,3 +113,54 @@ def test_rejects_bad_arguments():
         compton_cos(0.5, [0.1])
     with pytest.raises(Exception):
         compton_cos(60.0, [1.0])
+
+
+# ---- correction levels 1 and 2: Klein-Nishina x S(x)/Z per material
+
+def _kns_cdf(oracle, comp, e_kev, n=400001):
+    """CDF of cos(theta) under KN(cos) S(x)/Z, x = E sin(theta/2) / hc, with
+    S(x)/Z from the oracle's own reading of the physics list, integrated by the
+    trapezoid rule on a fine grid."""
+    c = np.linspace(-1.0, 1.0, n)
+    k = e_kev / ME_C2
+    P = 1.0 / (1.0 + k * (1.0 - c))
+    kn = P * P * (P + 1.0 / P - (1.0 - c * c))
+    x = e_kev / 12.398419843320026 * np.sqrt(np.maximum(0.0, 0.5 * (1.0 - c)))
+    s, _, z = oracle.scatter_functions(comp, x)
+    pdf = kn * np.minimum(1.0, s / z)
+    cum = np.concatenate([[0.0], np.cumsum(0.5 * (pdf[1:] + pdf[:-1]) * np.diff(c))])
+    return c, cum / cum[-1]
+
+
+WATER = {1: 0.111894, 8: 0.888106}
+MATERIALS = {"water": WATER, "bone-like CaO": {20: 0.4, 8: 0.6}, "iodine": {53: 1.0}}
+
+
+@pytest.mark.parametrize("name", sorted(MATERIALS))
+def test_material_table_inverts_kn_times_scatter_function(oracle, name):
+    """The kernels' per-material table (rows 12 per octave, nodes denser
+    towards forward scatter) against the CDF integrated from the oracle's
+    S(x): within 1.5e-4 over 1-200 keV, row energies and between them."""
+    comp = MATERIALS[name]
+    u = np.linspace(0.0, 1.0, 8001, endpoint=False).astype(np.float32)
+    worst = 0.0
+    for e in np.append(np.exp(np.linspace(0.0, np.log(200.0), 60)), [11.0, 19.0, 20.0, 61.7]):
+        c = compton_cos(float(e), u, comp).astype(np.float64)
+        assert c[0] == -1.0 and np.all(np.diff(c) >= 0.0) and c.max() <= 1.0
+        cc, cum = _kns_cdf(oracle, comp, float(e))
+        worst = max(worst, np.abs(np.interp(c, cc, cum) - u).max())
+    assert worst < 1.5e-4, worst
+
+
+def test_scatter_function_shapes_the_material_table(oracle):
+    """S(x) suppresses forward scattering, more for low energies and heavy
+    atoms: the table differs from Klein-Nishina alone, and the water table
+    would fail the iodine CDF (a material mix-up is seen)."""
+    u = np.linspace(0.0, 1.0, 4001, endpoint=False).astype(np.float32)
+    e = 30.0
+    kn = compton_cos(e, u).astype(np.float64)
+    w = compton_cos(e, u, WATER).astype(np.float64)
+    i = compton_cos(e, u, {53: 1.0}).astype(np.float64)
+    assert np.mean(w) < np.mean(kn) and np.mean(i) < np.mean(w)
+    cc, cum = _kns_cdf(oracle, {53: 1.0}, e)
+    assert np.abs(np.interp(w, cc, cum) - u).max() > 1e-2
