@@ -35,6 +35,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -97,6 +99,10 @@ __device__ inline bf16x8 asOperand(u32x4 v) { return __builtin_bit_cast(bf16x8, 
 
 __device__ inline u32x4 load16(const uint16_t* p) { return *reinterpret_cast<const u32x4*>(p); }
 
+// One row of 64 voxels x 32 channels of zeros (never written): the
+// convolution's stand-in for input rows and halo voxels outside the volume.
+__device__ __align__(256) uint16_t kZeroRow[64 * 32];
+
 // ---------------------------------------------------------------- kernels
 
 __global__ __launch_bounds__(64) void mfma_probe_kernel(const float* a, const float* b, float* d)
@@ -128,6 +134,7 @@ struct ConvArgs {
     int chunk;               // work-group tiles (4 wave tiles) per work-group
     const int* tiles;        // null: every tile; else the (ascending) tile indices to compute (pair entries: conv3d_pair_kernel)
     int in4;                 // the network's first layer: x is the 4-channel compact input (8 bytes per voxel), conv3d_first_kernel
+    const uint16_t* zrow;    // kZeroRow's device address (set by launchConv)
 };
 
 // Weight fragment (tap, out channel co, 16-B chunk q) lives at chunk
@@ -470,21 +477,49 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
 
     constexpr int kRing = 3;
     constexpr int kRows = 3 * (R + 2);
-    const u32x4 zero = { 0, 0, 0, 0 };
     u32x4 buf[kRing][4], lbuf[kRing], rbuf[kRing];
     bool have[kRing] = { false, false, false };
+    // Input rows are addressed from the tile's own row (voxel x0 of row y,
+    // plane z) by whole-row and whole-plane strides.  Rows outside the volume
+    // (skipped) and the halo voxels left of x = 0 and right of x = nx - 1 are
+    // read from the zero row, so no address is clamped and the shifted
+    // operands need no selects.
+    const ptrdiff_t row_stride = static_cast<ptrdiff_t>(a.nx) * kC, plane_stride = row_stride * a.ny;
+    const uint16_t* zrow = a.zrow;
     auto loadRow = [&](const Tile& tc, int q, int b) {
-        const int zz = tc.z + q / (R + 2) - 1, yy = tc.y + q % (R + 2) - 1;
+        const int dz = q / (R + 2) - 1, dy = q % (R + 2) - 1;
+        const int zz = tc.z + dz, yy = tc.y + dy;
         have[b] = tc.valid && static_cast<unsigned>(zz) < static_cast<unsigned>(a.nz) && static_cast<unsigned>(yy) < static_cast<unsigned>(a.ny);
-        const int zc = min(max(zz, 0), a.nz - 1), yc = min(max(yy, 0), a.ny - 1);
-        const size_t rowb = (static_cast<size_t>(tc.n * a.nz + zc) * a.ny + yc) * a.nx;
-        const uint16_t* base = a.x + (rowb + tc.x0 + 4 * lr) * kC + 8 * lq;
+        const uint16_t* own = a.x + ((static_cast<size_t>(tc.n * a.nz + tc.z) * a.ny + tc.y) * a.nx + tc.x0) * kC;
+        const uint16_t* row = have[b] ? own + dz * plane_stride + dy * row_stride : zrow;
+        const uint16_t* left = have[b] && tc.x0 > 0 ? row - kC : zrow;
+        const uint16_t* right = have[b] && tc.x0 + 64 < a.nx ? row + 64 * kC : zrow;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            buf[b][j] = load16(base + j * kC);
-        lbuf[b] = load16(a.x + (rowb + max(tc.x0 - 1, 0)) * kC + 8 * lq);
-        rbuf[b] = load16(a.x + (rowb + min(tc.x0 + 64, a.nx - 1)) * kC + 8 * lq);
+            buf[b][j] = load16(row + (4 * lr + j) * kC + 8 * lq);
+        lbuf[b] = load16(left + 8 * lq);
+        rbuf[b] = load16(right + 8 * lq);
     };
+    // The weight fragments of a step (one tap, both channel halves) are read
+    // from LDS one step ahead of their MFMAs.  The steps of a tile run over
+    // input rows q, then dx, then the output rows o the tap row dy = q % (R + 2) - o
+    // of which lies in 0..2; the step after the last is the next tile's first
+    // (q 0, dx -1, o 0: tap 0).
+    auto tapOf = [](int q, int dx, int o) { return (q / (R + 2)) * 9 + (q % (R + 2) - o) * 3 + dx + 1; };
+    auto nextTap = [&](int q, int dx, int o) {
+        const int rr = q % (R + 2), o_hi = min(R - 1, rr);
+        if (o < o_hi)
+            return tapOf(q, dx, o + 1);
+        if (dx < 1)
+            return tapOf(q, dx + 1, max(0, rr - 2));
+        if (q + 1 < kRows)
+            return tapOf(q + 1, -1, max(0, (q + 1) % (R + 2) - 2));
+        return 0;
+    };
+    bf16x8 wcur[2];
+#pragma unroll
+    for (int m = 0; m < kHalves; ++m)
+        wcur[m] = asOperand(wl[wSlot(0, 16 * m + lr, lq)]);
 
     int cmask = 0;
     Tile cur = tileAt(begin, cmask);
@@ -531,9 +566,15 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
                 loadRow(cur, q + 2, (q + 2) % kRing);
             else
                 loadRow(nxt, q + 2 - kRows, (q + 2) % kRing);
-            if (!have[b])
+            if (!have[b]) {
+                // a row outside the volume: load the weights of the next row's first step
+                const int tap_next = nextTap(q, 1, R - 1);
+#pragma unroll
+                for (int m = 0; m < kHalves; ++m)
+                    wcur[m] = asOperand(wl[wSlot(tap_next, 16 * m + lr, lq)]);
                 continue;
-            const int dz = q / (R + 2), rr = q % (R + 2);   // input row y + rr - 1 of plane z + dz - 1
+            }
+            const int rr = q % (R + 2);   // input row y + rr - 1 of plane z + q / (R + 2) - 1
             const u32x4(&c)[4] = buf[b];
 #pragma unroll
             for (int dx = -1; dx <= 1; ++dx) {
@@ -543,25 +584,31 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
                     if (dx == 0)
                         bop[j] = asOperand(c[j]);
                     else if (dx < 0)
-                        bop[j] = asOperand(j > 0 ? c[j - 1] : dpp4<kRowShr1>(x0 > 0 ? lbuf[b] : zero, c[3]));   // lane n <- lane n-1
+                        bop[j] = asOperand(j > 0 ? c[j - 1] : dpp4<kRowShr1>(lbuf[b], c[3]));   // lane n <- lane n-1
                     else
-                        bop[j] = asOperand(j < 3 ? c[j + 1] : dpp4<kRowShl1>(x0 + 64 < a.nx ? rbuf[b] : zero, c[0]));   // lane n <- lane n+1
+                        bop[j] = asOperand(j < 3 ? c[j + 1] : dpp4<kRowShl1>(rbuf[b], c[0]));   // lane n <- lane n+1
                 }
 #pragma unroll
                 for (int o = 0; o < R; ++o) {
                     const int dy = rr - o;   // tap row of this input row for output row y + o
                     if (dy < 0 || dy > 2)
                         continue;
-                    const int tap = dz * 9 + dy * 3 + dx + 1;
-                    bf16x8 aop[2];
+                    const int tap_next = nextTap(q, dx, o);
+                    bf16x8 wnext[2];
 #pragma unroll
                     for (int m = 0; m < kHalves; ++m)
-                        aop[m] = asOperand(wl[wSlot(tap, 16 * m + lr, lq)]);
+                        wnext[m] = asOperand(wl[wSlot(tap_next, 16 * m + lr, lq)]);
 #pragma unroll
                     for (int m = 0; m < kHalves; ++m)
 #pragma unroll
                         for (int j = 0; j < 4; ++j)
-                            acc[o][m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop[m], bop[j], acc[o][m][j], 0, 0, 0);
+                            acc[o][m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wcur[m], bop[j], acc[o][m][j], 0, 0, 0);
+#pragma unroll
+                    for (int m = 0; m < kHalves; ++m)
+                        wcur[m] = wnext[m];
+                    // one step per scheduling region: without the fence the
+                    // scheduler hoists a whole row's weight reads and spills
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
         }
@@ -1227,8 +1274,29 @@ int checkDims(const char* fn, uint64_t batch, const uint64_t* dims, Dims& d)
     return DXG_OK;
 }
 
+// kZeroRow's address on the current device (each device holds its own copy).
+const uint16_t* zeroRow()
+{
+    static std::mutex mu;
+    static std::map<int, const uint16_t*> rows;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = rows.find(dev);
+    if (it != rows.end())
+        return it->second;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(kZeroRow)) != hipSuccess)
+        return nullptr;
+    return rows[dev] = static_cast<const uint16_t*>(p);
+}
+
 hipError_t launchConv(ConvArgs a, const Dims& d, hipStream_t s)
 {
+    a.zrow = zeroRow();
+    if (!a.zrow)
+        return hipErrorInvalidSymbol;
     a.nb = d.nb;
     a.nz = d.nz;
     a.ny = d.ny;
