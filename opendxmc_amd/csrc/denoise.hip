@@ -99,9 +99,11 @@ __device__ inline bf16x8 asOperand(u32x4 v) { return __builtin_bit_cast(bf16x8, 
 
 __device__ inline u32x4 load16(const uint16_t* p) { return *reinterpret_cast<const u32x4*>(p); }
 
-// One row of 64 voxels x 32 channels of zeros (never written): the
-// convolution's stand-in for input rows and halo voxels outside the volume.
-__device__ __align__(256) uint16_t kZeroRow[64 * 32];
+// A row of 64 voxels x 32 channels of zeros with one voxel of zeros on each
+// side (never written): the convolution's stand-in for input rows and halo
+// voxels outside the volume.  Kernels get the address of its voxel 0 (one
+// voxel in), so the halo voxels -1 and 64 of the zero row are zeros too.
+__device__ __align__(256) uint16_t kZeroRow[66 * 32];
 
 // ---------------------------------------------------------------- kernels
 
@@ -134,7 +136,7 @@ struct ConvArgs {
     int chunk;               // work-group tiles (4 wave tiles) per work-group
     const int* tiles;        // null: every tile; else the (ascending) tile indices to compute (pair entries: conv3d_pair_kernel)
     int in4;                 // the network's first layer: x is the 4-channel compact input (8 bytes per voxel), conv3d_first_kernel
-    const uint16_t* zrow;    // kZeroRow's device address (set by launchConv)
+    const uint16_t* zrow;    // voxel 0 of kZeroRow on the device (set by launchConv)
 };
 
 // Weight fragment (tap, out channel co, 16-B chunk q) lives at chunk
@@ -173,6 +175,14 @@ constexpr int kConvRows = DXG_CONV_ROWS;
 struct Tile {
     int x0, y, z, n;
     bool valid;
+};
+// A row-pair tile with what its input rows need: the address of voxel x0 of
+// row y, plane z; bit q of `rows` set when input row q lies inside the
+// volume; whether the halo voxels x0 - 1 and x0 + 64 exist.
+struct RowsTile : Tile {
+    const uint16_t* own;
+    uint32_t rows;
+    bool left, right;
 };
 
 template <bool SHIFT>
@@ -452,8 +462,11 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
     const int begin = logical * a.chunk;
     const int end = min(begin + a.chunk, wg_tiles);
     const int nxs = a.nx >> 6, nyp = (a.ny + R - 1) / R;
+    const ptrdiff_t row_stride = static_cast<ptrdiff_t>(a.nx) * kC, plane_stride = row_stride * a.ny;
+    const uint16_t* zrow = a.zrow;
     auto tileAt = [&](int wt, int& mask) {
-        Tile tc { 0, 0, 0, 0, false };
+        RowsTile tc {};
+        tc.own = zrow;
         mask = 0;
         const int k = wt * kConvWaves + wave;
         if (wt < end && k < a.wave_tiles) {
@@ -471,6 +484,21 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
             tc.valid = true;
             if (!a.tiles)
                 mask = (1 << min(R, a.ny - tc.y)) - 1;
+            tc.own = a.x + ((static_cast<size_t>(tc.n * a.nz + tc.z) * a.ny + tc.y) * a.nx + tc.x0) * kC;
+            // bit q: input row y + q % (R + 2) - 1 of plane z + q / (R + 2) - 1 lies inside
+            uint32_t ym = (1u << (R + 2)) - 1;
+            if (tc.y == 0)
+                ym &= ~1u;
+            const int below_top = a.ny - tc.y + 1;
+            if (below_top < R + 2)
+                ym &= (1u << below_top) - 1;
+            tc.rows = ym << (R + 2);
+            if (tc.z > 0)
+                tc.rows |= ym;
+            if (tc.z + 1 < a.nz)
+                tc.rows |= ym << (2 * (R + 2));
+            tc.left = tc.x0 > 0;
+            tc.right = tc.x0 + 64 < a.nx;
         }
         return tc;
     };
@@ -483,20 +511,18 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
     // plane z) by whole-row and whole-plane strides.  Rows outside the volume
     // (skipped) and the halo voxels left of x = 0 and right of x = nx - 1 are
     // read from the zero row, so no address is clamped and the shifted
-    // operands need no selects.
-    const ptrdiff_t row_stride = static_cast<ptrdiff_t>(a.nx) * kC, plane_stride = row_stride * a.ny;
-    const uint16_t* zrow = a.zrow;
-    auto loadRow = [&](const Tile& tc, int q, int b) {
+    // operands need no selects; a skipped row's halo voxels are the zero
+    // row's padding.
+    const uint32_t lane_off = 4 * lr * kC + 8 * lq;
+    auto loadRow = [&](const RowsTile& tc, int q, int b) {
         const int dz = q / (R + 2) - 1, dy = q % (R + 2) - 1;
-        const int zz = tc.z + dz, yy = tc.y + dy;
-        have[b] = tc.valid && static_cast<unsigned>(zz) < static_cast<unsigned>(a.nz) && static_cast<unsigned>(yy) < static_cast<unsigned>(a.ny);
-        const uint16_t* own = a.x + ((static_cast<size_t>(tc.n * a.nz + tc.z) * a.ny + tc.y) * a.nx + tc.x0) * kC;
-        const uint16_t* row = have[b] ? own + dz * plane_stride + dy * row_stride : zrow;
-        const uint16_t* left = have[b] && tc.x0 > 0 ? row - kC : zrow;
-        const uint16_t* right = have[b] && tc.x0 + 64 < a.nx ? row + 64 * kC : zrow;
+        have[b] = (tc.rows >> q) & 1u;
+        const uint16_t* row = have[b] ? tc.own + dz * plane_stride + dy * row_stride : zrow;
+        const uint16_t* left = tc.left ? row - kC : zrow;
+        const uint16_t* right = tc.right ? row + 64 * kC : zrow;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            buf[b][j] = load16(row + (4 * lr + j) * kC + 8 * lq);
+            buf[b][j] = load16(row + lane_off + j * kC);
         lbuf[b] = load16(left + 8 * lq);
         rbuf[b] = load16(right + 8 * lq);
     };
@@ -522,14 +548,14 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
         wcur[m] = asOperand(wl[wSlot(0, 16 * m + lr, lq)]);
 
     int cmask = 0;
-    Tile cur = tileAt(begin, cmask);
+    RowsTile cur = tileAt(begin, cmask);
     if (cur.valid) {
         loadRow(cur, 0, 0);
         loadRow(cur, 1, 1);
     }
     for (int wt = begin; cur.valid; ++wt) {
         int nmask = 0;
-        const Tile nxt = tileAt(wt + 1, nmask);
+        const RowsTile nxt = tileAt(wt + 1, nmask);
         const int x0 = cur.x0, y = cur.y, z = cur.z, n = cur.n;
 
         f32x4 acc[R][2][4];   // [output row][channel half][voxel group]
@@ -1289,7 +1315,7 @@ const uint16_t* zeroRow()
     void* p = nullptr;
     if (hipGetSymbolAddress(&p, HIP_SYMBOL(kZeroRow)) != hipSuccess)
         return nullptr;
-    return rows[dev] = static_cast<const uint16_t*>(p);
+    return rows[dev] = static_cast<const uint16_t*>(p) + kC;
 }
 
 hipError_t launchConv(ConvArgs a, const Dims& d, hipStream_t s)
