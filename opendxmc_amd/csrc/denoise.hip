@@ -171,6 +171,11 @@ constexpr int kConvWaves = DXG_CONV_WAVES;
 #define DXG_CONV_ROWS 2
 #endif
 constexpr int kConvRows = DXG_CONV_ROWS;
+// Input-row buffers of conv3d_rows_kernel: rows load kConvRing - 1 rows ahead of their MFMAs.
+#ifndef DXG_CONV_RING
+#define DXG_CONV_RING 3
+#endif
+constexpr int kConvRing = DXG_CONV_RING;
 // One wavefront tile: 64 voxels of one x row.
 struct Tile {
     int x0, y, z, n;
@@ -429,7 +434,7 @@ __device__ inline bool epiHas(int flag, bool runtime)
 }
 
 template <int R, int EPI>
-__global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu(2))) void conv3d_rows_kernel(ConvArgs a)
+__global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu(kConvWaves / 2))) void conv3d_rows_kernel(ConvArgs a)
 {
     const bool e_relu = epiHas<EPI>(kEpiRelu, a.relu != 0);
     const bool e_res = epiHas<EPI>(kEpiRes, a.res != nullptr);
@@ -503,10 +508,11 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
         return tc;
     };
 
-    constexpr int kRing = 3;
+    constexpr int kRing = kConvRing, kAhead = kRing - 1;
     constexpr int kRows = 3 * (R + 2);
+    static_assert(kRows % kRing == 0, "a tile's rows must fill whole rings");
     u32x4 buf[kRing][4], lbuf[kRing], rbuf[kRing];
-    bool have[kRing] = { false, false, false };
+    bool have[kRing] = {};
     // Input rows are addressed from the tile's own row (voxel x0 of row y,
     // plane z) by whole-row and whole-plane strides.  Rows outside the volume
     // (skipped) and the halo voxels left of x = 0 and right of x = nx - 1 are
@@ -549,10 +555,9 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
 
     int cmask = 0;
     RowsTile cur = tileAt(begin, cmask);
-    if (cur.valid) {
-        loadRow(cur, 0, 0);
-        loadRow(cur, 1, 1);
-    }
+    if (cur.valid)
+        for (int q = 0; q < kAhead; ++q)
+            loadRow(cur, q, q);
     for (int wt = begin; cur.valid; ++wt) {
         int nmask = 0;
         const RowsTile nxt = tileAt(wt + 1, nmask);
@@ -567,7 +572,7 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
         // the epilogue's residual (or gate) values of both rows, loaded before the MFMAs
         const size_t orow0 = (static_cast<size_t>(n * a.nz + z) * a.ny + y) * a.nx;
         // (up to two rows: more would not leave registers for the ring)
-        constexpr bool kPrefetchSide = R <= 2;
+        constexpr bool kPrefetchSide = R <= 2 && kConvWaves <= 4;
         u32x2 side[kPrefetchSide ? R : 1][4][2];
         const uint16_t* side_src = e_res ? a.res : (e_gate ? a.gate : nullptr);
         if (kPrefetchSide && (e_res || e_gate)) {
@@ -588,10 +593,10 @@ __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
         for (int q = 0; q < kRows; ++q) {
             const int b = q % kRing;
-            if (q + 2 < kRows)
-                loadRow(cur, q + 2, (q + 2) % kRing);
+            if (q + kAhead < kRows)
+                loadRow(cur, q + kAhead, (q + kAhead) % kRing);
             else
-                loadRow(nxt, q + 2 - kRows, (q + 2) % kRing);
+                loadRow(nxt, q + kAhead - kRows, (q + kAhead) % kRing);
             if (!have[b]) {
                 // a row outside the volume: load the weights of the next row's first step
                 const int tap_next = nextTap(q, 1, R - 1);
